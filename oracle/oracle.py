@@ -1,0 +1,232 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes loader for the C oracle (fvt_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, as the checker (or the timed CPU baseline).  The product package never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+FLUX = {"ac": 0, "hll": 1, "rusanov": 2, "bgnvc": 3}
+GRAD = {"lsq": 0, "gauss": 1}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class OrcCase(C.Structure):
+    _fields_ = [("n_pts", C.c_int32), ("n_elems", C.c_int32), ("n_edges", C.c_int32),
+                ("x", _dp), ("y", _dp), ("tri", _ip), ("edge_p", _ip), ("edge_e", _ip),
+                ("z_at_vtx", C.c_int32), ("h_at_vtx", C.c_int32), ("u_at_vtx", C.c_int32),
+                ("v_at_vtx", C.c_int32), ("cd_at_vtx", C.c_int32),
+                ("z", _dp), ("h", _dp), ("u", _dp), ("v", _dp), ("cd", _dp),
+                ("n_in", C.c_int32), ("in_type", _ip), ("in_ptr", _ip), ("in_nodes", _ip),
+                ("qts_ptr", _ip), ("qts_t", _dp), ("qts_v", _dp),
+                ("n_out", C.c_int32), ("out_type", _ip), ("out_ptr", _ip), ("out_nodes", _ip),
+                ("hts_ptr", _ip), ("hts_t", _dp), ("hts_v", _dp)]
+
+
+class OrcOpts(C.Structure):
+    _fields_ = [("rkorder", C.c_int32), ("dt_op", C.c_int32), ("steady", C.c_int32),
+                ("stop_cr", C.c_int32), ("istop", C.c_int32), ("btype", C.c_int32),
+                ("hvalve", C.c_int32), ("limiter", C.c_int32), ("flux", C.c_int32),
+                ("gradient", C.c_int32), ("friction", C.c_int32), ("vclip", C.c_int32),
+                ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
+                ("h_wet", C.c_double), ("vceiling", C.c_double)]
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "fvt_oracle.c")
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcOpts), C.c_char_p, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_run.restype = C.c_int
+        L.orc_run.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp]
+        L.orc_finalize.argtypes = [C.c_void_p]
+        L.orc_get_state.argtypes = [C.c_void_p] + [_dp] * 7 + [_ip, _dp, _dp]
+        L.orc_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_flux.argtypes = [C.c_int32] + [C.c_double] * 11 + [_dp]
+        L.orc_equals.restype = C.c_int32
+        L.orc_equals.argtypes = [C.c_double, C.c_double]
+        L.orc_cell_grads.argtypes = [C.c_void_p, _dp]
+        L.orc_dt_cfl.restype = C.c_double
+        L.orc_dt_cfl.argtypes = [C.c_void_p]
+        L.orc_topology.restype = C.c_int64
+        L.orc_topology.argtypes = [C.c_void_p, _ip, _ip, _ip]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def opts_struct(o: Dict[str, object]) -> OrcOpts:
+    h_dry = float(o["h_dry"])
+    return OrcOpts(rkorder=int(o["rkorder"]), dt_op=int(o["dt_op"]), steady=int(bool(o["steady"])),
+                   stop_cr=int(o["stop_cr"]), istop=int(o["istop"]), btype=int(o["btype"]),
+                   hvalve=int(bool(o["hvalve"])), limiter=int(o["limiter"]),
+                   flux=FLUX[str(o["flux"])], gradient=GRAD[str(o["gradient"])],
+                   friction=int(o["friction"]), vclip=int(bool(o["vclip"])),
+                   cfl=float(o["cfl"]), rstop=float(o["rstop"]), h_dry=h_dry,
+                   h_wet=max(1.0, float(o["c_threshold"])) * h_dry,
+                   vceiling=float(o["vceiling"]))
+
+
+class Oracle:
+    """The CPU oracle on one case (cell-centred h, u, v, n; bed at vertices)."""
+
+    def __init__(self, case, opts: Optional[Dict[str, object]] = None):
+        o = dict(case.opts)
+        if opts:
+            o.update(opts)
+        self.opts = o
+        self.case = case
+        k = self._keep = {}
+        k["x"] = np.ascontiguousarray(case.x, dtype=np.float64)
+        k["y"] = np.ascontiguousarray(case.y, dtype=np.float64)
+        k["tri"] = np.ascontiguousarray(case.tri, dtype=np.int32)
+        k["ep"] = np.ascontiguousarray(case.edge_p, dtype=np.int32)
+        k["ee"] = np.ascontiguousarray(case.edge_e, dtype=np.int32)
+        for nm in ("zvtx", "h", "u", "v", "cd"):
+            k[nm] = np.ascontiguousarray(getattr(case, nm), dtype=np.float64)
+
+        def csr(bs, field):
+            ptr = np.zeros(len(bs) + 1, dtype=np.int32)
+            for i, b in enumerate(bs):
+                ptr[i + 1] = ptr[i] + len(getattr(b, field))
+            return ptr
+        ins, outs = case.inflows, case.outflows
+        k["in_type"] = np.array([b.kind for b in ins] or [0], dtype=np.int32)
+        k["in_ptr"] = csr(ins, "nodes")
+        k["in_nodes"] = np.concatenate([b.nodes for b in ins] or [np.zeros(1)]).astype(np.int32)
+        k["qts_ptr"] = csr(ins, "t")
+        k["qts_t"] = np.concatenate([b.t for b in ins] or [np.zeros(1)]).astype(np.float64)
+        k["qts_v"] = np.concatenate([b.v for b in ins] or [np.zeros(1)]).astype(np.float64)
+        k["out_type"] = np.array([b.kind for b in outs] or [0], dtype=np.int32)
+        k["out_ptr"] = csr(outs, "nodes")
+        k["out_nodes"] = np.concatenate([b.nodes for b in outs] or [np.zeros(1)]).astype(np.int32)
+        k["hts_ptr"] = csr(outs, "t")
+        k["hts_t"] = np.concatenate([b.t for b in outs] or [np.zeros(1)]).astype(np.float64)
+        k["hts_v"] = np.concatenate([b.v for b in outs] or [np.zeros(1)]).astype(np.float64)
+        cs = OrcCase(n_pts=case.n_pts, n_elems=case.n_elems, n_edges=case.n_edges,
+                     x=_p(k["x"], C.c_double), y=_p(k["y"], C.c_double),
+                     tri=_p(k["tri"], C.c_int32), edge_p=_p(k["ep"], C.c_int32),
+                     edge_e=_p(k["ee"], C.c_int32), z_at_vtx=1, h_at_vtx=0, u_at_vtx=0,
+                     v_at_vtx=0, cd_at_vtx=0, z=_p(k["zvtx"], C.c_double),
+                     h=_p(k["h"], C.c_double), u=_p(k["u"], C.c_double),
+                     v=_p(k["v"], C.c_double), cd=_p(k["cd"], C.c_double),
+                     n_in=len(ins), in_type=_p(k["in_type"], C.c_int32),
+                     in_ptr=_p(k["in_ptr"], C.c_int32), in_nodes=_p(k["in_nodes"], C.c_int32),
+                     qts_ptr=_p(k["qts_ptr"], C.c_int32), qts_t=_p(k["qts_t"], C.c_double),
+                     qts_v=_p(k["qts_v"], C.c_double),
+                     n_out=len(outs), out_type=_p(k["out_type"], C.c_int32),
+                     out_ptr=_p(k["out_ptr"], C.c_int32), out_nodes=_p(k["out_nodes"], C.c_int32),
+                     hts_ptr=_p(k["hts_ptr"], C.c_int32), hts_t=_p(k["hts_t"], C.c_double),
+                     hts_v=_p(k["hts_v"], C.c_double))
+        err = C.create_string_buffer(256)
+        self._opts_s = opts_struct(o)
+        self.h = lib().orc_create(C.byref(cs), C.byref(self._opts_s), err, 256)
+        if not self.h:
+            raise RuntimeError("oracle: " + err.value.decode())
+        self.N, self.P = case.n_elems, case.n_pts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def run(self, nsteps: int):
+        it = C.c_int32(0)
+        tt = C.c_double(0)
+        rm = C.c_double(0)
+        n = lib().orc_run(self.h, int(nsteps), C.byref(it), C.byref(tt), C.byref(rm))
+        return dict(taken=n, iter=it.value, ttime=tt.value, rmax=rm.value)
+
+    def finalize(self):
+        lib().orc_finalize(self.h)
+
+    def state(self) -> Dict[str, np.ndarray]:
+        N, P = self.N, self.P
+        out = {k: np.zeros(N) for k in ("h", "u", "v", "qh", "qu", "qv")}
+        phi = np.zeros(3 * N)
+        dry = np.zeros(N, dtype=np.int32)
+        hv = np.zeros(P)
+        zv = np.zeros(P)
+        lib().orc_get_state(self.h, *[_p(out[k], C.c_double) for k in ("h", "u", "v", "qh", "qu", "qv")],
+                            _p(phi, C.c_double), _p(dry, C.c_int32), _p(hv, C.c_double), _p(zv, C.c_double))
+        out.update(phi1=phi[:N], phi2=phi[N:2 * N], phi3=phi[2 * N:], dry=dry, hvtx=hv, zetavtx=zv)
+        return out
+
+    def set_state(self, h, u, v):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lib().orc_set_state(self.h, _p(h, C.c_double), _p(u, C.c_double), _p(v, C.c_double))
+
+    def cell_grads(self) -> np.ndarray:
+        g = np.zeros(6 * self.N)
+        lib().orc_cell_grads(self.h, _p(g, C.c_double))
+        return g.reshape(self.N, 6)
+
+    def dt_cfl(self) -> float:
+        return float(lib().orc_dt_cfl(self.h))
+
+    def topology(self):
+        n = lib().orc_topology(self.h, None, None, None)
+        ptr = np.zeros(self.N + 1, dtype=np.int32)
+        lst = np.zeros(max(n, 1), dtype=np.int32)
+        t3 = np.zeros(3 * self.N, dtype=np.int32)
+        lib().orc_topology(self.h, _p(ptr, C.c_int32), _p(lst, C.c_int32), _p(t3, C.c_int32))
+        return ptr, lst[:n], t3.reshape(self.N, 3)
+
+
+def flux(kind: str, nx, ny, hl, ul, vl, hr, ur, vr, cl=0.0, cr=0.0, h0=1e-3):
+    f = np.zeros(3)
+    lib().orc_flux(FLUX[kind], nx, ny, hl, ul, vl, hr, ur, vr, cl, cr, h0, _p(f, C.c_double))
+    return f
+
+
+def equals(a: float, b: float) -> bool:
+    return bool(lib().orc_equals(a, b))
+
+
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+
+def run_reference(case, workdir: str, timeout: float = 600.0):
+    """Run the flang-built reference on ``case`` (written as a .run case under
+    ``workdir``).  Needs oracle/_ref/ref_driver (built by ``make -C oracle``
+    from the reference sources, in this container only)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from storm_amd import cases as _cases
+    run = _cases.write_run(case, workdir)
+    out = os.path.join(workdir, "ref_out.bin")
+    with open(os.path.join(workdir, "ref_log.txt"), "w") as log:
+        subprocess.run([REF_DRIVER, os.path.basename(run), "ref_out.bin"], cwd=workdir,
+                       stdout=log, stderr=subprocess.STDOUT, check=True, timeout=timeout)
+    return _cases.read_ref_output(out)

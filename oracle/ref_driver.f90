@@ -1,0 +1,57 @@
+! Test infrastructure only: a driver that runs the reference SToRM FVT path
+! (compiled from the reference's own Fortran sources by oracle/Makefile) on a
+! ".run" case and dumps the final cell state in raw little-endian binary.
+!
+! It follows the reference's own main-program sequence for a ".run" case,
+! Source_xsy-64bit/SToRM.f90:64-66 -> read_data.f90:80-83 (read_mfa) ->
+! solver.f90:50,57 (dataprepFVT, solverFVT), without the command-line prompt
+! of read_data.f90:41-52 and without the final Tecplot/CGNS writers.
+!
+! Usage: ref_driver <case.run> <out.bin>
+!
+! Output layout (stream access, native endianness):
+!   int32  n_elems, n_pts, iter, rkorder
+!   real64 ttime, rmax
+!   real64 h(N), u(N), v(N)                 after the final to_vertices call
+!   real64 rkh(N), rku(N), rkv(N)           the RK slot rkorder (q^{n+1})
+!   real64 phi(N,1), phi(N,2), phi(N,3)
+!   int32  drycell(N)                       1 = dry
+!   real64 hvtx(P), zetavtx(P)
+PROGRAM ref_driver
+  USE parameters
+  USE geometry
+  USE dep_vars
+  USE options
+  USE io
+  USE RKparams
+  IMPLICIT NONE
+  CHARACTER (LEN=250) :: fname, oname
+  CHARACTER (LEN=160) :: title
+  INTEGER :: iter, i, u_out
+  INTEGER, ALLOCATABLE :: dryi(:)
+  REAL (KIND=mp) :: ttime, rmax
+
+  CALL GET_COMMAND_ARGUMENT(1, fname)
+  CALL GET_COMMAND_ARGUMENT(2, oname)
+  cgns = .FALSE.
+  CALL read_mfa(fname, title)
+  CALL dataprepFVT
+  CALL solverFVT(iter, ttime, rmax, fname, title)
+
+  ALLOCATE(dryi(n_elems))
+  DO i = 1, n_elems
+    dryi(i) = 0
+    IF (drycell(i)) dryi(i) = 1
+  END DO
+  OPEN (NEWUNIT=u_out, FILE=oname, ACCESS='STREAM', FORM='UNFORMATTED', &
+        STATUS='REPLACE')
+  WRITE (u_out) n_elems, n_pts, iter, rkorder
+  WRITE (u_out) ttime, rmax
+  WRITE (u_out) h(1:n_elems), u(1:n_elems), v(1:n_elems)
+  WRITE (u_out) rkh(1:n_elems, rkorder), rku(1:n_elems, rkorder), &
+                rkv(1:n_elems, rkorder)
+  WRITE (u_out) phi(1:n_elems, 1), phi(1:n_elems, 2), phi(1:n_elems, 3)
+  WRITE (u_out) dryi(1:n_elems)
+  WRITE (u_out) hvtx(1:n_pts), zetavtx(1:n_pts)
+  CLOSE (u_out)
+END PROGRAM ref_driver
