@@ -1,0 +1,142 @@
+/*
+ * storm_fvt.h -- C ABI of the MI355X-native SToRM FVT time step.
+ *
+ * The library replaces the reference's FVT data preparation and time loop:
+ *
+ *   storm_fvt_create          Source_xsy-64bit/dataprepFVT.f90:1-342 (called
+ *                             at solver.f90:50), plus the mesh tables the
+ *                             reference builds in find_edges/t_areas/walls/
+ *                             ibc_arrays/node_db/elemnt_db (read_mfa.f90:161-173)
+ *   storm_fvt_set_state       the initial h,u,v of dataprepFVT.f90:90-130
+ *   storm_fvt_set_bc_values   the Q / stage values of get_time_bcs.f90:31-77
+ *   storm_fvt_run             the time loop solverFVT.f90:129-478 (called at
+ *                             solver.f90:57), nsteps at a time
+ *   storm_fvt_finalize        the post-loop to_vertices of solverFVT.f90:483
+ *   storm_fvt_get_state,
+ *   storm_fvt_get_conserved,
+ *   storm_fvt_get_vertex      what solverFVT leaves in the dep_vars / RKparams
+ *                             modules (h,u,v,phi,drycell,rk*(:,rkorder),hvtx,
+ *                             zetavtx; dep_vars.f90:16-57, RKparams.f90:19)
+ *   storm_fvt_last_error,
+ *   storm_fvt_destroy         the error/shutdown role of byebye.f90:29-215
+ *
+ * Conventions: plain pointers and sizes, 0-based indices, -1 = none, FP64.
+ * Every int-returning call returns 0 on success or a negative STORM_E_* code
+ * with a message from storm_fvt_last_error(); no C++ exception crosses the
+ * ABI.  Host arrays stay owned by the caller (copied in and out); the context
+ * owns its device buffers and one HIP stream.  One context per host thread.
+ */
+#ifndef STORM_FVT_H
+#define STORM_FVT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STORM_FVT_ABI_VERSION 1
+
+enum {
+  STORM_OK = 0,
+  STORM_E_ARG = -1,       /* invalid argument / option */
+  STORM_E_MESH = -2,      /* inconsistent mesh connectivity or boundary data */
+  STORM_E_DEVICE = -3,    /* HIP runtime error or no device */
+  STORM_E_NOMEM = -4,     /* allocation failure */
+  STORM_E_STATE = -5      /* call out of sequence */
+};
+
+/* Interior-edge numerical flux (the call at invisc_fluxes.f90:137 and the
+ * commented alternatives at :138,143-145). */
+enum { STORM_FLUX_AC = 0, STORM_FLUX_HLL = 1, STORM_FLUX_RUSANOV = 2, STORM_FLUX_BGNVC = 3 };
+/* Cell gradients (cell_grads.f90:46-54). */
+enum { STORM_GRAD_LSQ = 0, STORM_GRAD_GAUSS = 1 };
+
+/* Mesh and initial data: the contents of the reference's geometry/dep_vars/
+ * vbc_arrays modules after read_mfa (tecplot_read.f90, ioflow.f90). */
+typedef struct storm_mesh {
+  int32_t n_pts, n_elems, n_edges;   /* n_edges = 0: edges are built here
+                                        (find_edges.f90:47-54 order) */
+  const double *x, *y;               /* [n_pts] node coordinates */
+  const int32_t *tri;                /* [3*n_elems] counter-clockwise vertex ids */
+  const int32_t *edge_p;             /* [2*n_edges] edges%p (or NULL) */
+  const int32_t *edge_e;             /* [2*n_edges] edges%e, -1 = none (or NULL) */
+  /* datav flags (dataprepFVT.f90:73-140): 1 = given at vertices, 0 = cells */
+  int32_t z_at_vtx, h_at_vtx, u_at_vtx, v_at_vtx, cd_at_vtx;
+  const double *z, *h, *u, *v, *cd;  /* bed, depth, velocity, Manning n */
+  /* inflow strings (ioflow.f90:184-253): vtype, node chains, Q(t) tables */
+  int32_t n_in;
+  const int32_t *in_type, *in_ptr, *in_nodes, *qts_ptr;
+  const double *qts_t, *qts_v;
+  /* outflow strings (ioflow.f90:255-312): htype, node chains, H(t) tables */
+  int32_t n_out;
+  const int32_t *out_type, *out_ptr, *out_nodes, *hts_ptr;
+  const double *hts_t, *hts_v;
+} storm_mesh_t;
+
+/* Options (options.f90; defaults of initialize_options.f90:23-141). */
+typedef struct storm_fvt_opts {
+  int32_t rkorder;   /* RK ORDER: 1, 2, 3 */
+  int32_t dt_op;     /* DELT TYPE: 0 fixed dt = cfl; 1 CFL reduction (the
+                        corrected deltatFVT: cell area and perimeter) */
+  int32_t steady;    /* TIME STEPPING 0: steady, local time steps */
+  int32_t stop_cr;   /* STOPPING CRIT: 0 max|phi|, 1 RMS change of h */
+  int32_t istop;     /* MAX ITERATION */
+  int32_t btype;     /* WALLS: 0 no-slip (wenslp3), 1 slip (weslp2) */
+  int32_t hvalve;    /* HVALVE */
+  int32_t limiter;   /* LIMITER: 0 zero gradients, 1 unlimited, 2 Barth-Jespersen */
+  int32_t flux;      /* STORM_FLUX_* */
+  int32_t gradient;  /* STORM_GRAD_* */
+  int32_t friction;  /* 0 implicit Manning (fric_terms_impl), 1 explicit (fric_terms) */
+  int32_t vclip;     /* MAX VELOCITY given */
+  double cfl, rstop, h_dry, h_wet, vceiling;
+  /* MI355X execution options (no reference counterpart) */
+  int32_t reorder;   /* 1 = Reverse Cuthill-McKee cell order (default), 0 = input order */
+  int32_t block;     /* threads per block for the cell kernels, 0 = tuned default */
+  int32_t graph;     /* 1 = replay captured HIP graphs of whole steps */
+  int32_t reserved;
+} storm_fvt_opts_t;
+
+typedef struct storm_fvt storm_fvt_t;
+
+/* Fills *o with the reference defaults. */
+void storm_fvt_default_opts(storm_fvt_opts_t *o);
+
+int storm_fvt_create(const storm_mesh_t *mesh, const storm_fvt_opts_t *opts, storm_fvt_t **ctx);
+/* Cell-centred h, u, v; restarts the time loop (iter = 0). */
+int storm_fvt_set_state(storm_fvt_t *ctx, const double *h, const double *u, const double *v);
+/* Replace every string's table by one constant: qin[n_in] (Q, or stage for
+ * vtype 4) and hbc[n_out] (stage of htype 1 strings).  NULL keeps a side. */
+int storm_fvt_set_bc_values(storm_fvt_t *ctx, const double *qin, const double *hbc);
+/* Advance up to nsteps time steps (fewer when istop is reached or a steady
+ * run converges).  iter/ttime/rmax as solverFVT's dummy arguments. */
+int storm_fvt_run(storm_fvt_t *ctx, int32_t nsteps, int32_t *iter, double *ttime, double *rmax);
+/* The to_vertices call that ends solverFVT (zeroes u,v in partially wet
+ * cells and refreshes the vertex depths). */
+int storm_fvt_finalize(storm_fvt_t *ctx);
+/* Cell state in input order; any pointer may be NULL.  phi: [3*n_elems],
+ * component-major (phi(:,1), phi(:,2), phi(:,3)). */
+int storm_fvt_get_state(storm_fvt_t *ctx, double *h, double *u, double *v, double *phi,
+                        uint8_t *drycell);
+/* rkh, rku, rkv(:,rkorder): h, hu, hv after the last step. */
+int storm_fvt_get_conserved(storm_fvt_t *ctx, double *qh, double *qu, double *qv);
+/* Vertex depth and water-surface elevation of the last to_vertices. */
+int storm_fvt_get_vertex(storm_fvt_t *ctx, double *hvtx, double *zetavtx);
+/* Device pointers of the cell state in the library's internal (reordered)
+ * order and the permutation perm[internal] = input id, for callers that keep
+ * data resident on the GPU. */
+int storm_fvt_device_info(storm_fvt_t *ctx, int32_t *n_elems, int32_t *perm, void **stream);
+/* Seconds of device time of the last storm_fvt_run (HIP events on the
+ * context's stream). */
+double storm_fvt_last_run_seconds(storm_fvt_t *ctx);
+const char *storm_fvt_last_error(storm_fvt_t *ctx);
+void storm_fvt_destroy(storm_fvt_t *ctx);
+
+/* Host-only inspection of the set-up (no device needed): the least-squares
+ * molecule of every cell (elemnt_db.f90:42-128) as CSR in input numbering.
+ * Returns the number of molecule entries or a negative error code. */
+int64_t storm_fvt_inspect_molecule(const storm_mesh_t *mesh, int32_t *ptr, int32_t *cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,670 @@
+// Host-side set-up of the FVT step (see fvt_prep.h).
+//
+// Every coefficient is formed with the same operations, in the same order,
+// as the reference set-up routine cited beside it, so that the device step
+// reproduces the reference arithmetic.  Topology searches that the reference
+// does by linear scans (is_edge.f90, in_array*.f90) use hash maps here; their
+// results are identical.
+#include "fvt_prep.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <queue>
+#include <unordered_map>
+
+namespace storm {
+
+namespace {
+
+constexpr double HALF = 0.5;
+constexpr double ONE = 1.0;
+constexpr double ZERO = 0.0;
+const double ONE_THIRD = 1.0 / 3.0;
+
+inline uint64_t pair_key(int a, int b) {
+  uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+  return ((uint64_t)lo << 32) | hi;
+}
+
+// cde_t2t.f90:30-47 (inner sweep trip count fixed at sweep start).
+void cde_t2t(int* row, int& n) {
+  int l = 1;
+  for (;;) {
+    int j = row[l];
+    const int mend = n;
+    for (int m = l + 1; m <= mend; ++m) {
+      int k = row[m];
+      if (j == k) {
+        n = n - 1;
+        for (int nn = m; nn <= n; ++nn) row[nn] = row[nn + 1];
+      }
+    }
+    l = l + 1;
+    if (l > n) break;
+  }
+}
+
+// elemnt_db.f90:42-128 on the edge table (ee: e(1), e(2) per edge).
+int molecule(int N, int E, const std::vector<int>& ee, std::vector<int>& ptr,
+             std::vector<int>& cells, std::string& err) {
+  const int MAXD = 20, W = MAXD + 2;
+  if (N < 4) { err = "too few triangles in computational domain (< 4)"; return STORM_E_MESH; }
+  std::vector<int> w((size_t)N * W, 0);
+  auto row = [&](int i) { return &w[(size_t)i * W]; };
+  for (int i = 0; i < E; ++i) {
+    int e1 = ee[2 * i], e2 = ee[2 * i + 1];
+    if (e1 < 0 || e2 < 0) continue;
+    int* r1 = row(e1); int* r2 = row(e2);
+    if (r1[0] >= 3 || r2[0] >= 3) { err = "triangle connectivity table incorrect"; return STORM_E_MESH; }
+    r1[++r1[0]] = e2;
+    r2[++r2[0]] = e1;
+  }
+  for (int i = 0; i < N; ++i) {
+    int* ri = row(i);
+    const int n = ri[0];
+    if (n == 3) continue;
+    if (n < 1 || n > 3) { err = "triangle connectivity table incorrect"; return STORM_E_MESH; }
+    for (int l = 1; l <= n; ++l) {
+      int j = ri[l];
+      const int* rj = row(j);
+      const int mm = std::min(3, rj[0]);
+      for (int m = 1; m <= mm; ++m) {
+        int k = rj[m];
+        if (k == i) continue;
+        if (ri[0] + 1 > MAXD) { err = "molecule overflow"; return STORM_E_MESH; }
+        ri[++ri[0]] = k;
+      }
+    }
+    int c = ri[0];
+    cde_t2t(ri, c);
+    ri[0] = c;
+  }
+  for (int i = 0; i < N; ++i) {
+    int* ri = row(i);
+    const int n = ri[0];
+    if (n >= 3) continue;
+    int j = ri[n];
+    const int* rj = row(j);
+    const int mm = rj[0];
+    for (int m = 1; m <= mm; ++m) {
+      int k = rj[m];
+      if (k == i) continue;
+      if (ri[0] + 1 > MAXD) { err = "molecule overflow"; return STORM_E_MESH; }
+      ri[++ri[0]] = k;
+    }
+    int c = ri[0];
+    cde_t2t(ri, c);
+    ri[0] = c;
+  }
+  ptr.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) ptr[i + 1] = ptr[i] + row(i)[0];
+  cells.resize(ptr[N]);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < row(i)[0]; ++k) cells[ptr[i] + k] = row(i)[k + 1];
+  return STORM_OK;
+}
+
+// Edge table: given, with the swap rule of tecplot_read.f90:139-142, or built.
+int edge_table(const storm_mesh_t* m, std::vector<int>& ep, std::vector<int>& ee,
+               std::string& err) {
+  if (m->n_edges > 0) {
+    if (!m->edge_p || !m->edge_e) { err = "n_edges > 0 but edge_p/edge_e missing"; return STORM_E_ARG; }
+    ep.assign(m->edge_p, m->edge_p + 2 * (size_t)m->n_edges);
+    ee.assign(m->edge_e, m->edge_e + 2 * (size_t)m->n_edges);
+    for (int k = 0; k < m->n_edges; ++k)
+      if (ee[2 * k] < 0) { ee[2 * k] = ee[2 * k + 1]; ee[2 * k + 1] = -1; }
+    for (int k = 0; k < m->n_edges; ++k) {
+      if (ep[2 * k] < 0 || ep[2 * k] >= m->n_pts || ep[2 * k + 1] < 0 || ep[2 * k + 1] >= m->n_pts ||
+          ee[2 * k] < 0 || ee[2 * k] >= m->n_elems || ee[2 * k + 1] >= m->n_elems) {
+        err = "edge table entry out of range"; return STORM_E_MESH;
+      }
+    }
+  } else {
+    build_edges(m->n_elems, m->n_pts, m->tri, ep, ee);
+  }
+  return STORM_OK;
+}
+
+// Reverse Cuthill-McKee over the cell adjacency (edge neighbours), each
+// connected component started from a pseudo-peripheral cell.
+std::vector<int> rcm_order(int N, const std::vector<int>& nb3) {
+  std::vector<int> deg(N, 0);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) deg[i] += nb3[3 * i + j] >= 0;
+  std::vector<int> order;
+  order.reserve(N);
+  std::vector<int> level(N, -1);
+  std::vector<char> placed(N, 0);
+  auto bfs_far = [&](int s, std::vector<int>& touched) {
+    std::queue<int> q;
+    q.push(s);
+    level[s] = 0;
+    touched.push_back(s);
+    int far = s;
+    while (!q.empty()) {
+      int c = q.front(); q.pop();
+      if (level[c] > level[far] || (level[c] == level[far] && deg[c] < deg[far])) far = c;
+      for (int j = 0; j < 3; ++j) {
+        int n = nb3[3 * c + j];
+        if (n >= 0 && level[n] < 0) { level[n] = level[c] + 1; touched.push_back(n); q.push(n); }
+      }
+    }
+    return far;
+  };
+  for (int seed = 0; seed < N; ++seed) {
+    if (placed[seed]) continue;
+    std::vector<int> touched;
+    int s = seed;
+    for (int it = 0; it < 2; ++it) {
+      int f = bfs_far(s, touched);
+      for (int t : touched) level[t] = -1;
+      touched.clear();
+      s = f;
+    }
+    std::vector<int> comp;
+    std::queue<int> q;
+    q.push(s);
+    placed[s] = 1;
+    while (!q.empty()) {
+      int c = q.front(); q.pop();
+      comp.push_back(c);
+      int nbs[3], k = 0;
+      for (int j = 0; j < 3; ++j) {
+        int n = nb3[3 * c + j];
+        if (n >= 0 && !placed[n]) nbs[k++] = n;
+      }
+      std::sort(nbs, nbs + k, [&](int a, int b) { return deg[a] < deg[b] || (deg[a] == deg[b] && a < b); });
+      for (int t = 0; t < k; ++t) { placed[nbs[t]] = 1; q.push(nbs[t]); }
+    }
+    std::reverse(comp.begin(), comp.end());
+    order.insert(order.end(), comp.begin(), comp.end());
+  }
+  return order;
+}
+
+}  // namespace
+
+void build_edges(int N, int P, const int32_t* tri, std::vector<int>& ep, std::vector<int>& ee) {
+  (void)P;
+  std::unordered_map<uint64_t, int> idx;
+  idx.reserve((size_t)N * 2);
+  ep.clear();
+  ee.clear();
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int a = tri[3 * i + j], b = tri[3 * i + (j + 1) % 3];
+      auto it = idx.find(pair_key(a, b));
+      if (it == idx.end()) {
+        idx.emplace(pair_key(a, b), (int)(ep.size() / 2));
+        ep.push_back(a); ep.push_back(b);
+        ee.push_back(i); ee.push_back(-1);
+      } else {
+        ee[2 * it->second + 1] = i;
+      }
+    }
+}
+
+int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int>& cells,
+                   std::string& err) {
+  std::vector<int> ep, ee;
+  int rc = edge_table(m, ep, ee, err);
+  if (rc) return rc;
+  return molecule(m->n_elems, (int)(ep.size() / 2), ee, ptr, cells, err);
+}
+
+int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
+  if (!m || !o) { S.err = "null mesh or options"; return STORM_E_ARG; }
+  if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
+  if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 1 || o->limiter < 0 ||
+      o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
+      o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 1) {
+    S.err = "invalid option value"; return STORM_E_ARG;
+  }
+  if (!(o->h_dry > 0) || !(o->h_wet >= 0) || !(o->cfl > 0)) { S.err = "invalid threshold or cfl"; return STORM_E_ARG; }
+  if (m->n_pts < 3 || m->n_elems < 1 || !m->x || !m->y || !m->tri || !m->z || !m->h || !m->u ||
+      !m->v || !m->cd) {
+    S.err = "incomplete mesh description"; return STORM_E_ARG;
+  }
+  if (m->n_in < 0 || m->n_out < 0 || m->n_in > MAX_STRINGS || m->n_out > MAX_STRINGS) {
+    S.err = "too many boundary strings (max 64 each)"; return STORM_E_ARG;
+  }
+  const int N = m->n_elems, P = m->n_pts;
+  for (int t = 0; t < 3 * N; ++t)
+    if (m->tri[t] < 0 || m->tri[t] >= P) { S.err = "triangle vertex id out of range"; return STORM_E_MESH; }
+  S.o = *o;
+  S.N = N; S.P = P; S.R = o->rkorder;
+  std::vector<int> ep, ee;
+  int rcode = edge_table(m, ep, ee, S.err);
+  if (rcode) return rcode;
+  const int E = (int)(ep.size() / 2);
+  S.E = E;
+  const double* x = m->x;
+  const double* y = m->y;
+  const int32_t* tri = m->tri;
+
+  // find_edges.f90:74-114: signed local edges
+  std::vector<int> gedge(3 * (size_t)N, -1), gsign(3 * (size_t)N, 0);
+  for (int j = 0; j < E; ++j)
+    for (int kk = 0; kk < 2; ++kk) {
+      int i = ee[2 * j + kk];
+      if (i < 0) continue;
+      for (int l = 0; l < 3; ++l) {
+        int a = tri[3 * i + l], b = tri[3 * i + (l + 1) % 3];
+        if (a == ep[2 * j] && b == ep[2 * j + 1]) { gedge[3 * i + l] = j; gsign[3 * i + l] = 1; break; }
+        if (a == ep[2 * j + 1] && b == ep[2 * j]) { gedge[3 * i + l] = j; gsign[3 * i + l] = -1; break; }
+      }
+    }
+  for (size_t t = 0; t < gedge.size(); ++t)
+    if (gedge[t] < 0) { S.err = "edge data structure incomplete"; return STORM_E_MESH; }
+  // edge_normals.f90:30-39 and find_edges.f90:159-163
+  std::vector<double> nrm(2 * (size_t)E), tng(2 * (size_t)E), len(E);
+  for (int k = 0; k < E; ++k) {
+    double x1 = x[ep[2 * k]], y1 = y[ep[2 * k]], x2 = x[ep[2 * k + 1]], y2 = y[ep[2 * k + 1]];
+    nrm[2 * k] = y2 - y1; nrm[2 * k + 1] = x1 - x2;
+    tng[2 * k] = x2 - x1; tng[2 * k + 1] = y2 - y1;
+    double dx = x1 - x2, dy = y1 - y2;
+    len[k] = std::sqrt(dx * dx + dy * dy);
+  }
+  // t_areas.f90:33-52
+  std::vector<double> area(N), perim(N), xc(N), yc(N);
+  for (int i = 0; i < N; ++i) {
+    int a = tri[3 * i], b = tri[3 * i + 1], d = tri[3 * i + 2];
+    double x1 = x[a], y1 = y[a], x2 = x[b], y2 = y[b], x3 = x[d], y3 = y[d];
+    area[i] = HALF * std::fabs((x1 - x2) * (y1 + y2) + (x2 - x3) * (y2 + y3) + (x3 - x1) * (y3 + y1));
+    xc[i] = ONE_THIRD * (x1 + x2 + x3);
+    yc[i] = ONE_THIRD * (y1 + y2 + y3);
+    perim[i] = len[gedge[3 * i]] + len[gedge[3 * i + 1]] + len[gedge[3 * i + 2]];
+    if (!(area[i] > 0)) { S.err = "degenerate triangle (zero area)"; return STORM_E_MESH; }
+  }
+  // boundary strings (ioflow.f90) and walls.f90:40-148
+  S.n_in = m->n_in; S.n_out = m->n_out;
+  std::vector<char> isbc(P, 0);
+  const int nin_nodes = m->n_in ? m->in_ptr[m->n_in] : 0;
+  const int nout_nodes = m->n_out ? m->out_ptr[m->n_out] : 0;
+  for (int t = 0; t < nin_nodes; ++t) {
+    if (m->in_nodes[t] < 0 || m->in_nodes[t] >= P) { S.err = "inflow node out of range"; return STORM_E_MESH; }
+    isbc[m->in_nodes[t]] = 1;
+  }
+  for (int t = 0; t < nout_nodes; ++t) {
+    if (m->out_nodes[t] < 0 || m->out_nodes[t] >= P) { S.err = "outflow node out of range"; return STORM_E_MESH; }
+    isbc[m->out_nodes[t]] = 1;
+  }
+  std::vector<int> cnt(E, 0);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) cnt[gedge[3 * i + j]]++;
+  std::vector<char> wall(E, 0);
+  int nbpoly = 0, nwall1 = 0;
+  for (int i = 0; i < E; ++i) {
+    if (cnt[i] == 1) {
+      nbpoly++;
+      if (!(isbc[ep[2 * i]] && isbc[ep[2 * i + 1]])) { wall[i] = 1; nwall1++; }
+    } else if (cnt[i] != 2) {
+      S.err = "computational grid construction error, in edge assignment"; return STORM_E_MESH;
+    }
+  }
+  std::unordered_map<uint64_t, int> emap;
+  emap.reserve((size_t)E * 2);
+  for (int k = E - 1; k >= 0; --k) emap[pair_key(ep[2 * k], ep[2 * k + 1])] = k;  // first match wins
+  // ibc_arrays.f90:59-181,377-466: string edges, bcedges, qbar, hbar
+  // edge -> (routine kind, string, position)
+  std::vector<int> e_kind(E, -1), e_str(E, -1), e_pos(E, -1);
+  std::vector<char> hbar_e(E, 0), qbar_e(E, 0);
+  int nchain_edges = 0;
+  S.in_type.assign(m->in_type, m->in_type + m->n_in);
+  S.out_type.assign(m->out_type, m->out_type + m->n_out);
+  auto chain_edges = [&](const int32_t* nodes, int a, int b, std::vector<int>& out) -> int {
+    out.clear();
+    for (int t = a; t + 1 < b; ++t) {
+      auto it = emap.find(pair_key(nodes[t], nodes[t + 1]));
+      if (it == emap.end()) {
+        S.err = "boundary string nodes " + std::to_string(nodes[t]) + " " + std::to_string(nodes[t + 1]) +
+                " are not connected by an edge";
+        return STORM_E_MESH;
+      }
+      out.push_back(it->second);
+    }
+    return STORM_OK;
+  };
+  std::vector<int> ce;
+  std::vector<std::vector<int>> in_edges(m->n_in), out_edges(m->n_out);
+  for (int ch = 0; ch < m->n_in; ++ch) {
+    int vt = m->in_type[ch];
+    if (m->in_ptr[ch + 1] - m->in_ptr[ch] < 2) { S.err = "ERROR: n_qin <= 1."; return STORM_E_MESH; }
+    if (chain_edges(m->in_nodes, m->in_ptr[ch], m->in_ptr[ch + 1], ce)) return STORM_E_MESH;
+    in_edges[ch] = ce;
+    nchain_edges += (int)ce.size();
+    int kind = (vt == 1) ? SK_IN_Q1 : (vt == 5) ? SK_IN_Q5 : (vt == 4) ? SK_IN_H : SK_NONE;
+    if (vt < 0 || vt > 5) { S.err = "invalid value in vtype"; return STORM_E_ARG; }
+    if (vt == 2) { S.err = "VTYPE = 2: option not implemented (enforce_bcs.f90:151-155)"; return STORM_E_ARG; }
+    for (size_t p = 0; p < ce.size(); ++p) {
+      if (cnt[ce[p]] != 1) { S.err = "inflow string runs through the interior of the mesh"; return STORM_E_MESH; }
+      qbar_e[ce[p]] = 1;
+      if (e_kind[ce[p]] < 0) { e_kind[ce[p]] = kind; e_str[ce[p]] = ch; e_pos[ce[p]] = (int)p; }
+    }
+  }
+  for (int ch = 0; ch < m->n_out; ++ch) {
+    int ht = m->out_type[ch];
+    if (m->out_ptr[ch + 1] - m->out_ptr[ch] < 2) { S.err = "ERROR: n_hbc <= 1."; return STORM_E_MESH; }
+    if (ht < 0 || ht > 2) { S.err = "invalid value in htype"; return STORM_E_ARG; }
+    if (chain_edges(m->out_nodes, m->out_ptr[ch], m->out_ptr[ch + 1], ce)) return STORM_E_MESH;
+    out_edges[ch] = ce;
+    nchain_edges += (int)ce.size();
+    int kind = (ht == 1) ? SK_OUT_H : (ht == 2) ? SK_OUT_CRIT : SK_OUT_FREE;
+    for (size_t p = 0; p < ce.size(); ++p) {
+      if (cnt[ce[p]] != 1) { S.err = "outflow string runs through the interior of the mesh"; return STORM_E_MESH; }
+      hbar_e[ce[p]] = 1;
+      if (e_kind[ce[p]] < 0) { e_kind[ce[p]] = kind; e_str[ce[p]] = ch; e_pos[ce[p]] = (int)p; }
+    }
+  }
+  if (nbpoly - nwall1 != nchain_edges) {
+    S.err = "number of boundary condition edges does not match number of boundary condition chains";
+    return STORM_E_MESH;
+  }
+  for (int k = 0; k < E; ++k)
+    if (cnt[k] == 1 && !wall[k] && e_kind[k] < 0) {
+      S.err = "wall edges and inflow/outflow edges do not add up"; return STORM_E_MESH;
+    }
+  // time tables
+  S.qts_ptr.assign(m->n_in + 1, 0);
+  S.hts_ptr.assign(m->n_out + 1, 0);
+  if (m->n_in) {
+    S.qts_ptr.assign(m->qts_ptr, m->qts_ptr + m->n_in + 1);
+    S.qts_t.assign(m->qts_t, m->qts_t + S.qts_ptr[m->n_in]);
+    S.qts_v.assign(m->qts_v, m->qts_v + S.qts_ptr[m->n_in]);
+  }
+  if (m->n_out) {
+    S.hts_ptr.assign(m->hts_ptr, m->hts_ptr + m->n_out + 1);
+    S.hts_t.assign(m->hts_t, m->hts_t + S.hts_ptr[m->n_out]);
+    S.hts_v.assign(m->hts_v, m->hts_v + S.hts_ptr[m->n_out]);
+  }
+  // node_db.f90:60-68: vertex -> cells, ascending cell ids
+  std::vector<int> n2t_ptr(P + 1, 0), n2t;
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) n2t_ptr[tri[3 * i + j] + 1]++;
+  for (int p = 0; p < P; ++p) n2t_ptr[p + 1] += n2t_ptr[p];
+  n2t.resize(n2t_ptr[P]);
+  {
+    std::vector<int> fill(P, 0);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int p = tri[3 * i + j];
+        n2t[n2t_ptr[p] + fill[p]++] = i;
+      }
+  }
+  // elemnt_db.f90:42-128 (molecule) and :226-233 (edge neighbours)
+  std::vector<int> t2t_ptr, t2t;
+  rcode = molecule(N, E, ee, t2t_ptr, t2t, S.err);
+  if (rcode) return rcode;
+  std::vector<int> t2t3(3 * (size_t)N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int k = gedge[3 * i + j];
+      int e1 = ee[2 * k];
+      if (e1 == i) e1 = ee[2 * k + 1];
+      t2t3[3 * i + j] = e1;
+    }
+  // c2v_setup.f90:106-119 (type 3)
+  std::vector<double> c2vw(n2t.size()), c2vwid(n2t.size());
+  for (int i = 0; i < P; ++i) {
+    double tdist = ZERO;
+    for (int j = n2t_ptr[i]; j < n2t_ptr[i + 1]; ++j) {
+      int k = n2t[j];
+      double xk = xc[k] - x[i], yk = yc[k] - y[i];
+      c2vw[j] = ONE / std::sqrt(xk * xk + yk * yk);
+      c2vwid[j] = c2vw[j];
+      tdist = tdist + c2vw[j];
+    }
+    for (int j = n2t_ptr[i]; j < n2t_ptr[i + 1]; ++j) c2vw[j] = c2vw[j] / tdist;
+  }
+  auto vtx2ctr = [&](const double* va, std::vector<double>& ca) {  // vtx2ctr.f90:32-35
+    ca.resize(N);
+    for (int i = 0; i < N; ++i)
+      ca[i] = (va[tri[3 * i]] + va[tri[3 * i + 1]] + va[tri[3 * i + 2]]) * ONE_THIRD;
+  };
+  auto ctr2vtx = [&](const double* ca, std::vector<double>& va) {  // ctr2vtx.f90:73-80
+    va.resize(P);
+    for (int i = 0; i < P; ++i) {
+      double a = ZERO;
+      for (int j = n2t_ptr[i]; j < n2t_ptr[i + 1]; ++j) a = a + c2vw[j] * ca[n2t[j]];
+      va[i] = a;
+    }
+  };
+  // dataprepFVT.f90:73-140
+  std::vector<double> z, zvtx, h, u, v, cd, tmp;
+  if (m->z_at_vtx) { vtx2ctr(m->z, z); zvtx.assign(m->z, m->z + P); }
+  else { ctr2vtx(m->z, zvtx); vtx2ctr(zvtx.data(), z); }
+  if (m->h_at_vtx) vtx2ctr(m->h, h); else h.assign(m->h, m->h + N);
+  if (m->u_at_vtx) vtx2ctr(m->u, u); else u.assign(m->u, m->u + N);
+  if (m->v_at_vtx) vtx2ctr(m->v, v); else v.assign(m->v, m->v + N);
+  if (m->cd_at_vtx) vtx2ctr(m->cd, cd); else cd.assign(m->cd, m->cd + N);
+  // dataprepFVT.f90:164-181: rc vectors; unit normals and tangents
+  std::vector<double> rcx(3 * (size_t)N), rcy(3 * (size_t)N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int k = gedge[3 * i + j];
+      double xm = HALF * (x[ep[2 * k]] + x[ep[2 * k + 1]]);
+      double ym = HALF * (y[ep[2 * k]] + y[ep[2 * k + 1]]);
+      rcx[3 * i + j] = xm - xc[i];
+      rcy[3 * i + j] = ym - yc[i];
+    }
+  for (int k = 0; k < E; ++k) {
+    nrm[2 * k] = nrm[2 * k] / len[k]; nrm[2 * k + 1] = nrm[2 * k + 1] / len[k];
+    tng[2 * k] = tng[2 * k] / len[k]; tng[2 * k + 1] = tng[2 * k + 1] / len[k];
+  }
+  // lsq_setup.f90:105-145 (weighted, type 2)
+  std::vector<double> mdx(t2t.size()), mdy(t2t.size()), mw(t2t.size()), ai(N), bi(N), ci(N), det(N);
+  for (int i = 0; i < N; ++i) {
+    double a = ZERO, b = ZERO, c = ZERO;
+    for (int t = t2t_ptr[i]; t < t2t_ptr[i + 1]; ++t) {
+      int l = t2t[t];
+      mdx[t] = xc[l] - xc[i];
+      mdy[t] = yc[l] - yc[i];
+      mw[t] = ONE / (mdx[t] * mdx[t] + mdy[t] * mdy[t]);
+    }
+    for (int t = t2t_ptr[i]; t < t2t_ptr[i + 1]; ++t) {
+      a = a + mw[t] * mdx[t] * mdx[t];
+      b = b + mw[t] * mdx[t] * mdy[t];
+      c = c + mw[t] * mdy[t] * mdy[t];
+    }
+    ai[i] = a; bi[i] = b; ci[i] = c;
+    det[i] = a * c - b * b;
+  }
+  // Green-Gauss gradient coefficients on the triangle of neighbour centroids
+  // (gaussgrad_setup.f90:33-81 and ghost_cell.f90:34-41, corrected: the
+  // neighbour's centroid, t2t3(j,i), outward normals of the auxiliary triangle)
+  std::vector<int> gconn(3 * (size_t)N);
+  std::vector<double> gcx(3 * (size_t)N), gcy(3 * (size_t)N), gwt(3 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    double xp[3], yp[3];
+    for (int j = 0; j < 3; ++j) {
+      int k = t2t3[3 * i + j];
+      if (k >= 0) { xp[j] = xc[k]; yp[j] = yc[k]; gconn[3 * i + j] = k; }
+      else {
+        int l = gedge[3 * i + j];
+        double x1 = x[ep[2 * l]], y1 = y[ep[2 * l]], x2 = x[ep[2 * l + 1]], y2 = y[ep[2 * l + 1]];
+        double ax = xc[i] - x1, ay = yc[i] - y1, bx = x2 - x1, by = y2 - y1;
+        double den = bx * bx + by * by;
+        double cx = (bx * (ax * bx + ay * by) + by * (ay * bx - ax * by)) / den;
+        double cy = (bx * (ax * by - ay * bx) + by * (ax * bx + ay * by)) / den;
+        xp[j] = cx + x1; yp[j] = cy + y1; gconn[3 * i + j] = i;
+      }
+    }
+    double ar = HALF * ((xp[1] - xp[0]) * (yp[2] - yp[0]) - (xp[2] - xp[0]) * (yp[1] - yp[0]));
+    double n1x = yp[1] - yp[0], n1y = xp[0] - xp[1];
+    double n2x = yp[2] - yp[1], n2y = xp[1] - xp[2];
+    double n3x = yp[0] - yp[2], n3y = xp[2] - xp[0];
+    gcx[3 * i] = HALF * (n1x + n3x) / ar; gcx[3 * i + 1] = HALF * (n1x + n2x) / ar; gcx[3 * i + 2] = HALF * (n2x + n3x) / ar;
+    gcy[3 * i] = HALF * (n1y + n3y) / ar; gcy[3 * i + 1] = HALF * (n1y + n2y) / ar; gcy[3 * i + 2] = HALF * (n2y + n3y) / ar;
+    double den = (xp[1] - xp[0]) * (yp[2] - yp[1]) - (xp[2] - xp[1]) * (yp[1] - yp[0]);
+    gwt[3 * i] = ((xp[1] - xc[i]) * (yp[2] - yp[1]) - (xp[2] - xp[1]) * (yp[1] - yc[i])) / den;
+    gwt[3 * i + 1] = ((xp[2] - xc[i]) * (yp[0] - yp[2]) - (xp[0] - xp[2]) * (yp[2] - yc[i])) / den;
+    gwt[3 * i + 2] = ((xp[0] - xc[i]) * (yp[1] - yp[0]) - (xp[1] - xp[0]) * (yp[0] - yc[i])) / den;
+  }
+  // dataprepFVT.f90:278-286
+  S.bch_max = ZERO;
+  for (int k = 0; k < E; ++k)
+    if (qbar_e[k] && cnt[k] == 1) {
+      S.bch_max = std::max(S.bch_max, zvtx[ep[2 * k]]);
+      S.bch_max = std::max(S.bch_max, zvtx[ep[2 * k + 1]]);
+    }
+  // dataprepFVT.f90:293-306: highest vertex; bed_slope_VTX.f90:22-38
+  std::vector<double> zhigh(N), dzx(N), dzy(N);
+  for (int i = 0; i < N; ++i) {
+    int p[3] = {tri[3 * i], tri[3 * i + 1], tri[3 * i + 2]};
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k)
+        if (zvtx[p[k]] > zvtx[p[k + 1]]) std::swap(p[k], p[k + 1]);
+    zhigh[i] = zvtx[p[2]];
+    int a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
+    double x1 = x[a], y1 = y[a], x2 = x[b], y2 = y[b], x3 = x[c], y3 = y[c];
+    double z1 = zvtx[a], z2 = zvtx[b], z3 = zvtx[c];
+    dzx[i] = HALF * (z1 * (y2 - y3) + z2 * (y3 - y1) + z3 * (y1 - y2)) / area[i];
+    dzy[i] = HALF * (z1 * (x3 - x2) + z2 * (x1 - x3) + z3 * (x2 - x1)) / area[i];
+  }
+
+  // ---- cell ordering --------------------------------------------------
+  std::vector<int> order;
+  if (o->reorder) order = rcm_order(N, t2t3);
+  else { order.resize(N); std::iota(order.begin(), order.end(), 0); }
+  S.perm = order;
+  S.iperm.assign(N, 0);
+  for (int i = 0; i < N; ++i) S.iperm[order[i]] = i;
+  // vertices in order of first touch by the ordered cells
+  S.vperm.clear(); S.viperm.assign(P, -1);
+  for (int ii = 0; ii < N; ++ii) {
+    int i = order[ii];
+    for (int j = 0; j < 3; ++j) {
+      int p = tri[3 * i + j];
+      if (S.viperm[p] < 0) { S.viperm[p] = (int)S.vperm.size(); S.vperm.push_back(p); }
+    }
+  }
+  for (int p = 0; p < P; ++p)
+    if (S.viperm[p] < 0) { S.viperm[p] = (int)S.vperm.size(); S.vperm.push_back(p); }
+  const std::vector<int>& ip = S.iperm;
+  const std::vector<int>& vip = S.viperm;
+  auto N3 = 3 * (size_t)N;
+  S.cvtx.assign(N3, 0); S.cnbr.assign(N3, -1); S.ep1.assign(N3, 0); S.ep2.assign(N3, 0);
+  S.cinfo.assign(N3, 0); S.corder.assign(N, 0);
+  S.rc.assign(6 * (size_t)N, 0);
+  S.enx.assign(N3, 0); S.eny.assign(N3, 0); S.etx.assign(N3, 0); S.ety.assign(N3, 0);
+  S.elen.assign(N3, 0); S.ezmid.assign(N3, 0);
+  S.area.resize(N); S.perim.resize(N); S.z.resize(N); S.cd.resize(N); S.delzx.resize(N);
+  S.delzy.resize(N); S.zhigh.resize(N);
+  S.vdx.assign(N3, 0); S.vdy.assign(N3, 0); S.zv3.assign(N3, 0);
+  S.gconn.assign(N3, 0); S.gcx.assign(N3, 0); S.gcy.assign(N3, 0); S.gwt.assign(N3, 0);
+  S.mptr.assign(N + 1, 0);
+  S.ai.resize(N); S.bi.resize(N); S.ci.resize(N); S.det.resize(N);
+  S.h0.resize(N); S.u0.resize(N); S.v0.resize(N);
+  for (int ii = 0; ii < N; ++ii) S.mptr[ii + 1] = S.mptr[ii] + (t2t_ptr[order[ii] + 1] - t2t_ptr[order[ii]]);
+  S.mcell.resize(S.mptr[N]); S.mdx.resize(S.mptr[N]); S.mdy.resize(S.mptr[N]); S.mw.resize(S.mptr[N]);
+  for (int ii = 0; ii < N; ++ii) {
+    const int i = order[ii];
+    S.area[ii] = area[i]; S.perim[ii] = perim[i]; S.z[ii] = z[i]; S.cd[ii] = cd[i];
+    S.delzx[ii] = dzx[i]; S.delzy[ii] = dzy[i]; S.zhigh[ii] = zhigh[i];
+    S.ai[ii] = ai[i]; S.bi[ii] = bi[i]; S.ci[ii] = ci[i]; S.det[ii] = det[i];
+    S.h0[ii] = h[i]; S.u0[ii] = u[i]; S.v0[ii] = v[i];
+    for (int t = 0; t < t2t_ptr[i + 1] - t2t_ptr[i]; ++t) {
+      int src = t2t_ptr[i] + t, dst = S.mptr[ii] + t;
+      S.mcell[dst] = ip[t2t[src]]; S.mdx[dst] = mdx[src]; S.mdy[dst] = mdy[src]; S.mw[dst] = mw[src];
+    }
+    uint64_t keys[3];
+    for (int j = 0; j < 3; ++j) {
+      const size_t s = (size_t)j * N + ii;
+      const int k = gedge[3 * i + j];
+      const int vtxid = tri[3 * i + j];
+      S.cvtx[s] = vip[vtxid];
+      S.vdx[s] = x[vtxid] - xc[i];
+      S.vdy[s] = y[vtxid] - yc[i];
+      S.zv3[s] = zvtx[vtxid];
+      S.rc[(size_t)(2 * j) * N + ii] = rcx[3 * i + j];
+      S.rc[(size_t)(2 * j + 1) * N + ii] = rcy[3 * i + j];
+      S.gconn[s] = ip[gconn[3 * i + j]];
+      S.gcx[s] = gcx[3 * i + j]; S.gcy[s] = gcy[3 * i + j]; S.gwt[s] = gwt[3 * i + j];
+      S.ep1[s] = vip[ep[2 * k]]; S.ep2[s] = vip[ep[2 * k + 1]];
+      S.elen[s] = len[k];
+      S.ezmid[s] = HALF * (zvtx[ep[2 * k]] + zvtx[ep[2 * k + 1]]);
+      // the normal/tangent the reference uses for edge k: that of edges(k)
+      // flipped when e(1) holds it with negative orientation
+      const int e1 = ee[2 * k];
+      int m1 = -1;
+      for (int l = 0; l < 3; ++l) if (gedge[3 * e1 + l] == k) { m1 = l; break; }
+      const double sg = (gsign[3 * e1 + m1] < 0) ? -1.0 : 1.0;
+      S.enx[s] = sg < 0 ? -nrm[2 * k] : nrm[2 * k];
+      S.eny[s] = sg < 0 ? -nrm[2 * k + 1] : nrm[2 * k + 1];
+      S.etx[s] = sg < 0 ? -tng[2 * k] : tng[2 * k];
+      S.ety[s] = sg < 0 ? -tng[2 * k + 1] : tng[2 * k + 1];
+      uint32_t info = 0;
+      if (cnt[k] == 2) {
+        int nb = t2t3[3 * i + j];
+        S.cnbr[s] = ip[nb];
+        int nl = -1;
+        for (int l = 0; l < 3; ++l) if (gedge[3 * nb + l] == k) { nl = l; break; }
+        info = SK_INTERIOR | ((uint32_t)nl << CI_NBRLOC_SHIFT);
+        if (e1 == i) info |= CI_LEFT;
+        keys[j] = ((uint64_t)0 << 60) | (uint64_t)k;
+      } else if (wall[k]) {
+        info = SK_WALL | CI_LEFT;
+        keys[j] = ((uint64_t)1 << 60) | (uint64_t)k;
+      } else {
+        uint32_t kind = (uint32_t)e_kind[k];
+        // reference routine order: inflowQbyH (vtype 1 strings, then 5),
+        // inflowH, outflowbyH, crit_outflow, free_flow
+        uint64_t rank = (kind == SK_IN_Q1) ? 0 : (kind == SK_IN_Q5) ? 1 : (kind == SK_IN_H) ? 2
+                      : (kind == SK_OUT_H) ? 3 : (kind == SK_OUT_CRIT) ? 4 : (kind == SK_OUT_FREE) ? 5 : 6;
+        info = kind | CI_LEFT | ((uint32_t)e_str[k] << CI_CHAIN_SHIFT);
+        keys[j] = ((uint64_t)2 << 60) | (rank << 52) | ((uint64_t)e_str[k] << 32) | (uint64_t)e_pos[k];
+      }
+      if (hbar_e[k] && cnt[k] == 1) info |= CI_HBAR;
+      S.cinfo[s] = info;
+    }
+    int ord[3] = {0, 1, 2};
+    std::sort(ord, ord + 3, [&](int a, int b) { return keys[a] < keys[b]; });
+    S.corder[ii] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4));
+  }
+  // vertices: inverse-distance weights over the cells in input order
+  S.vptr.assign(P + 1, 0);
+  for (int pp = 0; pp < P; ++pp) {
+    int p = S.vperm[pp];
+    S.vptr[pp + 1] = S.vptr[pp] + (n2t_ptr[p + 1] - n2t_ptr[p]);
+  }
+  S.vcell.resize(S.vptr[P]); S.vw.resize(S.vptr[P]); S.zvtx.resize(P); S.vbc.assign(P, -1);
+  S.xv_int.resize(P); S.yv_int.resize(P);
+  for (int pp = 0; pp < P; ++pp) {
+    int p = S.vperm[pp];
+    S.zvtx[pp] = zvtx[p];
+    S.xv_int[pp] = x[p];
+    S.yv_int[pp] = y[p];
+    for (int t = 0; t < n2t_ptr[p + 1] - n2t_ptr[p]; ++t) {
+      S.vcell[S.vptr[pp] + t] = ip[n2t[n2t_ptr[p] + t]];
+      S.vw[S.vptr[pp] + t] = c2vwid[n2t_ptr[p] + t];
+    }
+  }
+  // enforce_bcs.f90:40-56: nodes of stage strings (last string wins, as the
+  // reference loops over strings in order)
+  for (int ch = 0; ch < m->n_out; ++ch) {
+    if (m->out_type[ch] != 1) continue;
+    for (int t = m->out_ptr[ch]; t < m->out_ptr[ch + 1]; ++t) S.vbc[vip[m->out_nodes[t]]] = ch;
+  }
+  // inflowQbyH strings: (cell, slot) in string order
+  S.qs_ptr.assign(m->n_in + 1, 0);
+  S.qs_cell.clear(); S.qs_slot.clear();
+  for (int ch = 0; ch < m->n_in; ++ch) {
+    int vt = m->in_type[ch];
+    if (vt == 1 || vt == 5) {
+      for (int k : in_edges[ch]) {
+        int i = ee[2 * k];
+        int l = -1;
+        for (int t = 0; t < 3; ++t) if (gedge[3 * i + t] == k) { l = t; break; }
+        S.qs_cell.push_back(ip[i]);
+        S.qs_slot.push_back(l);
+      }
+    }
+    S.qs_ptr[ch + 1] = (int)S.qs_cell.size();
+  }
+  S.last_cell = ip[N - 1];
+  return STORM_OK;
+}
+
+}  // namespace storm

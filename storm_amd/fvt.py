@@ -1,0 +1,289 @@
+"""Python host side of the MI355X FVT step: a ctypes binding of
+libstorm_fvt.so (include/storm_fvt.h) shaped like the reference's call
+surface.
+
+The reference drives the FVT path with two module-state subroutines,
+``CALL dataprepFVT`` and ``CALL solverFVT(iter, ttime, rmax, fn, title)``
+(Source_xsy-64bit/solver.f90:50,57).  :class:`FVTSolver` keeps that shape:
+``dataprepFVT`` builds the device context from a case (mesh, bed, initial
+state, boundary strings, options) and ``solverFVT`` runs the time loop to
+MAX ITERATION (or convergence) and ends with the reference's final
+``to_vertices``.  ``run`` advances a given number of steps without that
+final pass.  Errors raise :class:`StormError` (the role of byebye.f90).
+
+There is no CPU fallback: if the library or a HIP device is missing, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstorm_fvt.so")
+
+FLUX = {"ac": 0, "hll": 1, "rusanov": 2, "bgnvc": 3}
+GRAD = {"lsq": 0, "gauss": 1}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class StormError(RuntimeError):
+    pass
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_pts", C.c_int32), ("n_elems", C.c_int32), ("n_edges", C.c_int32),
+                ("x", _dp), ("y", _dp), ("tri", _ip), ("edge_p", _ip), ("edge_e", _ip),
+                ("z_at_vtx", C.c_int32), ("h_at_vtx", C.c_int32), ("u_at_vtx", C.c_int32),
+                ("v_at_vtx", C.c_int32), ("cd_at_vtx", C.c_int32),
+                ("z", _dp), ("h", _dp), ("u", _dp), ("v", _dp), ("cd", _dp),
+                ("n_in", C.c_int32), ("in_type", _ip), ("in_ptr", _ip), ("in_nodes", _ip),
+                ("qts_ptr", _ip), ("qts_t", _dp), ("qts_v", _dp),
+                ("n_out", C.c_int32), ("out_type", _ip), ("out_ptr", _ip), ("out_nodes", _ip),
+                ("hts_ptr", _ip), ("hts_t", _dp), ("hts_v", _dp)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("rkorder", C.c_int32), ("dt_op", C.c_int32), ("steady", C.c_int32),
+                ("stop_cr", C.c_int32), ("istop", C.c_int32), ("btype", C.c_int32),
+                ("hvalve", C.c_int32), ("limiter", C.c_int32), ("flux", C.c_int32),
+                ("gradient", C.c_int32), ("friction", C.c_int32), ("vclip", C.c_int32),
+                ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
+                ("h_wet", C.c_double), ("vceiling", C.c_double),
+                ("reorder", C.c_int32), ("block", C.c_int32), ("graph", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_set_state",
+           "storm_fvt_set_bc_values", "storm_fvt_run", "storm_fvt_finalize",
+           "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
+           "storm_fvt_device_info", "storm_fvt_last_run_seconds", "storm_fvt_last_error",
+           "storm_fvt_destroy", "storm_fvt_inspect_molecule"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libstorm_fvt.so; raises StormError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise StormError(f"{path} is not built (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    L.storm_fvt_default_opts.argtypes = [C.POINTER(Opts)]
+    L.storm_fvt_create.restype = C.c_int
+    L.storm_fvt_create.argtypes = [C.POINTER(Mesh), C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.storm_fvt_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.storm_fvt_set_bc_values.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_run.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
+    L.storm_fvt_finalize.argtypes = [C.c_void_p]
+    L.storm_fvt_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8)]
+    L.storm_fvt_get_conserved.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.storm_fvt_get_vertex.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
+    L.storm_fvt_last_run_seconds.restype = C.c_double
+    L.storm_fvt_last_run_seconds.argtypes = [C.c_void_p]
+    L.storm_fvt_last_error.restype = C.c_char_p
+    L.storm_fvt_last_error.argtypes = [C.c_void_p]
+    L.storm_fvt_destroy.argtypes = [C.c_void_p]
+    L.storm_fvt_inspect_molecule.restype = C.c_int64
+    L.storm_fvt_inspect_molecule.argtypes = [C.POINTER(Mesh), _ip, _ip]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _csr(bs, field):
+    ptr = np.zeros(len(bs) + 1, dtype=np.int32)
+    for i, b in enumerate(bs):
+        ptr[i + 1] = ptr[i] + len(getattr(b, field))
+    return ptr
+
+
+def mesh_struct(case, keep: dict) -> Mesh:
+    """C-ABI mesh of a :class:`storm_amd.cases.Case` (arrays kept alive in
+    ``keep``)."""
+    k = keep
+    k["x"] = np.ascontiguousarray(case.x, dtype=np.float64)
+    k["y"] = np.ascontiguousarray(case.y, dtype=np.float64)
+    k["tri"] = np.ascontiguousarray(case.tri, dtype=np.int32)
+    k["ep"] = np.ascontiguousarray(case.edge_p, dtype=np.int32)
+    k["ee"] = np.ascontiguousarray(case.edge_e, dtype=np.int32)
+    for nm in ("zvtx", "h", "u", "v", "cd"):
+        k[nm] = np.ascontiguousarray(getattr(case, nm), dtype=np.float64)
+    ins, outs = case.inflows, case.outflows
+    z1 = np.zeros(1)
+    k["in_type"] = np.array([b.kind for b in ins] or [0], dtype=np.int32)
+    k["in_ptr"] = _csr(ins, "nodes")
+    k["in_nodes"] = np.concatenate([b.nodes for b in ins] or [z1]).astype(np.int32)
+    k["qts_ptr"] = _csr(ins, "t")
+    k["qts_t"] = np.concatenate([b.t for b in ins] or [z1]).astype(np.float64)
+    k["qts_v"] = np.concatenate([b.v for b in ins] or [z1]).astype(np.float64)
+    k["out_type"] = np.array([b.kind for b in outs] or [0], dtype=np.int32)
+    k["out_ptr"] = _csr(outs, "nodes")
+    k["out_nodes"] = np.concatenate([b.nodes for b in outs] or [z1]).astype(np.int32)
+    k["hts_ptr"] = _csr(outs, "t")
+    k["hts_t"] = np.concatenate([b.t for b in outs] or [z1]).astype(np.float64)
+    k["hts_v"] = np.concatenate([b.v for b in outs] or [z1]).astype(np.float64)
+    d, i = C.c_double, C.c_int32
+    return Mesh(n_pts=case.n_pts, n_elems=case.n_elems, n_edges=case.n_edges,
+                x=_p(k["x"], d), y=_p(k["y"], d), tri=_p(k["tri"], i),
+                edge_p=_p(k["ep"], i), edge_e=_p(k["ee"], i),
+                z_at_vtx=1, h_at_vtx=0, u_at_vtx=0, v_at_vtx=0, cd_at_vtx=0,
+                z=_p(k["zvtx"], d), h=_p(k["h"], d), u=_p(k["u"], d), v=_p(k["v"], d),
+                cd=_p(k["cd"], d), n_in=len(ins), in_type=_p(k["in_type"], i),
+                in_ptr=_p(k["in_ptr"], i), in_nodes=_p(k["in_nodes"], i),
+                qts_ptr=_p(k["qts_ptr"], i), qts_t=_p(k["qts_t"], d), qts_v=_p(k["qts_v"], d),
+                n_out=len(outs), out_type=_p(k["out_type"], i), out_ptr=_p(k["out_ptr"], i),
+                out_nodes=_p(k["out_nodes"], i), hts_ptr=_p(k["hts_ptr"], i),
+                hts_t=_p(k["hts_t"], d), hts_v=_p(k["hts_v"], d))
+
+
+def opts_struct(o: Dict[str, object]) -> Opts:
+    """Options in the reference's meaning (options.f90); h_wet follows
+    read_options.f90:441 (C THRESHOLD times H THRESHOLD)."""
+    s = Opts()
+    load().storm_fvt_default_opts(C.byref(s))
+    h_dry = float(o.get("h_dry", 0.001))
+    s.rkorder = int(o.get("rkorder", 2))
+    s.dt_op = int(o.get("dt_op", 0))
+    s.steady = int(bool(o.get("steady", True)))
+    s.stop_cr = int(o.get("stop_cr", 0))
+    s.istop = int(o.get("istop", 0))
+    s.btype = int(o.get("btype", 0))
+    s.hvalve = int(bool(o.get("hvalve", True)))
+    s.limiter = int(o.get("limiter", 2))
+    s.flux = FLUX[str(o.get("flux", "ac"))]
+    s.gradient = GRAD[str(o.get("gradient", "lsq"))]
+    s.friction = int(o.get("friction", 0))
+    s.vclip = int(bool(o.get("vclip", False)))
+    s.cfl = float(o.get("cfl", 1.0))
+    s.rstop = float(o.get("rstop", 1e-4))
+    s.h_dry = h_dry
+    s.h_wet = max(1.0, float(o.get("c_threshold", 1.0))) * h_dry
+    s.vceiling = float(o.get("vceiling", 35.0))
+    s.reorder = int(o.get("reorder", 1))
+    s.block = int(o.get("block", 0))
+    s.graph = int(o.get("graph", 0))
+    return s
+
+
+class FVTSolver:
+    """One FVT context on the current HIP device."""
+
+    def __init__(self, case, opts: Optional[Dict[str, object]] = None):
+        self.dataprepFVT(case, opts)
+
+    # -- reference call surface -----------------------------------------
+    def dataprepFVT(self, case, opts: Optional[Dict[str, object]] = None):
+        """dataprepFVT.f90: set up mesh tables, coefficients and state."""
+        L = load()
+        o = dict(case.opts)
+        if opts:
+            o.update(opts)
+        self.opts = o
+        self._keep = {}
+        m = mesh_struct(case, self._keep)
+        self._o = opts_struct(o)
+        h = C.c_void_p()
+        rc = L.storm_fvt_create(C.byref(m), C.byref(self._o), C.byref(h))
+        if rc != 0 or not h.value:
+            raise StormError(f"storm_fvt_create failed ({rc})")
+        self.h = h
+        self.N, self.P = case.n_elems, case.n_pts
+        self.n_in, self.n_out = len(case.inflows), len(case.outflows)
+
+    def solverFVT(self):
+        """solverFVT.f90: run to MAX ITERATION (or steady convergence), then
+        the final to_vertices.  Returns (iter, ttime, rmax)."""
+        r = self.run(int(self.opts.get("istop", 0)))
+        self.finalize()
+        return r
+
+    # -- finer control ------------------------------------------------------
+    def _chk(self, rc):
+        if rc != 0:
+            msg = load().storm_fvt_last_error(self.h)
+            raise StormError(f"storm_fvt error {rc}: {msg.decode() if msg else ''}")
+
+    def run(self, nsteps: int):
+        it = C.c_int32(0)
+        tt = C.c_double(0)
+        rm = C.c_double(0)
+        self._chk(load().storm_fvt_run(self.h, int(nsteps), C.byref(it), C.byref(tt), C.byref(rm)))
+        return it.value, tt.value, rm.value
+
+    def finalize(self):
+        self._chk(load().storm_fvt_finalize(self.h))
+
+    def set_state(self, h, u, v):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._chk(load().storm_fvt_set_state(self.h, _p(h, C.c_double), _p(u, C.c_double),
+                                             _p(v, C.c_double)))
+
+    def set_bc_values(self, qin=None, hbc=None):
+        q = None if qin is None else np.ascontiguousarray(qin, dtype=np.float64)
+        hb = None if hbc is None else np.ascontiguousarray(hbc, dtype=np.float64)
+        self._chk(load().storm_fvt_set_bc_values(
+            self.h, None if q is None else _p(q, C.c_double), None if hb is None else _p(hb, C.c_double)))
+
+    def state(self) -> Dict[str, np.ndarray]:
+        N = self.N
+        h, u, v = np.zeros(N), np.zeros(N), np.zeros(N)
+        phi = np.zeros(3 * N)
+        dry = np.zeros(N, dtype=np.uint8)
+        self._chk(load().storm_fvt_get_state(self.h, _p(h, C.c_double), _p(u, C.c_double),
+                                             _p(v, C.c_double), _p(phi, C.c_double),
+                                             _p(dry, C.c_uint8)))
+        qh, qu, qv = np.zeros(N), np.zeros(N), np.zeros(N)
+        self._chk(load().storm_fvt_get_conserved(self.h, _p(qh, C.c_double), _p(qu, C.c_double),
+                                                 _p(qv, C.c_double)))
+        hv, zv = np.zeros(self.P), np.zeros(self.P)
+        self._chk(load().storm_fvt_get_vertex(self.h, _p(hv, C.c_double), _p(zv, C.c_double)))
+        return dict(h=h, u=u, v=v, qh=qh, qu=qu, qv=qv, phi1=phi[:N], phi2=phi[N:2 * N],
+                    phi3=phi[2 * N:], dry=dry.astype(np.int32), hvtx=hv, zetavtx=zv)
+
+    def last_run_seconds(self) -> float:
+        return float(load().storm_fvt_last_run_seconds(self.h))
+
+    def perm(self) -> np.ndarray:
+        p = np.zeros(self.N, dtype=np.int32)
+        self._chk(load().storm_fvt_device_info(self.h, None, _p(p, C.c_int32), None))
+        return p
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().storm_fvt_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def inspect_molecule(case):
+    """The least-squares molecule (elemnt_db.f90:42-128) the library builds,
+    without a device."""
+    keep = {}
+    m = mesh_struct(case, keep)
+    L = load()
+    n = L.storm_fvt_inspect_molecule(C.byref(m), None, None)
+    if n < 0:
+        raise StormError(f"inspect_molecule failed ({n})")
+    ptr = np.zeros(case.n_elems + 1, dtype=np.int32)
+    cells = np.zeros(max(n, 1), dtype=np.int32)
+    L.storm_fvt_inspect_molecule(C.byref(m), _p(ptr, C.c_int32), _p(cells, C.c_int32))
+    return ptr, cells[:n]

@@ -1,0 +1,27 @@
+"""Small seeded cases shared by the parity tests (the oracle runs them in
+well under a second each)."""
+from storm_amd import cases
+
+
+def small_cases():
+    return {
+        "flume_rk2": (cases.flume(nx=20, ny=5, length=100.0, width=25.0, seed=7, dt_op=0,
+                                  cfl=0.05, istop=20, q_in=20.0), 20),
+        "dambreak_wetdry": (cases.dambreak(nq=14, side=200.0, seed=5, x_dam=80.0, h_res=2.0,
+                                           flux="ac", dt_op=0, cfl=0.02, istop=60, h_dry=1e-3), 60),
+        "meander_crit": (cases.meander(ns=40, nn=12, length=400.0, width=120.0, amp=20.0,
+                                       wavelength=200.0, chan_w=40.0, chan_d=2.0, n_bumps=5,
+                                       q0=5.0, q1=20.0, t_ramp=10.0, dt_op=0, cfl=0.02,
+                                       istop=80), 80),
+        "flume_rk3_slip": (cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=11, dt_op=0,
+                                       cfl=0.05, istop=30, q_in=20.0, rkorder=3, btype=1), 30),
+        "flume_rk1_steady": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=12,
+                                         dt_op=0, cfl=0.3, istop=40, q_in=15.0, rkorder=1,
+                                         steady=True, stop_cr=1, rstop=1e-12, hvalve=False), 40),
+        "flume_unlimited_vclip": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=13,
+                                              dt_op=0, cfl=0.05, istop=25, q_in=15.0, limiter=1,
+                                              vclip=True, vceiling=0.3), 25),
+        "flume_zero_grad_hyst": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=14,
+                                             dt_op=0, cfl=0.05, istop=25, q_in=15.0, limiter=0,
+                                             c_threshold=3.0, h0=None, h_out=0.3), 25),
+    }

@@ -1,0 +1,44 @@
+"""Writes the golden reference vectors under tests/golden/.
+
+Runs the reference SToRM FVT path -- built from the reference's own Fortran
+sources by oracle/Makefile into oracle/_ref/ref_driver (this container only)
+-- on the small seeded cases of tests/cases_small.py and stores its final
+state (h, u, v, rk*(:,rkorder), phi, drycell, hvtx, zetavtx, iter, ttime) as
+.npz fixtures.  The fixtures are data: inputs are regenerated from the case
+generator and seed recorded with them.
+
+Usage (from the repository root):  python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle as O  # noqa: E402
+from cases_small import small_cases  # noqa: E402
+
+KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx")
+
+
+def main():
+    out_dir = os.path.dirname(os.path.abspath(__file__))
+    for name, (case, n) in small_cases().items():
+        assert int(case.opts["istop"]) == n
+        with tempfile.TemporaryDirectory() as wd:
+            ref = O.run_reference(case, wd)
+        data = {k: ref[k] for k in KEYS}
+        data["iter"] = np.int64(ref["iter"])
+        data["ttime"] = np.float64(ref["ttime"])
+        data["meta"] = np.array(repr(dict(case.meta)))
+        np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), **data)
+        print(name, case.n_elems, "cells", n, "steps")
+
+
+if __name__ == "__main__":
+    main()

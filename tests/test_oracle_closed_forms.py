@@ -1,0 +1,222 @@
+"""Closed-form and invariant checks of the oracle functions that no
+reference vector pins (corrected HLL / Roe-BGNVC fluxes, Rusanov, Green-Gauss
+gradients, the corrected CFL time step) and of the invariants of the step."""
+import math
+
+import numpy as np
+import pytest
+
+from storm_amd import cases
+
+G = 9.81
+
+
+def phys_flux(h, u, v, nx, ny):
+    return np.array([h * u * nx + h * v * ny,
+                     (h * u * u + 0.5 * G * h * h) * nx + h * u * v * ny,
+                     h * u * v * nx + (h * v * v + 0.5 * G * h * h) * ny])
+
+
+def unit(a):
+    return math.cos(a), math.sin(a)
+
+
+@pytest.mark.parametrize("kind", ["ac", "hll", "rusanov", "bgnvc"])
+def test_flux_consistency(kind):
+    import oracle as O
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        h = rng.uniform(0.05, 5.0)
+        u, v = rng.normal(0, 1.5, 2)
+        nx, ny = unit(rng.uniform(0, 2 * math.pi))
+        c = math.sqrt(G * h)
+        f = O.flux(kind, nx, ny, h, u, v, h, u, v, c, c, 1e-3)
+        np.testing.assert_allclose(f, phys_flux(h, u, v, nx, ny), rtol=1e-13, atol=1e-13)
+
+
+def test_hll_upwind_limits():
+    import oracle as O
+    nx, ny = unit(0.7)
+    hl, hr = 1.0, 0.8
+    # supercritical flow to the right: F = F_L
+    ul, vl = 8.0 * nx, 8.0 * ny
+    ur, vr = 7.5 * nx, 7.5 * ny
+    f = O.flux("hll", nx, ny, hl, ul, vl, hr, ur, vr)
+    np.testing.assert_array_equal(f, phys_flux(hl, ul, vl, nx, ny))
+    f = O.flux("hll", nx, ny, hl, -ul, -vl, hr, -ur, -vr)
+    np.testing.assert_array_equal(f, phys_flux(hr, -ur, -vr, nx, ny))
+
+
+def test_hll_is_the_two_wave_average():
+    import oracle as O
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        hl, hr = rng.uniform(0.2, 3.0, 2)
+        ul, vl, ur, vr = rng.normal(0, 0.5, 4)
+        nx, ny = unit(rng.uniform(0, 2 * math.pi))
+        f = O.flux("hll", nx, ny, hl, ul, vl, hr, ur, vr)
+        cl, cr = math.sqrt(G * hl), math.sqrt(G * hr)
+        us = 0.5 * ((ul + ur) * nx + (vl + vr) * ny) + cl - cr
+        cs = 0.5 * (cl + cr) + ((ul - ur) * nx + (vl - vr) * ny) / 4
+        sl = min(ul * nx + vl * ny - cl, us - cs)
+        sr = max(ur * nx + vr * ny + cr, us + cs)
+        assert sl < 0 < sr
+        FL, FR = phys_flux(hl, ul, vl, nx, ny), phys_flux(hr, ur, vr, nx, ny)
+        q = np.array([hr - hl, hr * ur - hl * ul, hr * vr - hl * vl])
+        # the star-state conservation identity F* (sR - sL) = sR FL - sL FR + sL sR dq
+        np.testing.assert_allclose(f * (sr - sl), sr * FL - sl * FR + sl * sr * q, rtol=1e-12, atol=1e-12)
+
+
+def test_bgnvc_corrected_equals_roe_ac_away_from_entropy_fix():
+    """Both are Roe fluxes with the same averages: with every |lambda| above
+    the entropy-fix floor of flux_ac (0.1) they must agree to round-off.  The
+    reference's defective left eigenvectors would miss by O(1)."""
+    import oracle as O
+    rng = np.random.default_rng(3)
+    n = 0
+    while n < 50:
+        hl, hr = rng.uniform(0.3, 3.0, 2)
+        ul, vl, ur, vr = rng.normal(0, 0.8, 4)
+        nx, ny = unit(rng.uniform(0, 2 * math.pi))
+        sl, sr = math.sqrt(hl), math.sqrt(hr)
+        ua = (ur * sr + ul * sl) / (sr + sl)
+        va = (vr * sr + vl * sl) / (sr + sl)
+        ca = math.sqrt(0.5 * G * (hl + hr))
+        un = ua * nx + va * ny
+        if min(abs(un), abs(un - ca), abs(un + ca)) < 0.2:
+            continue
+        n += 1
+        fa = O.flux("ac", nx, ny, hl, ul, vl, hr, ur, vr)
+        fb = O.flux("bgnvc", nx, ny, hl, ul, vl, hr, ur, vr)
+        np.testing.assert_allclose(fb, fa, rtol=1e-11, atol=1e-11)
+
+
+def test_rusanov_closed_form():
+    import oracle as O
+    nx, ny = unit(1.1)
+    hl, ul, vl, hr, ur, vr = 1.3, 0.4, -0.2, 0.9, -0.1, 0.3
+    cl, cr = 2.0, 1.5
+    f = O.flux("rusanov", nx, ny, hl, ul, vl, hr, ur, vr, cl, cr)
+    s = max(abs(ul * nx + vl * ny) + cl, abs(ur * nx + vr * ny) + cr)
+    q = np.array([hr - hl, hr * ur - hl * ul, hr * vr - hl * vl])
+    ref = 0.5 * (phys_flux(hl, ul, vl, nx, ny) + phys_flux(hr, ur, vr, nx, ny) - s * q)
+    np.testing.assert_allclose(f, ref, rtol=1e-14, atol=1e-14)
+
+
+def test_equals_semantics():
+    import oracle as O
+    assert O.equals(0.0, 0.0)
+    assert O.equals(1.0, 1.0 + 1e-17)
+    assert not O.equals(1.0, 1.0 + 1e-15)
+    assert O.equals(0.0, 5e-324)          # eps underflows to tiny: subnormal equals 0
+    assert not O.equals(1e-300, 0.0)
+
+
+def _linear_case(gradient, limiter):
+    c = cases.flume(nx=12, ny=6, length=60.0, width=30.0, seed=21, dt_op=0, cfl=0.01,
+                    istop=1, gradient=gradient, limiter=limiter, rough_sd=0.0, slope=0.0)
+    x = c.x[c.tri].mean(axis=1)
+    y = c.y[c.tri].mean(axis=1)
+    zc = c.zvtx[c.tri].mean(axis=1)
+    # linear water surface, linear velocities (all cells wet)
+    zeta = 3.0 + 0.01 * x - 0.02 * y
+    c.h = zeta - zc
+    c.u = 0.5 + 0.003 * x + 0.001 * y
+    c.v = -0.2 + 0.002 * x - 0.004 * y
+    return c
+
+
+def test_lsq_gradient_exact_on_linear_fields():
+    import oracle as O
+    c = _linear_case("lsq", 1)
+    g = O.Oracle(c).cell_grads()
+    np.testing.assert_allclose(g[:, 0], 0.01, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g[:, 1], -0.02, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g[:, 2:4], np.tile([0.003, 0.001], (c.n_elems, 1)), atol=1e-12)
+    np.testing.assert_allclose(g[:, 4:6], np.tile([0.002, -0.004], (c.n_elems, 1)), atol=1e-12)
+
+
+def test_gauss_gradient_exact_on_linear_fields_in_the_interior():
+    import oracle as O
+    c = _linear_case("gauss", 1)
+    o = O.Oracle(c)
+    g = o.cell_grads()
+    _, _, t3 = o.topology()
+    interior = np.all(t3 >= 0, axis=1)
+    inner = interior & np.all(interior[np.where(t3 >= 0, t3, 0)], axis=1)
+    assert inner.sum() > 50
+    np.testing.assert_allclose(g[inner, 0], 0.01, atol=1e-12)
+    np.testing.assert_allclose(g[inner, 1], -0.02, atol=1e-12)
+    np.testing.assert_allclose(g[inner, 2], 0.003, atol=1e-12)
+    np.testing.assert_allclose(g[inner, 5], -0.004, atol=1e-12)
+
+
+def test_barth_jespersen_bounds():
+    import oracle as O
+    c = cases.flume(nx=12, ny=6, length=60.0, width=30.0, seed=22, dt_op=0, cfl=0.01, istop=1)
+    rng = np.random.default_rng(5)
+    c.h = 2.0 + rng.uniform(-0.3, 0.3, c.n_elems)
+    c.u = rng.normal(0, 0.5, c.n_elems)
+    c.v = rng.normal(0, 0.5, c.n_elems)
+    o = O.Oracle(c)
+    g = o.cell_grads()
+    _, _, t3 = o.topology()
+    xc = c.x[c.tri].mean(axis=1)
+    yc = c.y[c.tri].mean(axis=1)
+    zc = c.zvtx[c.tri].mean(axis=1)
+    for comp, q in ((0, c.h + zc), (2, c.u), (4, c.v)):
+        for j in range(3):
+            a, b = c.tri[:, j], c.tri[:, (j + 1) % 3]
+            rx = 0.5 * (c.x[a] + c.x[b]) - xc
+            ry = 0.5 * (c.y[a] + c.y[b]) - yc
+            qf = q + g[:, comp] * rx + g[:, comp + 1] * ry
+            nb = t3[:, j]
+            qn = np.where(nb >= 0, q[np.where(nb >= 0, nb, 0)], q)
+            lo = np.minimum(q, qn)
+            hi = np.maximum(q, qn)
+            assert np.all(qf >= lo - 1e-12) and np.all(qf <= hi + 1e-12)
+
+
+def test_cfl_dt_is_the_brute_force_minimum():
+    import oracle as O
+    c = cases.dambreak(nq=10, side=100.0, seed=6, x_dam=40.0, h_res=2.0, dt_op=1, cfl=0.5)
+    rng = np.random.default_rng(7)
+    c.u = rng.normal(0, 1, c.n_elems) * (c.h > 0)
+    c.v = rng.normal(0, 1, c.n_elems) * (c.h > 0)
+    o = O.Oracle(c)
+    dt = o.dt_cfl()
+    x, y = c.x[c.tri], c.y[c.tri]
+    area = 0.5 * np.abs((x[:, 0] - x[:, 1]) * (y[:, 0] + y[:, 1]) + (x[:, 1] - x[:, 2]) * (y[:, 1] + y[:, 2])
+                        + (x[:, 2] - x[:, 0]) * (y[:, 2] + y[:, 0]))
+    per = sum(np.hypot(x[:, j] - x[:, (j + 1) % 3], y[:, j] - y[:, (j + 1) % 3]) for j in range(3))
+    wet = c.h >= 1e-4
+    um = np.where(wet, np.hypot(c.u, c.v), 0.0)
+    wc = np.where(wet, np.sqrt(G * c.h), 0.0)
+    small = float(np.float32(1e-9))
+    ref = np.min(0.5 * area / ((um + wc + small) * per))
+    assert dt == pytest.approx(ref, rel=1e-14)
+
+
+def test_mass_conserved_with_closed_walls():
+    import oracle as O
+    c = cases.dambreak(nq=12, side=120.0, seed=8, x_dam=50.0, h_res=1.5, flux="ac",
+                       dt_op=0, cfl=0.02, istop=100, h_dry=1e-3)
+    x, y = c.x[c.tri], c.y[c.tri]
+    area = 0.5 * np.abs((x[:, 0] - x[:, 1]) * (y[:, 0] + y[:, 1]) + (x[:, 1] - x[:, 2]) * (y[:, 1] + y[:, 2])
+                        + (x[:, 2] - x[:, 0]) * (y[:, 2] + y[:, 0]))
+    m0 = (c.h * area).sum()
+    o = O.Oracle(c)
+    o.run(100)
+    m1 = (o.state()["h"] * area).sum()
+    assert abs(m1 - m0) / m0 < 1e-12
+
+
+def test_flat_lake_at_rest_stays_at_rest():
+    import oracle as O
+    c = cases.dambreak(nq=10, side=100.0, seed=9, x_dam=1e9, h_res=2.0, n_bumps=0,
+                       dt_op=0, cfl=0.05, istop=30)
+    o = O.Oracle(c)
+    o.run(30)
+    s = o.state()
+    assert np.abs(s["u"]).max() < 1e-12 and np.abs(s["v"]).max() < 1e-12
+    np.testing.assert_allclose(s["h"], 2.0, atol=1e-12)
