@@ -1,0 +1,125 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X FVT time step on the single-reach workload.
+
+Workload (one GPU): sinusoidal meandering channel, 20 km long, 2,000,000
+jittered triangles (4000 x 250 quads, each split in two), parabolic main
+channel 80 m x 4 m, 1 km floodplain initially dry with 200 Gaussian bumps,
+Manning n 0.035 / 0.06, inflow ramp 500 -> 3000 m^3/s over 1 h, critical-depth
+outflow, SSP-RK3, CFL 0.9 with the time step from the CFL reduction, moving
+wet/dry fronts.  A step is one full SSP-RK3 time step of the whole reach
+(three stages, implicit friction, CFL reduction, residual); the state is
+resident in HBM when timing starts.
+
+Prints one JSON line.  `value` is cell-updates/s (triangles x steps / s).
+`roofline` prices the step with the algorithmic bytes of one cell-update of
+SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
+residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
+times the CPU oracle (a serial C restatement of the reference step) on the
+same reach for a bounded number of steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_STEP_RK3 = 1248.0          # algorithmic bytes per cell-update, SSP-RK3
+PEAK_HBM = 8.0e12            # B/s, MI355X
+METRIC = "FP64 cell-updates/s + HBM % roofline; time-to-steady warm vs cold"
+
+
+def cpu_baseline(case, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline only
+    O.build()
+    o = O.Oracle(case, dict(istop=10 ** 9))
+    o.run(1)  # warm-up step (first-touch of the arrays)
+    t0 = time.perf_counter()
+    o.run(steps)
+    t = time.perf_counter() - t0
+    return case.n_elems * steps / t, t
+
+
+def traffic_from_profiles(workload):
+    path = os.path.join(ROOT, "profiles", "traffic_meander2m.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("workload") == workload:
+            return d.get("bytes_per_step")
+    except Exception:
+        return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the FVT step on one GPU only; --gpus %d is not "
+              "supported (DESIGN.md describes how the reach would shard)." % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from storm_amd import cases, fvt
+    case = cases.meander()
+    workload = "meander-2M-rk3"
+    t0 = time.perf_counter()
+    solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph))
+    setup_s = time.perf_counter() - t0
+    solver.run(args.warmup)
+    t0 = time.perf_counter()
+    it, ttime, rmax = solver.run(args.steps)
+    wall = time.perf_counter() - t0
+    gpu_s = solver.last_run_seconds()
+    n = case.n_elems
+    value = n * args.steps / wall
+    step_s = gpu_s / args.steps
+    achieved = B_STEP_RK3 * n / step_s
+    traffic = traffic_from_profiles(workload)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded meander generator, storm_amd/cases.py)",
+        "config": {"workload": workload, "triangles": n, "vertices": case.n_pts,
+                   "scheme": "Roe flux_ac, LSQ+Barth-Jespersen, SSP-RK3, CFL 0.9",
+                   "boundaries": "Q ramp 500->3000 m3/s inflow, critical outflow, walls",
+                   "setup_s": round(setup_s, 3), "sim_time_s": ttime},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM,
+                     "traffic": traffic,
+                     "kernel": "one SSP-RK3 time step (all its launches)",
+                     "algorithmic_bytes_per_launch": B_STEP_RK3 * n,
+                     "avg_launch_ms": 1e3 * step_s},
+    }
+    if not args.no_cpu:
+        cpu_rate, cpu_t = cpu_baseline(case, args.cpu_steps)
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "%d SSP-RK3 steps of the same 2M-triangle reach "
+                                         "(%.1f s), serial C oracle" % (args.cpu_steps, cpu_t)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -40,7 +40,6 @@ struct Ctrl {
   double dt;          // dt of the current step
   double delt_last;   // steady: local dt of the reference's last cell
   double ttime, stime, rmax, sumtop, sumbot;
-  unsigned long long dtmin;  // bits of the next step's CFL dt (positive doubles order as integers)
   int iter, done, pending, pad;
   double qin[MAX_STRINGS], hbc[MAX_STRINGS];
   double atotal[MAX_STRINGS];
@@ -99,16 +98,18 @@ __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_
 }
 
 // ---------------------------------------------------------------- init
-__global__ void k_init(Geo g, Opts o, double* slotR, const double* h0, const double* u0,
-                       const double* v0, double* u, double* v, uint8_t* dry, Ctrl* ctrl) {
-  // solverFVT.f90:123-127 and the first find_dry_cells; CFL candidate of step 1
+__global__ void __launch_bounds__(256)
+k_init(Geo g, Opts o, double* slot0, const double* h0, const double* u0, const double* v0,
+       double* u, double* v, uint8_t* dry, double* part) {
+  // solverFVT.f90:123-127 and the first find_dry_cells; the CFL candidates of
+  // step 1 go to the per-block partials (slot 3), reduced by k_control
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double cand = DBL_MAX;
   if (i < g.N) {
     const double h = h0[i], uu = u0[i], vv = v0[i];
-    slotR[g.N + i] = h * uu;
-    slotR[2 * g.N + i] = h * vv;
-    slotR[i] = h;
+    slot0[g.N + i] = h * uu;
+    slot0[2 * g.N + i] = h * vv;
+    slot0[i] = h;
     u[i] = uu;
     v[i] = vv;
     double um;
@@ -118,9 +119,19 @@ __global__ void k_init(Geo g, Opts o, double* slotR, const double* h0, const dou
     double dd = um + wc + SMALL;
     cand = o.cfl * g.area[i] / (dd * g.perim[i]);
   }
-  if (o.dt_op == 1 && !o.steady) {
-    for (int off = 32; off > 0; off >>= 1) cand = dmin(cand, __shfl_xor(cand, off));
-    if ((threadIdx.x & 63) == 0) atomicMin(&ctrl->dtmin, (unsigned long long)__double_as_longlong(cand));
+  __shared__ double s_c[256];
+  const int t = threadIdx.x;
+  s_c[t] = cand;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s_c[t] = dmin(s_c[t], s_c[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[4 * blockIdx.x] = ZERO;
+    part[4 * blockIdx.x + 1] = ZERO;
+    part[4 * blockIdx.x + 2] = ZERO;
+    part[4 * blockIdx.x + 3] = s_c[0];
   }
 }
 
@@ -168,63 +179,59 @@ __global__ void __launch_bounds__(CTRL_THREADS)
 k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, const int* out_type,
           const int* qts_ptr, const double* qts_t, const double* qts_v, const int* hts_ptr,
           const double* hts_t, const double* hts_v) {
-  __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS];
-  __shared__ int s_done;
+  __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS],
+      s_dt[CTRL_THREADS];
   const int t = threadIdx.x;
   if (c->done) return;
-  if (c->pending) {
-    // fixed-order reduction of the per-block partials of the last stage
-    double top = ZERO, bot = ZERO, mx = ZERO;
-    for (int b = t; b < nparts; b += CTRL_THREADS) {
-      top = top + part[3 * b];
-      bot = bot + part[3 * b + 1];
-      mx = dmax(mx, part[3 * b + 2]);
-    }
-    s_top[t] = top; s_bot[t] = bot; s_max[t] = mx;
-    __syncthreads();
-    for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
-      if (t < w) {
-        s_top[t] = s_top[t] + s_top[t + w];
-        s_bot[t] = s_bot[t] + s_bot[t + w];
-        s_max[t] = dmax(s_max[t], s_max[t + w]);
-      }
-      __syncthreads();
-    }
-    if (t == 0) {
-      double sumtop = s_top[0], sumbot = s_bot[0];
-      c->sumtop = sumtop;
-      c->sumbot = sumbot;
-      int done = 0;
-      if (o.steady) {
-        if (o.stop_cr == 0) c->rmax = s_max[0];
-        else {
-          if (equals(sumbot, ZERO)) sumbot = ONE;
-          c->rmax = sqrt(sumtop / sumbot);
-        }
-        if (c->rmax < o.rstop) done = 1;
-        c->stime = (double)c->iter;
-      } else {
-        c->stime = c->stime + c->dt;
-        c->rmax = s_max[0];
-        get_time_bcs(g, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, c->stime, c);
-      }
-      c->pending = 0;
-      c->done = done;
-      s_done = done;
-    }
-    __syncthreads();
-    if (s_done) return;
+  // fixed-order reduction of the per-block partials of the last stage
+  // (residual sums, max|phi|) and of the next step's CFL candidates
+  double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX;
+  for (int b = t; b < nparts; b += CTRL_THREADS) {
+    top = top + part[4 * b];
+    bot = bot + part[4 * b + 1];
+    mx = dmax(mx, part[4 * b + 2]);
+    dtm = dmin(dtm, part[4 * b + 3]);
   }
-  if (begin && t == 0) {
+  s_top[t] = top; s_bot[t] = bot; s_max[t] = mx; s_dt[t] = dtm;
+  __syncthreads();
+  for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      s_top[t] = s_top[t] + s_top[t + w];
+      s_bot[t] = s_bot[t] + s_bot[t + w];
+      s_max[t] = dmax(s_max[t], s_max[t + w]);
+      s_dt[t] = dmin(s_dt[t], s_dt[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  if (c->pending) {
+    double sumtop = s_top[0], sumbot = s_bot[0];
+    c->sumtop = sumtop;
+    c->sumbot = sumbot;
+    if (o.steady) {
+      if (o.stop_cr == 0) c->rmax = s_max[0];
+      else {
+        if (equals(sumbot, ZERO)) sumbot = ONE;
+        c->rmax = sqrt(sumtop / sumbot);
+      }
+      c->stime = (double)c->iter;
+      c->pending = 0;
+      if (c->rmax < o.rstop) { c->done = 1; return; }
+    } else {
+      c->stime = c->stime + c->dt;
+      c->rmax = s_max[0];
+      get_time_bcs(g, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, c->stime, c);
+      c->pending = 0;
+    }
+  }
+  if (begin) {
     c->iter = c->iter + 1;
     if (!o.steady) {
-      if (o.dt_op == 0) c->dt = o.cfl;
-      else c->dt = __longlong_as_double((long long)c->dtmin);
+      c->dt = (o.dt_op == 0) ? o.cfl : s_dt[0];
       c->ttime = c->ttime + c->dt;
     } else {
       c->ttime = c->ttime + c->delt_last;
     }
-    c->dtmin = (unsigned long long)__double_as_longlong(DBL_MAX);
     c->pending = 1;
   }
 }
@@ -877,14 +884,18 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
     __syncthreads();
   }
   if (t == 0) {
-    part[3 * blockIdx.x] = s_top[0];
-    part[3 * blockIdx.x + 1] = s_bot[0];
-    part[3 * blockIdx.x + 2] = s_mx[0];
+    part[4 * blockIdx.x] = s_top[0];
+    part[4 * blockIdx.x + 1] = s_bot[0];
+    part[4 * blockIdx.x + 2] = s_mx[0];
   }
-  if (o.dt_op == 1 && !o.steady) {
-    for (int off = 32; off > 0; off >>= 1) cand = dmin(cand, __shfl_xor(cand, off));
-    if ((t & 63) == 0) atomicMin(&S.ctrl->dtmin, (unsigned long long)__double_as_longlong(cand));
+  __syncthreads();
+  s_top[t] = cand;
+  __syncthreads();
+  for (int w = UPD_THREADS / 2; w > 0; w >>= 1) {
+    if (t < w) s_top[t] = dmin(s_top[t], s_top[t + w]);
+    __syncthreads();
   }
+  if (t == 0) part[4 * blockIdx.x + 3] = s_top[0];
 }
 
 // to_vertices' cell pass alone (solverFVT.f90:483 -> to_vertices.f90:44-55),
@@ -1090,7 +1101,7 @@ void setup_device(storm_fvt* C) {
   S.vwet = dalloc<uint8_t>(own, P.P);
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
-  S.part = dalloc<double>(own, 3 * (size_t)C->nblk_upd);
+  S.part = dalloc<double>(own, 4 * (size_t)C->nblk_upd);
   C->dry_a = dalloc<uint8_t>(own, N);
   C->dry_b = dalloc<uint8_t>(own, N);
   S.ctrl = dalloc<Ctrl>(own, 1);
@@ -1102,14 +1113,13 @@ void start_run(storm_fvt* C) {
   Prep& P = C->prep;
   Ctrl hc;
   std::memset(&hc, 0, sizeof(hc));
-  hc.dtmin = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
   host_time_bcs(P, 0.0, hc.qin, hc.hbc);
   HIPCHK(hipMemcpyAsync(C->S.ctrl, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, C->stream));
   C->r1_flip = 0;
   C->last_hin = C->slot_ptr(0);
   const int nb = (P.N + 255) / 256;
   hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, C->stream, C->g, C->o, C->slot_ptr(0),
-                     C->d_h0, C->d_u0, C->d_v0, C->S.u, C->S.v, C->dry_a, C->S.ctrl);
+                     C->d_h0, C->d_u0, C->d_v0, C->S.u, C->S.v, C->dry_a, C->S.part);
   HIPCHK(hipGetLastError());
   C->S.dry_in = C->dry_a;
   C->S.dry_out = C->dry_b;
