@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter "
+                         "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the FVT step on one GPU only; --gpus %d is not "
@@ -73,6 +76,13 @@ def main():
         sys.exit(2)
 
     from storm_amd import cases, fvt
+    if args.calib_copy > 0:
+        import torch
+        nbytes = args.calib_copy << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     case = cases.meander()
     workload = "meander-2M-rk3"
     t0 = time.perf_counter()

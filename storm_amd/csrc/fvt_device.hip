@@ -34,7 +34,7 @@ using namespace storm;
 
 namespace {
 
-constexpr int CTRL_THREADS = 256;
+constexpr int CTRL_THREADS = 1024;
 
 struct Ctrl {
   double dt;          // dt of the current step
@@ -46,21 +46,27 @@ struct Ctrl {
   int null_area[MAX_STRINGS], qskip[MAX_STRINGS];
 };
 
+// Cell records, read and written whole with 16-byte loads: static data,
+// stage state (double-buffered), and the face record k_grad writes for
+// k_update (one 32-byte slot per side, so a neighbour fetches the state at
+// the shared side with one load).
+struct alignas(32) CellS { double xc, yc, z, cd; };
+struct alignas(32) CellD { double h, u, v, dry; };        // dry: 1 or 0
+struct alignas(32) FSlot { double h, u, v, z; };          // face state + the cell's bed
+struct alignas(32) CellF { FSlot s[3]; double dhx, dhy, umag, h; };  // umag < 0: dry cell
+
 struct Geo {
   int N, P, R;
-  const int *cvtx, *cnbr, *ep1, *ep2;
-  const uint32_t* cinfo;
-  const uint8_t* corder;
-  const double *rc, *enx, *eny, *etx, *ety, *elen, *ezmid;
-  const double *area, *perim, *z, *cd, *delzx, *delzy, *zhigh, *vdx, *vdy, *zv3;
-  const double *xv, *yv;
+  const int *cvtx, *cnbr;
+  const uint64_t* cmeta;
+  const CellS* cs;
+  const double4* vxyz;
   const int *mptr, *mcell;
-  const double *mdx, *mdy, *mw, *ai, *bi, *ci, *det;
   const int* gconn;
   const double *gcx, *gcy, *gwt;
   const int *vptr, *vcell, *vbc;
-  const double *vw, *zvtx;
   const int *qs_ptr, *qs_cell, *qs_slot;
+  const double *qs_rcx, *qs_rcy, *qs_nx, *qs_ny, *qs_len;
   const int* in_type;
   int n_in, n_out, last_cell;
   double bch_max;
@@ -75,16 +81,104 @@ struct Stage {
   const double* slot[4];  // rk slots 0..s-1 (h at +0, hu at +N, hv at +2N)
   double alpha[4];
   double* out;            // slot s
-  const double* hin;      // h of the stage input (slot s-1)
+  double* hin;            // slot s-1 (its hu, hv are zeroed by hgradfix2)
   double beta;
   int s, last;
 };
 
 struct State {
-  double *u, *v, *up, *vp, *grad, *gwork, *hvtx, *zetavtx, *phi, *part;
-  uint8_t *dry_in, *dry_out, *dry_post, *vwet;
+  const CellD* din;   // state at the start of the stage
+  CellD* dout;        // state for the next stage
+  CellF* f;
+  double *up, *vp;    // modified velocities, explicit-friction mode only
+  double *gwork, *hvtx, *zetavtx, *phi, *part;
+  uint8_t *dry_post, *vwet;
   Ctrl* ctrl;
 };
+
+__device__ __forceinline__ CellD ld(const CellD* p) {
+  const double4 q = *reinterpret_cast<const double4*>(p);
+  return CellD{q.x, q.y, q.z, q.w};
+}
+__device__ __forceinline__ CellS ld(const CellS* p) {
+  const double4 q = *reinterpret_cast<const double4*>(p);
+  return CellS{q.x, q.y, q.z, q.w};
+}
+__device__ __forceinline__ FSlot ld(const FSlot* p) {
+  const double4 q = *reinterpret_cast<const double4*>(p);
+  return FSlot{q.x, q.y, q.z, q.w};
+}
+__device__ __forceinline__ void st4(CellD* p, double a, double b, double c, double d) {
+  *reinterpret_cast<double4*>(p) = make_double4(a, b, c, d);
+}
+__device__ __forceinline__ void st4(double* p, double a, double b, double c, double d) {
+  *reinterpret_cast<double4*>(p) = make_double4(a, b, c, d);
+}
+
+// ---------------------------------------------------------------- XCD-aware tiles
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Mapping dispatch index b to tile start(b % 8) + b / 8 gives every XCD one
+// contiguous range of the (RCM-ordered) cells, so the neighbour gathers of a
+// tile hit the L2 that fetched the neighbouring tiles.  The map is a
+// bijection on [0, gridDim.x).
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int x = b % NXCD, k = b / NXCD;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------- cell geometry
+// Everything geometric is formed on the fly from the three vertices, with the
+// operations and operand order of the reference set-up routine cited, so the
+// values equal the ones the reference stores.
+struct Tri {
+  double x[3], y[3], zv[3];
+};
+
+__device__ __forceinline__ Tri load_tri(const Geo& g, int i) {
+  Tri t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double4 q = g.vxyz[g.cvtx[k * g.N + i]];
+    t.x[k] = q.x; t.y[k] = q.y; t.zv[k] = q.z;
+  }
+  return t;
+}
+
+// t_areas.f90:40-41
+__device__ __forceinline__ double tri_area(const Tri& t) {
+  return HALF * fabs((t.x[0] - t.x[1]) * (t.y[0] + t.y[1]) + (t.x[1] - t.x[2]) * (t.y[1] + t.y[2]) +
+                     (t.x[2] - t.x[0]) * (t.y[2] + t.y[0]));
+}
+
+// find_edges.f90:159-163 (length), edge_normals.f90:35-38 + dataprepFVT.f90:176-181
+// (unit normal and tangent of the side from vertex j to vertex j+1; the
+// reference's edges(k) normal is this one, negated when edges(k)%p runs
+// the other way, and flipped to e(1)'s orientation before use)
+struct Side {
+  double len, nx, ny, tx, ty, mx, my, zm;
+};
+template <int J>
+__device__ __forceinline__ Side side(const Tri& t) {
+  constexpr int A = J, B = (J + 1) % 3;
+  Side s;
+  const double dx = t.x[A] - t.x[B], dy = t.y[A] - t.y[B];
+  s.len = sqrt(dx * dx + dy * dy);
+  s.nx = (t.y[B] - t.y[A]) / s.len;
+  s.ny = (t.x[A] - t.x[B]) / s.len;
+  s.tx = (t.x[B] - t.x[A]) / s.len;
+  s.ty = (t.y[B] - t.y[A]) / s.len;
+  s.mx = HALF * (t.x[A] + t.x[B]);    // dataprepFVT.f90:168-169
+  s.my = HALF * (t.y[A] + t.y[B]);
+  s.zm = HALF * (t.zv[A] + t.zv[B]);  // invisc_fluxes.f90:89
+  return s;
+}
+
+__device__ __forceinline__ uint32_t slot_bits(uint64_t meta, int j) {
+  return (uint32_t)(meta >> (CM_SLOT0 + CM_SLOT_BITS * j)) & ((1u << CM_SLOT_BITS) - 1);
+}
 
 __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_dry, double h_wet,
                                          double* umag) {
@@ -100,24 +194,25 @@ __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_
 // ---------------------------------------------------------------- init
 __global__ void __launch_bounds__(256)
 k_init(Geo g, Opts o, double* slot0, const double* h0, const double* u0, const double* v0,
-       double* u, double* v, uint8_t* dry, double* part) {
+       CellD* d, double* part) {
   // solverFVT.f90:123-127 and the first find_dry_cells; the CFL candidates of
   // step 1 go to the per-block partials (slot 3), reduced by k_control
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double cand = DBL_MAX;
   if (i < g.N) {
     const double h = h0[i], uu = u0[i], vv = v0[i];
     slot0[g.N + i] = h * uu;
     slot0[2 * g.N + i] = h * vv;
     slot0[i] = h;
-    u[i] = uu;
-    v[i] = vv;
     double um;
-    bool d = find_dry(h, uu, vv, o.h_dry, o.h_wet, &um);
-    dry[i] = d;
-    const double wc = d ? ZERO : sqrt(G * h);
+    const bool dr = find_dry(h, uu, vv, o.h_dry, o.h_wet, &um);
+    st4(&d[i], h, uu, vv, dr ? ONE : ZERO);
+    const double wc = dr ? ZERO : sqrt(G * h);
+    const Tri t = load_tri(g, i);
+    const double area = tri_area(t);
+    const double perim = side<0>(t).len + side<1>(t).len + side<2>(t).len;
     double dd = um + wc + SMALL;
-    cand = o.cfl * g.area[i] / (dd * g.perim[i]);
+    cand = o.cfl * area / (dd * perim);
   }
   __shared__ double s_c[256];
   const int t = threadIdx.x;
@@ -237,25 +332,35 @@ k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, con
 }
 
 // ---------------------------------------------------------------- vertex pass
-// to_vertices.f90:26-41 with ctr2vtxWSE.f90:89-105 and positivity_vtx.f90,
-// then the stage strings' nodes of enforce_bcs.f90:40-56.
-__global__ void k_vertex(Geo g, Opts o, const double* hin, const uint8_t* dry, double* hvtx,
-                         uint8_t* vwet, double* zetavtx, const Ctrl* c) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.P || c->done) return;
-  const double zv = g.zvtx[p];
+// to_vertices.f90:26-41 with ctr2vtxWSE.f90:89-105 (inverse-distance weights
+// of c2v_setup.f90:106-113) and positivity_vtx.f90, then the stage strings'
+// nodes of enforce_bcs.f90:40-56.  `dry` overrides the dry flags of `d` (the
+// post-loop call uses the flags of the post-step find_dry_cells).
+__global__ void __launch_bounds__(256)
+k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_t* vwet,
+         double* zetavtx, const Ctrl* c) {
+  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  const double4 q = g.vxyz[p];
+  const double zv = q.z;
   double id = ZERO, a = ZERO, zr = zv;
   int counter = 0;
-  for (int t = g.vptr[p]; t < g.vptr[p + 1]; ++t) {
+  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
+  for (int t = t0; t < t1; ++t) {
     const int k = g.vcell[t];
-    if (dry[k]) continue;
+    const CellD dk = ld(&d[k]);
+    const CellS sk = ld(&g.cs[k]);
+    const bool dk_dry = dry ? (dry[k] != 0) : (dk.dry != ZERO);
+    if (dk_dry) continue;
     counter++;
-    const double w = g.vw[t];
-    const double zk = hin[k] + g.z[k];
+    const double xk = sk.xc - q.x, yk = sk.yc - q.y;
+    const double w = ONE / sqrt(xk * xk + yk * yk);
+    const double zk = dk.h + sk.z;
     id = id + w;
     a = a + w * zk;
     zr = dmax(zk, zr);
   }
+  if (c->done) return;
   double zeta = (counter > 0) ? a / id : zv;
   zeta = dmin(zeta, zr);
   double hv = zeta - zv;
@@ -268,22 +373,22 @@ __global__ void k_vertex(Geo g, Opts o, const double* hin, const uint8_t* dry, d
 }
 
 // ---------------------------------------------------------------- Green-Gauss pass 1
-__global__ void k_gauss1(Geo g, const double* hin, const double* u, const double* v,
-                         double* gwork, const Ctrl* c) {
+__global__ void k_gauss1(Geo g, const CellD* d, double* gwork, const Ctrl* c) {
   // gaussgrad.f90:37-42 for zeta, u, v
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= g.N || c->done) return;
   const int N = g.N;
   const int n0 = g.gconn[i], n1 = g.gconn[N + i], n2 = g.gconn[2 * N + i];
   const double cx0 = g.gcx[i], cx1 = g.gcx[N + i], cx2 = g.gcx[2 * N + i];
   const double cy0 = g.gcy[i], cy1 = g.gcy[N + i], cy2 = g.gcy[2 * N + i];
-  const double z0 = hin[n0] + g.z[n0], z1 = hin[n1] + g.z[n1], z2 = hin[n2] + g.z[n2];
+  const CellD d0 = ld(&d[n0]), d1 = ld(&d[n1]), d2 = ld(&d[n2]);
+  const double z0 = d0.h + g.cs[n0].z, z1 = d1.h + g.cs[n1].z, z2 = d2.h + g.cs[n2].z;
   gwork[i] = cx0 * z0 + cx1 * z1 + cx2 * z2;
   gwork[N + i] = cy0 * z0 + cy1 * z1 + cy2 * z2;
-  gwork[2 * N + i] = cx0 * u[n0] + cx1 * u[n1] + cx2 * u[n2];
-  gwork[3 * N + i] = cy0 * u[n0] + cy1 * u[n1] + cy2 * u[n2];
-  gwork[4 * N + i] = cx0 * v[n0] + cx1 * v[n1] + cx2 * v[n2];
-  gwork[5 * N + i] = cy0 * v[n0] + cy1 * v[n1] + cy2 * v[n2];
+  gwork[2 * N + i] = cx0 * d0.u + cx1 * d1.u + cx2 * d2.u;
+  gwork[3 * N + i] = cy0 * d0.u + cy1 * d1.u + cy2 * d2.u;
+  gwork[4 * N + i] = cx0 * d0.v + cx1 * d1.v + cx2 * d2.v;
+  gwork[5 * N + i] = cy0 * d0.v + cy1 * d1.v + cy2 * d2.v;
 }
 
 // limiterBJ.f90:42-81, beta = 1 (cell_grads.f90:59)
@@ -308,16 +413,39 @@ __device__ __forceinline__ void bj_limit(double q0, const double qn[3], const do
 }
 
 // ---------------------------------------------------------------- gradients
-// cell_grads.f90:24-75, hgradfix2.f90:46-98 and to_vertices.f90:44-55.
-__global__ void k_grad(Geo g, Opts o, Stage st, State S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.N || S.ctrl->done) return;
+// cell_grads.f90:24-75 (least squares of lsq_grad_array.f90:60-73 with the
+// weights and normal-equation sums of lsq_setup.f90:105-145 formed on the fly),
+// hgradfix2.f90:46-98, the cell pass of to_vertices.f90:44-55, and the face
+// states of invisc_fluxes.f90:53-81 that k_update consumes.
+__global__ void __launch_bounds__(256)
+k_grad(Geo g, Opts o, Stage st, State S) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
   const int N = g.N;
-  const double* h = st.hin;
-  const double hi = h[i], zi = g.z[i];
+  // first level of loads: own records and neighbour ids
+  const uint64_t meta = g.cmeta[i];
+  const int nb[3] = {g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i]};
+  const CellD di = ld(&S.din[i]);
+  const CellS si = ld(&g.cs[i]);
+  const Tri tr = load_tri(g, i);
+  // second level: the edge neighbours' records and the vertex wet flags
+  CellD dn[3];
+  CellS sn[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int k = nb[j] < 0 ? i : nb[j];
+    dn[j] = ld(&S.din[k]);
+    sn[j] = ld(&g.cs[k]);
+  }
+  const bool vw = S.vwet[g.cvtx[i]] && S.vwet[g.cvtx[N + i]] && S.vwet[g.cvtx[2 * N + i]];
+  if (S.ctrl->done) return;
+  const double hi = di.h, zi = si.z, ui = di.u, vi = di.v;
+  const double xci = si.xc, yci = si.yc;
+  const bool dry_i = di.dry != ZERO;
   const double zeta_i = hi + zi;
-  const double ui = S.u[i], vi = S.v[i];
-  const bool dry_i = S.dry_in[i];
+  double nzeta[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) nzeta[j] = dn[j].h + sn[j].z;
   double gzx = ZERO, gzy = ZERO, gux = ZERO, guy = ZERO, gvx = ZERO, gvy = ZERO;
   if (o.limiter != 0) {
     if (o.gradient == STORM_GRAD_GAUSS) {
@@ -332,196 +460,221 @@ __global__ void k_grad(Geo g, Opts o, Stage st, State S) {
       gvx = HALF * (w[4 * N + i] + w0 * w[4 * N + n0] + w1 * w[4 * N + n1] + w2 * w[4 * N + n2]);
       gvy = HALF * (w[5 * N + i] + w0 * w[5 * N + n0] + w1 * w[5 * N + n1] + w2 * w[5 * N + n2]);
     } else {
-      // lsq_grad_array.f90:60-73 for zeta, u, v
       double dz = ZERO, ez = ZERO, du = ZERO, eu = ZERO, dv = ZERO, ev = ZERO;
-      for (int t = g.mptr[i]; t < g.mptr[i + 1]; ++t) {
-        const int k = g.mcell[t];
-        const double w = g.mw[t], dx = g.mdx[t], dy = g.mdy[t];
-        const double fz = (h[k] + g.z[k]) - zeta_i;
-        const double fu = S.u[k] - ui;
-        const double fv = S.v[k] - vi;
+      double a = ZERO, b = ZERO, cc = ZERO;
+      auto member = [&](double mxc, double myc, double mzeta, double mu, double mv) {
+        const double dx = mxc - xci, dy = myc - yci;
+        const double w = ONE / (dx * dx + dy * dy);
+        const double fz = mzeta - zeta_i;
+        const double fu = mu - ui;
+        const double fv = mv - vi;
+        a = a + w * dx * dx; b = b + w * dx * dy; cc = cc + w * dy * dy;
         dz = dz + w * fz * dx; ez = ez + w * fz * dy;
         du = du + w * fu * dx; eu = eu + w * fu * dy;
         dv = dv + w * fv * dx; ev = ev + w * fv * dy;
+      };
+      const bool interior = ((slot_bits(meta, 0) | slot_bits(meta, 1) | slot_bits(meta, 2)) &
+                             CMS_KIND) == SK_INTERIOR;
+      if (interior) {
+        // elemnt_db.f90:50-68: a cell with three edge neighbours keeps them,
+        // in edge-number order, which is the processing order of its sides
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt) {
+          const int j = (int)(meta >> (2 * tt)) & 3;
+          const double mxc = (j == 0) ? sn[0].xc : (j == 1) ? sn[1].xc : sn[2].xc;
+          const double myc = (j == 0) ? sn[0].yc : (j == 1) ? sn[1].yc : sn[2].yc;
+          const double mz = (j == 0) ? nzeta[0] : (j == 1) ? nzeta[1] : nzeta[2];
+          const double mu = (j == 0) ? dn[0].u : (j == 1) ? dn[1].u : dn[2].u;
+          const double mv = (j == 0) ? dn[0].v : (j == 1) ? dn[1].v : dn[2].v;
+          member(mxc, myc, mz, mu, mv);
+        }
+      } else {
+        const int t0 = g.mptr[i], t1 = g.mptr[i + 1];
+        for (int t = t0; t < t1; ++t) {
+          const int k = g.mcell[t];
+          const CellD dk = ld(&S.din[k]);
+          const CellS sk = ld(&g.cs[k]);
+          member(sk.xc, sk.yc, dk.h + sk.z, dk.u, dk.v);
+        }
       }
-      const double a = g.ai[i], b = g.bi[i], cc = g.ci[i], d = g.det[i];
+      const double d = a * cc - b * b;
       gzx = (cc * dz - b * ez) / d; gzy = (a * ez - dz * b) / d;
       gux = (cc * du - b * eu) / d; guy = (a * eu - du * b) / d;
       gvx = (cc * dv - b * ev) / d; gvy = (a * ev - dv * b) / d;
     }
-    if (o.limiter == 2) {
-      double qz[3], qu[3], qv[3], rcx[3], rcy[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int k = g.cnbr[j * N + i];
-        qz[j] = (k < 0) ? zeta_i : h[k] + g.z[k];
-        qu[j] = (k < 0) ? ui : S.u[k];
-        qv[j] = (k < 0) ? vi : S.v[k];
-        rcx[j] = g.rc[(2 * j) * N + i];
-        rcy[j] = g.rc[(2 * j + 1) * N + i];
-      }
-      bj_limit(zeta_i, qz, rcx, rcy, gzx, gzy);
-      bj_limit(ui, qu, rcx, rcy, gux, guy);
-      bj_limit(vi, qv, rcx, rcy, gvx, gvy);
-    }
   }
-  const double dzx = g.delzx[i], dzy = g.delzy[i];
+  const Side s0 = side<0>(tr), s1 = side<1>(tr), s2 = side<2>(tr);
+  const double rcx[3] = {s0.mx - xci, s1.mx - xci, s2.mx - xci};
+  const double rcy[3] = {s0.my - yci, s1.my - yci, s2.my - yci};
+  if (o.limiter == 2) {
+    double qz[3], qu[3], qv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      qz[j] = (nb[j] < 0) ? zeta_i : nzeta[j];
+      qu[j] = (nb[j] < 0) ? ui : dn[j].u;
+      qv[j] = (nb[j] < 0) ? vi : dn[j].v;
+    }
+    bj_limit(zeta_i, qz, rcx, rcy, gzx, gzy);
+    bj_limit(ui, qu, rcx, rcy, gux, guy);
+    bj_limit(vi, qv, rcx, rcy, gvx, gvy);
+  }
+  // bed_slope_VTX.f90:32-33
+  const double area = tri_area(tr);
+  const double dzx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                             tr.zv[2] * (tr.y[0] - tr.y[1])) / area;
+  const double dzy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                             tr.zv[2] * (tr.x[1] - tr.x[0])) / area;
   double dhx = gzx - dzx, dhy = gzy - dzy;
   double up = ui, vp = vi;
   if (!dry_i) {
-    // hgradfix2.f90:49-96
-    bool setgrad = zeta_i < g.zhigh[i];
+    // hgradfix2.f90:49-96 (zvtx of the highest vertex: dataprepFVT.f90:293-306)
+    const double zhigh = dmax(dmax(tr.zv[0], tr.zv[1]), tr.zv[2]);
+    bool setgrad = zeta_i < zhigh;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int l = g.cnbr[k * N + i];
-      if (l < 0) continue;
-      if (S.dry_in[l] && g.z[l] < zeta_i) setgrad = false;
+      if (nb[k] < 0) continue;
+      if (dn[k].dry != ZERO && sn[k].z < zeta_i) setgrad = false;
     }
     if (setgrad) {
       dhx = -dzx; dhy = -dzy;
       up = ZERO; vp = ZERO;
-      double* sl = const_cast<double*>(st.hin);  // slot s-1: zero its hu, hv
-      sl[N + i] = ZERO;
-      sl[2 * N + i] = ZERO;
+      st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1)
+      st.hin[2 * N + i] = ZERO;
       gux = ZERO; guy = ZERO; gvx = ZERO; gvy = ZERO;
     }
     // to_vertices.f90:44-55
-    const bool flag = S.vwet[g.cvtx[i]] && S.vwet[g.cvtx[N + i]] && S.vwet[g.cvtx[2 * N + i]];
-    if (!flag) { up = ZERO; vp = ZERO; }
+    if (!vw) { up = ZERO; vp = ZERO; }
   }
-  S.grad[i] = dhx; S.grad[N + i] = dhy;
-  S.grad[2 * N + i] = gux; S.grad[3 * N + i] = guy;
-  S.grad[4 * N + i] = gvx; S.grad[5 * N + i] = gvy;
-  S.up[i] = up;
-  S.vp[i] = vp;
+  // face records: interior sides as invisc_fluxes.f90:53-81 forms them (zero
+  // for a dry cell, depth clipped at 0), boundary sides unclipped
+  CellF* F = &S.f[i];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double fh = hi + dhx * rcx[j] + dhy * rcy[j];
+    double fu = up + gux * rcx[j] + guy * rcy[j];
+    double fv = vp + gvx * rcx[j] + gvy * rcy[j];
+    if (nb[j] >= 0) {
+      if (dry_i) { fh = ZERO; fu = ZERO; fv = ZERO; }
+      else if (fh < ZERO) fh = ZERO;
+    }
+    st4(reinterpret_cast<double*>(&F->s[j]), fh, fu, fv, zi);
+  }
+  const double umag = dry_i ? -ONE : sqrt(ui * ui + vi * vi);
+  st4(&F->dhx, dhx, dhy, umag, hi);
+  if (o.friction == 1) {
+    S.up[i] = up;
+    S.vp[i] = vp;
+  }
   if (o.steady && st.last && i == g.last_cell) {
     // the local dt of the reference's last cell, which fric_terms_impl uses
     // (solverFVT.f90:260-265,336)
-    double umag = ZERO;
-    bool d = find_dry(hi, ui, vi, o.h_dry, o.h_wet, &umag);
+    double umg = ZERO;
+    bool d = find_dry(hi, ui, vi, o.h_dry, o.h_wet, &umg);
     const double wc = d ? ZERO : sqrt(G * hi);
-    double dd = umag + wc;
+    double dd = umg + wc;
     if (dd < SMALL) dd = 1.044;
-    S.ctrl->delt_last = o.cfl * g.area[i] / (dd * g.perim[i]);
+    const double perim = s0.len + s1.len + s2.len;
+    S.ctrl->delt_last = o.cfl * area / (dd * perim);
   }
-}
-
-// face value of cell c at its local edge m: q + grad(q).rc(c,m)
-struct Face { double h, u, v; };
-__device__ __forceinline__ Face face(const Geo& g, const State& S, const double* h, int c, int m) {
-  const int N = g.N;
-  const double rx = g.rc[(2 * m) * N + c], ry = g.rc[(2 * m + 1) * N + c];
-  Face f;
-  f.h = h[c] + S.grad[c] * rx + S.grad[N + c] * ry;
-  f.u = S.up[c] + S.grad[2 * N + c] * rx + S.grad[3 * N + c] * ry;
-  f.v = S.vp[c] + S.grad[4 * N + c] * rx + S.grad[5 * N + c] * ry;
-  return f;
 }
 
 // ---------------------------------------------------------------- inflowQbyH
 // inflowQbyH.f90:31-126 (vtype 1) / :221-316 (vtype 5): the fixed point on
-// the string's wetted length, one block per string.  Sums run in string
-// order (per-edge terms formed in parallel, accumulated by one lane).
-constexpr int BCQ_THREADS = 256;
+// the string's wetted length, one wavefront per string.  Each lane holds the
+// face state of its edges; every sum runs over the edges in string order,
+// read lane by lane, so it is the reference's sequential sum.
+constexpr int BCQ_THREADS = 64;
+constexpr int BCQ_MAX = 8;  // edges per lane: strings of up to 512 edges
+
+__device__ __forceinline__ double lane_read(double x, int k) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), k);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __global__ void __launch_bounds__(BCQ_THREADS)
 k_bcq(Geo g, Opts o, Stage st, State S) {
   const int ch = blockIdx.x;
   Ctrl* c = S.ctrl;
-  if (c->done) return;
   const int vt = g.in_type[ch];
   if (vt != 1 && vt != 5) return;
+  if (c->done) return;
   const int t = threadIdx.x;
-  const int e0 = g.qs_ptr[ch], e1 = g.qs_ptr[ch + 1];
+  const int e0 = g.qs_ptr[ch], n = g.qs_ptr[ch + 1] - e0;
   const double qin = c->qin[ch];
   if (qin < DBL_MIN) { if (t == 0) c->qskip[ch] = 1; return; }
-  __shared__ double s_a[BCQ_THREADS], s_q[BCQ_THREADS];
-  __shared__ char s_ok[BCQ_THREADS];
-  __shared__ double s_atotal, s_new, s_qt;
-  __shared__ int s_null, s_exit;
   const double h_dry = o.h_dry;
-  const double* h = st.hin;
-  // initial wetted length
-  if (t == 0) s_atotal = ZERO;
-  __syncthreads();
-  for (int base = e0; base < e1; base += BCQ_THREADS) {
-    const int l = base + t;
-    s_ok[t] = 0;
-    if (l < e1) {
-      const int i = g.qs_cell[l], m = g.qs_slot[l];
-      if (!(vt == 5 && S.dry_in[i])) {
-        double hl;
-        if (S.dry_in[i]) hl = ZERO;
-        else {
-          hl = h[i] + S.grad[i] * g.rc[(2 * m) * g.N + i] + S.grad[g.N + i] * g.rc[(2 * m + 1) * g.N + i];
-          hl = dmax(hl, ZERO);
-        }
-        s_a[t] = hl * g.elen[m * g.N + i];
-        s_ok[t] = 1;
-      }
+  __shared__ double s_a[BCQ_THREADS * BCQ_MAX], s_q[BCQ_THREADS * BCQ_MAX];
+  __shared__ int s_ok[BCQ_THREADS * BCQ_MAX];
+  double fh[BCQ_MAX], fu[BCQ_MAX], fv[BCQ_MAX], nx[BCQ_MAX], ny[BCQ_MAX], L[BCQ_MAX];
+  bool act[BCQ_MAX];
+#pragma unroll
+  for (int r = 0; r < BCQ_MAX; ++r) {
+    const int l = t + 64 * r;
+    act[r] = false; fh[r] = fu[r] = fv[r] = nx[r] = ny[r] = L[r] = ZERO;
+    if (l < n) {
+      const int e = e0 + l;
+      const int i = g.qs_cell[e], m = g.qs_slot[e];
+      const FSlot fs = ld(&S.f[i].s[m]);
+      const bool d = S.f[i].umag < ZERO;
+      act[r] = !(vt == 5 && d);
+      fh[r] = fs.h; fu[r] = fs.u; fv[r] = fs.v;
+      nx[r] = g.qs_nx[e]; ny[r] = g.qs_ny[e]; L[r] = g.qs_len[e];
+      const double hl = d ? ZERO : dmax(fs.h, ZERO);
+      s_a[l] = hl * L[r];
+      s_ok[l] = act[r];
     }
-    __syncthreads();
-    if (t == 0) {
-      double a = s_atotal;
-      for (int k = 0; k < BCQ_THREADS && base + k < e1; ++k)
-        if (s_ok[k]) a = a + s_a[k];
-      s_atotal = a;
-    }
-    __syncthreads();
   }
-  if (t == 0) { s_null = equals(s_atotal, ZERO); s_exit = 0; }
   __syncthreads();
+  // initial wetted length (inflowQbyH.f90:40-56), summed in string order
+  double atotal = ZERO;
+  for (int k = 0; k < n; ++k)
+    if (s_ok[k]) atotal = atotal + s_a[k];
+  const bool null_area = equals(atotal, ZERO);
   int counter = 0;
+  double newatotal, qtotal;
   for (;;) {
-    if (t == 0) { s_new = ZERO; s_qt = ZERO; }
     __syncthreads();
-    const double qfrac = qin / s_atotal;
-    for (int base = e0; base < e1; base += BCQ_THREADS) {
-      const int l = base + t;
-      s_ok[t] = 0;
-      if (l < e1) {
-        const int i = g.qs_cell[l], m = g.qs_slot[l];
-        if (!(vt == 5 && S.dry_in[i])) {
-          Face f = face(g, S, h, i, m);
-          double hl = f.h, ue, ve;
-          if (s_null) hl = g.bch_max;
-          if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
-          else { ue = f.u; ve = f.v; }
-          const double nx = g.enx[m * g.N + i], ny = g.eny[m * g.N + i], L = g.elen[m * g.N + i];
-          double ur = -qfrac;
-          const double ul = ue * nx + ve * ny;
-          const double sq = sqrt(G * hl);
-          double hr = 2.0 * sq + ul - ur;
-          hr = hr * hr / 4.0 / G;
-          ur = -qfrac * nx;
-          const double vr = -qfrac * ny;
-          if (!(hr < h_dry)) {
-            s_a[t] = hr * L;
-            s_q[t] = (ur * nx + vr * ny) * hr * L;
-            s_ok[t] = 1;
-          }
+    const double qfrac = qin / atotal;
+#pragma unroll
+    for (int r = 0; r < BCQ_MAX; ++r) {
+      const int l = t + 64 * r;
+      if (l >= n) continue;
+      int ok = 0;
+      if (act[r]) {
+        double hl = fh[r], ue, ve;
+        if (null_area) hl = g.bch_max;
+        if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
+        else { ue = fu[r]; ve = fv[r]; }
+        double ur = -qfrac;
+        const double ul = ue * nx[r] + ve * ny[r];
+        const double sq = sqrt(G * hl);
+        double hr = 2.0 * sq + ul - ur;
+        hr = hr * hr / 4.0 / G;
+        ur = -qfrac * nx[r];
+        const double vr = -qfrac * ny[r];
+        if (!(hr < h_dry)) {
+          s_a[l] = hr * L[r];
+          s_q[l] = (ur * nx[r] + vr * ny[r]) * hr * L[r];
+          ok = 1;
         }
       }
-      __syncthreads();
-      if (t == 0) {
-        double a = s_new, q = s_qt;
-        for (int k = 0; k < BCQ_THREADS && base + k < e1; ++k)
-          if (s_ok[k]) { a = a + s_a[k]; q = q + s_q[k]; }
-        s_new = a; s_qt = q;
-      }
-      __syncthreads();
-    }
-    if (t == 0) {
-      const double qt = fabs(s_qt);
-      if (fabs(qt - qin) / qin < 0.005 || counter > 50) s_exit = 1;
-      else s_atotal = s_new;
+      s_ok[l] = ok;
     }
     __syncthreads();
-    if (s_exit) break;
+    newatotal = ZERO;
+    qtotal = ZERO;
+    for (int k = 0; k < n; ++k)
+      if (s_ok[k]) { newatotal = newatotal + s_a[k]; qtotal = qtotal + s_q[k]; }
+    qtotal = fabs(qtotal);
+    if (fabs(qtotal - qin) / qin < 0.005 || counter > 50) break;
     counter++;
+    atotal = newatotal;
   }
   if (t == 0) {
-    c->atotal[ch] = s_new;
-    c->null_area[ch] = s_null;
+    c->atotal[ch] = newatotal;
+    c->null_area[ch] = null_area;
     c->qskip[ch] = 0;
   }
 }
@@ -529,265 +682,315 @@ k_bcq(Geo g, Opts o, Stage st, State S) {
 // ---------------------------------------------------------------- update
 constexpr int UPD_THREADS = 256;
 
+// Flux through one boundary side of cell i (the routines of
+// invisc_fluxes.f90:195-238), as the change of phi(i,:).
+__device__ void boundary_flux(const Geo& g, const Opts& o, const State& S, const Ctrl* c,
+                              uint32_t bits, const Side& sd, double fh, double fu, double fv,
+                              double hi, bool dry_i, int p1, int p2, double& d0, double& d1,
+                              double& d2) {
+  const uint32_t kind = bits & CMS_KIND;
+  const double h_dry = o.h_dry;
+  const double nx = sd.nx, ny = sd.ny, L = sd.len;
+  d0 = ZERO; d1 = ZERO; d2 = ZERO;
+  if (kind == SK_WALL) {
+    if (dry_i) return;
+    if (o.btype == 0) {
+      // wenslp3_fluxes.f90:40-61
+      const double h1 = S.hvtx[p1], h2 = S.hvtx[p2];
+      double he = sqrt((h1 * h1 + h1 * h2 + h2 * h2) * ONE_THIRD);
+      if (he < ZERO) he = ZERO;
+      if (he < h_dry) return;
+      d1 = -(HALF * G * he * he * nx * L);
+      d2 = -(HALF * G * he * he * ny * L);
+    } else {
+      // weslp2_fluxes.f90:40-75
+      if (fh < h_dry) return;
+      double urr = fu * sd.tx + fv * sd.ty;
+      urr = urr + 2.0 * sqrt(G * fh);
+      const double hrr = urr * urr / 4.0 / G;
+      const double f1 = HALF * G * hrr * hrr * nx, f2 = HALF * G * hrr * hrr * ny;
+      const double dh = ZERO;
+      d1 = -((f1 + dh * nx) * L);
+      d2 = -((f2 + dh * ny) * L);
+    }
+    return;
+  }
+  const int ch = (bits >> CMS_CHAIN) & 0xFF;
+  if (kind == SK_IN_Q1 || kind == SK_IN_Q5) {
+    // inflowQbyH.f90:131-210 (vtype 1) / :321-400 (vtype 5)
+    if (c->qskip[ch]) return;
+    if (kind == SK_IN_Q5 && dry_i) return;
+    const double qin = c->qin[ch];
+    const double atotal = c->atotal[ch];
+    double hl = fh, ue, ve;
+    if (c->null_area[ch]) hl = g.bch_max;
+    if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
+    else { ue = fu; ve = fv; }
+    const double qfrac = qin / atotal;
+    double urr = -qfrac;
+    const double ul = ue * nx + ve * ny;
+    const double sq = sqrt(G * hl);
+    double hr = 2.0 * sq + ul - urr;
+    hr = hr * hr / 4.0 / G;
+    urr = -qfrac * nx;
+    const double vrr = -qfrac * ny;
+    if (hr < h_dry) return;
+    const double f0 = hr * (urr * nx + vrr * ny);
+    const double f1 = hr * urr * (urr * nx + vrr * ny) + HALF * G * hr * hr * nx;
+    const double f2 = hr * vrr * (urr * nx + vrr * ny) + HALF * G * hr * hr * ny;
+    d0 = -(f0 * L); d1 = -(f1 * L); d2 = -(f2 * L);
+  } else if (kind == SK_IN_H) {
+    // inflowH.f90:31-91
+    double hl = dmax(fh, ZERO), ue, ve;
+    if (hi < h_dry) { ue = ZERO; ve = ZERO; } else { ue = fu; ve = fv; }
+    const double hr = dmax(ZERO, c->qin[ch] - sd.zm);
+    const double ul = ue * nx + ve * ny;
+    const double sl = sqrt(G * hl), sr = sqrt(G * hr);
+    double urr = ul + (sl - sr);
+    const double vrr = urr * ny;
+    urr = urr * nx;
+    if (hl < h_dry && hr < h_dry) return;
+    double f[3];
+    flux_ac(nx, ny, hl, ue, ve, hr, urr, vrr, f);
+    d0 = -(f[0] * L); d1 = -(f[1] * L); d2 = -(f[2] * L);
+  } else if (kind == SK_OUT_H) {
+    // outflowbyH.f90:33-135
+    double hl = fh, ue, ve;
+    if (dry_i || hl < h_dry) { ue = ZERO; ve = ZERO; hl = dmax(hl, ZERO); }
+    else { ue = fu; ve = fv; }
+    double hr = dmax(ZERO, c->hbc[ch] - sd.zm);
+    if (hl < h_dry && hr < h_dry) return;
+    double ul, vl, urr, vrr;
+    if (hr < h_dry) { urr = ZERO; ul = ue; vrr = ZERO; vl = ve; }
+    else {
+      ul = ue * nx + ve * ny;
+      vl = ue * sd.tx + ve * sd.ty;
+      const double sl = sqrt(G * hl), sr = sqrt(G * hr);
+      urr = ul + (sl - sr);
+      vrr = vl;
+      ul = ue;
+      vl = ve;
+      const double a = urr, b = vrr;
+      urr = a * nx + b * sd.tx;
+      vrr = a * ny + b * sd.ty;
+    }
+    bool set_wall = false;
+    if (o.hvalve) {
+      const double dot = urr * nx + vrr * ny;
+      if (dot < ZERO) { set_wall = true; hr = hl; urr = ZERO; vrr = ZERO; }
+    }
+    double f[3];
+    flux_ac(nx, ny, hl, ul, vl, hr, urr, vrr, f);
+    if (set_wall) f[0] = ZERO;
+    d0 = -(f[0] * L); d1 = -(f[1] * L); d2 = -(f[2] * L);
+  } else if (kind == SK_OUT_CRIT) {
+    // crit_outflow.f90:32-105
+    double hl = fh, ue, ve;
+    if (dry_i || hl < h_dry) { ue = ZERO; ve = ZERO; hl = dmax(hl, ZERO); }
+    else { ue = fu; ve = fv; }
+    if (hl < h_dry) return;
+    const double ul = ue * nx + ve * ny, vl = ue * sd.tx + ve * sd.ty;
+    double urr = ul, vrr = vl;
+    double hr = urr * urr / G;
+    const double a = urr, b = vrr;
+    urr = a * nx + b * sd.tx;
+    vrr = a * ny + b * sd.ty;
+    if (o.hvalve) {
+      const double dot = urr * nx + vrr * ny;
+      if (dot < ZERO) { hr = hl; urr = ZERO; vrr = ZERO; }
+    }
+    const double f0 = hr * (urr * nx + vrr * ny);
+    const double f1 = hr * urr * (urr * nx + vrr * ny) + HALF * G * hr * hr * nx;
+    const double f2 = hr * vrr * (urr * nx + vrr * ny) + HALF * G * hr * hr * ny;
+    const double dh = ZERO;
+    d0 = -(f0 * L); d1 = -((f1 + dh * nx) * L); d2 = -((f2 + dh * ny) * L);
+  } else if (kind == SK_OUT_FREE) {
+    // free_flow.f90:33-90
+    const double hl = dmax(fh, ZERO);
+    if (hl < h_dry) return;
+    double ue, ve;
+    if (hi < h_dry) { ue = ZERO; ve = ZERO; } else { ue = fu; ve = fv; }
+    double urr = ue, vrr = ve;
+    bool set_wall = false;
+    if (o.hvalve) {
+      const double dot = urr * nx + vrr * ny;
+      if (dot < 0) { set_wall = true; urr = ZERO; vrr = ZERO; }
+    }
+    double f[3];
+    flux_ac(nx, ny, hl, ue, ve, hl, urr, vrr, f);
+    if (set_wall) f[0] = ZERO;
+    d0 = -(f[0] * L); d1 = -(f[1] * L); d2 = -(f[2] * L);
+  }
+}
+
+// Neighbour data an interior side needs, gathered before any flux work: its
+// face state at the shared side with its bed, its depth and dry flag.
+struct Nbr {
+  FSlot f;
+  double h;
+  bool dry;
+};
+
+__device__ __forceinline__ Nbr load_nbr(const State& S, int nb, int nl) {
+  Nbr n;
+  n.f = ld(&S.f[nb].s[nl]);
+  const double2 t = *reinterpret_cast<const double2*>(&S.f[nb].umag);
+  n.dry = t.x < ZERO;
+  n.h = t.y;
+  return n;
+}
+
+// invisc_fluxes.f90:39-166 for one interior side of cell i: the flux of the
+// edge computed exactly as the reference does for it (left state = its e(1)
+// cell, normal out of e(1)), returned as cell i's change of phi.
+__device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, const Side& sd,
+                                              bool dry_i, double hi, double zi, double fho,
+                                              double fuo, double fvo, const Nbr& nb, double& d0,
+                                              double& d1, double& d2) {
+  d0 = ZERO; d1 = ZERO; d2 = ZERO;
+  if (dry_i && nb.dry) return;
+  const double h_dry = o.h_dry, h_wet = o.h_wet;
+  const bool left = (bits >> CMS_LEFT) & 1;
+  double hl = left ? fho : nb.f.h, ul = left ? fuo : nb.f.u, vl = left ? fvo : nb.f.v;
+  double hr = left ? nb.f.h : fho, ur = left ? nb.f.u : fuo, vr = left ? nb.f.v : fvo;
+  const bool dL = left ? dry_i : nb.dry, dR = left ? nb.dry : dry_i;
+  const double zL = left ? zi : nb.f.z, zR = left ? nb.f.z : zi;
+  const double wL = left ? hi : nb.h, wR = left ? nb.h : hi;
+  const double nx = left ? sd.nx : -sd.nx, ny = left ? sd.ny : -sd.ny;
+  if (hl < h_dry && hr < h_dry) return;
+  bool set_wall = false;
+  if (dL || dR) {
+    const double h0 = dmax(hr, hl) + sd.zm;
+    if (dL && zL + h_wet > h0) { set_wall = true; hl = hr; }
+    if (dR && zR + h_wet > h0) { set_wall = true; hr = hl; }
+  }
+  double f[3];
+  if (set_wall) {
+    f[0] = ZERO;
+    hl = dmax(hl, hr);
+    f[1] = HALF * G * hl * hl * nx;
+    f[2] = HALF * G * hl * hl * ny;
+  } else if (o.flux == STORM_FLUX_AC) {
+    flux_ac(nx, ny, hl, ul, vl, hr, ur, vr, f);
+  } else if (o.flux == STORM_FLUX_RUSANOV) {
+    const double cl = dL ? ZERO : sqrt(G * wL), cr = dR ? ZERO : sqrt(G * wR);
+    flux_rusanov(nx, ny, hl, ul, vl, cl, hr, ur, vr, cr, f);
+  } else if (o.flux == STORM_FLUX_HLL) {
+    flux_hll(nx, ny, hl, ul, vl, hr, ur, vr, h_dry, f);
+  } else {
+    flux_bgnvc(nx, ny, hl, ul, vl, hr, ur, vr, f);
+  }
+  const double L = sd.len;
+  if (left) { d0 = -(f[0] * L); d1 = -(f[1] * L); d2 = -(f[2] * L); }
+  else { d0 = f[0] * L; d1 = f[1] * L; d2 = f[2] * L; }
+}
+
 __global__ void __launch_bounds__(UPD_THREADS)
 k_update(Geo g, Opts o, Stage st, State S, double* part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   const Ctrl* c = S.ctrl;
   const bool done = c->done;
   const int N = g.N;
   double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX;
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (i < N && !done) {
-    const double* h = st.hin;
-    const bool dry_i = S.dry_in[i];
-    const double hi = h[i];
-    double p0 = ZERO, p1 = ZERO, p2 = ZERO;
-    const uint8_t ord = g.corder[i];
-#pragma unroll 1
-    for (int tt = 0; tt < 3; ++tt) {
-      const int j = (ord >> (2 * tt)) & 3;
-      const uint32_t info = g.cinfo[j * N + i];
-      const uint32_t kind = info & CI_KIND_MASK;
-      const double nx = g.enx[j * N + i], ny = g.eny[j * N + i], L = g.elen[j * N + i];
-      if (kind == SK_INTERIOR) {
-        // invisc_fluxes.f90:39-166
-        const int nb = g.cnbr[j * N + i];
-        const bool dry_nb = S.dry_in[nb];
-        if (dry_i && dry_nb) continue;
-        const bool left = info & CI_LEFT;
-        const int nl = (info >> CI_NBRLOC_SHIFT) & 3;
-        const int cL = left ? i : nb, mL = left ? j : nl;
-        const int cR = left ? nb : i, mR = left ? nl : j;
-        const bool dL = left ? dry_i : dry_nb, dR = left ? dry_nb : dry_i;
-        double hl, ul, vl, hr, ur, vr;
-        if (dL) { hl = ZERO; ul = ZERO; vl = ZERO; }
-        else {
-          Face f = face(g, S, h, cL, mL);
-          hl = f.h; ul = f.u; vl = f.v;
-          if (hl < ZERO) hl = ZERO;
-        }
-        if (dR) { hr = ZERO; ur = ZERO; vr = ZERO; }
-        else {
-          Face f = face(g, S, h, cR, mR);
-          hr = f.h; ur = f.u; vr = f.v;
-          if (hr < ZERO) hr = ZERO;
-        }
-        if (hl < h_dry && hr < h_dry) continue;
-        bool set_wall = false;
-        if (dL || dR) {
-          const double h0 = dmax(hr, hl) + g.ezmid[j * N + i];
-          if (dL && g.z[cL] + h_wet > h0) { set_wall = true; hl = hr; }
-          if (dR && g.z[cR] + h_wet > h0) { set_wall = true; hr = hl; }
-        }
-        double f[3];
-        if (set_wall) {
-          f[0] = ZERO;
-          hl = dmax(hl, hr);
-          f[1] = HALF * G * hl * hl * nx;
-          f[2] = HALF * G * hl * hl * ny;
-        } else if (o.flux == STORM_FLUX_AC) {
-          flux_ac(nx, ny, hl, ul, vl, hr, ur, vr, f);
-        } else if (o.flux == STORM_FLUX_RUSANOV) {
-          const double wl = dL ? ZERO : sqrt(G * h[cL]), wr = dR ? ZERO : sqrt(G * h[cR]);
-          flux_rusanov(nx, ny, hl, ul, vl, wl, hr, ur, vr, wr, f);
-        } else if (o.flux == STORM_FLUX_HLL) {
-          flux_hll(nx, ny, hl, ul, vl, hr, ur, vr, h_dry, f);
-        } else {
-          flux_bgnvc(nx, ny, hl, ul, vl, hr, ur, vr, f);
-        }
-        if (left) { p0 = p0 - f[0] * L; p1 = p1 - f[1] * L; p2 = p2 - f[2] * L; }
-        else { p0 = p0 + f[0] * L; p1 = p1 + f[1] * L; p2 = p2 + f[2] * L; }
-      } else if (kind == SK_WALL) {
-        if (dry_i) continue;
-        if (o.btype == 0) {
-          // wenslp3_fluxes.f90:40-61
-          const double h1 = S.hvtx[g.ep1[j * N + i]], h2 = S.hvtx[g.ep2[j * N + i]];
-          double he = sqrt((h1 * h1 + h1 * h2 + h2 * h2) * ONE_THIRD);
-          if (he < ZERO) he = ZERO;
-          if (he < h_dry) continue;
-          p1 = p1 - HALF * G * he * he * nx * L;
-          p2 = p2 - HALF * G * he * he * ny * L;
-        } else {
-          // weslp2_fluxes.f90:40-75
-          Face f = face(g, S, h, i, j);
-          if (f.h < h_dry) continue;
-          const double tx = g.etx[j * N + i], ty = g.ety[j * N + i];
-          double urr = f.u * tx + f.v * ty;
-          urr = urr + 2.0 * sqrt(G * f.h);
-          const double hrr = urr * urr / 4.0 / G;
-          const double f1 = HALF * G * hrr * hrr * nx, f2 = HALF * G * hrr * hrr * ny;
-          const double dh = ZERO;
-          p1 = p1 - (f1 + dh * nx) * L;
-          p2 = p2 - (f2 + dh * ny) * L;
-        }
-      } else if (kind == SK_IN_Q1 || kind == SK_IN_Q5) {
-        // inflowQbyH.f90:131-210 (vtype 1) / :321-400 (vtype 5)
-        const int ch = (info >> CI_CHAIN_SHIFT) & 0xFF;
-        if (c->qskip[ch]) continue;
-        if (kind == SK_IN_Q5 && dry_i) continue;
-        const double qin = c->qin[ch];
-        const double atotal = c->atotal[ch];
-        Face fc = face(g, S, h, i, j);
-        double hl = fc.h, ue, ve;
-        if (c->null_area[ch]) hl = g.bch_max;
-        if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
-        else { ue = fc.u; ve = fc.v; }
-        const double qfrac = qin / atotal;
-        double urr = -qfrac;
-        const double ul = ue * nx + ve * ny;
-        const double sq = sqrt(G * hl);
-        double hr = 2.0 * sq + ul - urr;
-        hr = hr * hr / 4.0 / G;
-        urr = -qfrac * nx;
-        const double vrr = -qfrac * ny;
-        if (hr < h_dry) continue;
-        const double f0 = hr * (urr * nx + vrr * ny);
-        const double f1 = hr * urr * (urr * nx + vrr * ny) + HALF * G * hr * hr * nx;
-        const double f2 = hr * vrr * (urr * nx + vrr * ny) + HALF * G * hr * hr * ny;
-        p0 = p0 - f0 * L; p1 = p1 - f1 * L; p2 = p2 - f2 * L;
-      } else if (kind == SK_IN_H) {
-        // inflowH.f90:31-91
-        const int ch = (info >> CI_CHAIN_SHIFT) & 0xFF;
-        Face fc = face(g, S, h, i, j);
-        double hl = dmax(fc.h, ZERO), ue, ve;
-        if (hi < h_dry) { ue = ZERO; ve = ZERO; } else { ue = fc.u; ve = fc.v; }
-        const double hr = dmax(ZERO, c->qin[ch] - g.ezmid[j * N + i]);
-        const double ul = ue * nx + ve * ny;
-        const double sl = sqrt(G * hl), sr = sqrt(G * hr);
-        double urr = ul + (sl - sr);
-        const double vrr = urr * ny;
-        urr = urr * nx;
-        if (hl < h_dry && hr < h_dry) continue;
-        double f[3];
-        flux_ac(nx, ny, hl, ue, ve, hr, urr, vrr, f);
-        p0 = p0 - f[0] * L; p1 = p1 - f[1] * L; p2 = p2 - f[2] * L;
-      } else if (kind == SK_OUT_H) {
-        // outflowbyH.f90:33-135
-        const int ch = (info >> CI_CHAIN_SHIFT) & 0xFF;
-        Face fc = face(g, S, h, i, j);
-        double hl = fc.h, ue, ve;
-        if (dry_i || hl < h_dry) { ue = ZERO; ve = ZERO; hl = dmax(hl, ZERO); }
-        else { ue = fc.u; ve = fc.v; }
-        const double tx = g.etx[j * N + i], ty = g.ety[j * N + i];
-        double hr = dmax(ZERO, c->hbc[ch] - g.ezmid[j * N + i]);
-        if (hl < h_dry && hr < h_dry) continue;
-        double ul, vl, urr, vrr;
-        if (hr < h_dry) { urr = ZERO; ul = ue; vrr = ZERO; vl = ve; }
-        else {
-          ul = ue * nx + ve * ny;
-          vl = ue * tx + ve * ty;
-          const double sl = sqrt(G * hl), sr = sqrt(G * hr);
-          urr = ul + (sl - sr);
-          vrr = vl;
-          ul = ue;
-          vl = ve;
-          const double a = urr, b = vrr;
-          urr = a * nx + b * tx;
-          vrr = a * ny + b * ty;
-        }
-        bool set_wall = false;
-        if (o.hvalve) {
-          const double dot = urr * nx + vrr * ny;
-          if (dot < ZERO) { set_wall = true; hr = hl; urr = ZERO; vrr = ZERO; }
-        }
-        double f[3];
-        flux_ac(nx, ny, hl, ul, vl, hr, urr, vrr, f);
-        if (set_wall) f[0] = ZERO;
-        p0 = p0 - f[0] * L; p1 = p1 - f[1] * L; p2 = p2 - f[2] * L;
-      } else if (kind == SK_OUT_CRIT) {
-        // crit_outflow.f90:32-105
-        Face fc = face(g, S, h, i, j);
-        double hl = fc.h, ue, ve;
-        if (dry_i || hl < h_dry) { ue = ZERO; ve = ZERO; hl = dmax(hl, ZERO); }
-        else { ue = fc.u; ve = fc.v; }
-        if (hl < h_dry) continue;
-        const double tx = g.etx[j * N + i], ty = g.ety[j * N + i];
-        const double ul = ue * nx + ve * ny, vl = ue * tx + ve * ty;
-        double urr = ul, vrr = vl;
-        double hr = urr * urr / G;
-        const double a = urr, b = vrr;
-        urr = a * nx + b * tx;
-        vrr = a * ny + b * ty;
-        if (o.hvalve) {
-          const double dot = urr * nx + vrr * ny;
-          if (dot < ZERO) { hr = hl; urr = ZERO; vrr = ZERO; }
-        }
-        const double f0 = hr * (urr * nx + vrr * ny);
-        const double f1 = hr * urr * (urr * nx + vrr * ny) + HALF * G * hr * hr * nx;
-        const double f2 = hr * vrr * (urr * nx + vrr * ny) + HALF * G * hr * hr * ny;
-        const double dh = ZERO;
-        p0 = p0 - f0 * L; p1 = p1 - (f1 + dh * nx) * L; p2 = p2 - (f2 + dh * ny) * L;
-      } else if (kind == SK_OUT_FREE) {
-        // free_flow.f90:33-90
-        Face fc = face(g, S, h, i, j);
-        const double hl = dmax(fc.h, ZERO);
-        if (hl < h_dry) continue;
-        double ue, ve;
-        if (hi < h_dry) { ue = ZERO; ve = ZERO; } else { ue = fc.u; ve = fc.v; }
-        double urr = ue, vrr = ve;
-        bool set_wall = false;
-        if (o.hvalve) {
-          const double dot = urr * nx + vrr * ny;
-          if (dot < 0) { set_wall = true; urr = ZERO; vrr = ZERO; }
-        }
-        double f[3];
-        flux_ac(nx, ny, hl, ue, ve, hl, urr, vrr, f);
-        if (set_wall) f[0] = ZERO;
-        p0 = p0 - f[0] * L; p1 = p1 - f[1] * L; p2 = p2 - f[2] * L;
+    // own records
+    const uint64_t meta = g.cmeta[i];
+    const int nb0 = g.cnbr[i], nb1 = g.cnbr[N + i], nb2 = g.cnbr[2 * N + i];
+    const CellF* F = &S.f[i];
+    const FSlot f0 = ld(&F->s[0]), f1 = ld(&F->s[1]), f2 = ld(&F->s[2]);
+    const double4 tail = *reinterpret_cast<const double4*>(&F->dhx);
+    const CellS si = ld(&g.cs[i]);
+    const Tri tr = load_tri(g, i);
+    // neighbour data of the interior sides, gathered up front
+    const uint32_t b0 = slot_bits(meta, 0), b1 = slot_bits(meta, 1), b2 = slot_bits(meta, 2);
+    Nbr n0{}, n1{}, n2{};
+    if (nb0 >= 0) n0 = load_nbr(S, nb0, (b0 >> CMS_NL) & 3);
+    if (nb1 >= 0) n1 = load_nbr(S, nb1, (b1 >> CMS_NL) & 3);
+    if (nb2 >= 0) n2 = load_nbr(S, nb2, (b2 >> CMS_NL) & 3);
+    const double dhx = tail.x, dhy = tail.y;
+    const bool dry_i = tail.z < ZERO;
+    const double umag_in = dry_i ? ZERO : tail.z;  // |u| of the stage input (find_dry_cells)
+    const double hi = tail.w, zi = f0.z;
+    const double xci = si.xc, yci = si.yc;
+    const Side s0 = side<0>(tr), s1 = side<1>(tr), s2 = side<2>(tr);
+    const double area = tri_area(tr);
+    // per-side changes of phi
+    double c00, c01, c02, c10, c11, c12, c20, c21, c22;
+    auto side_flux = [&](uint32_t bits, const Side& sd, const FSlot& fo, const Nbr& nb, int j,
+                         double& d0, double& d1, double& d2) {
+      if ((bits & CMS_KIND) == SK_INTERIOR) {
+        interior_flux(o, bits, sd, dry_i, hi, zi, fo.h, fo.u, fo.v, nb, d0, d1, d2);
+      } else {
+        const int va = g.cvtx[j * N + i], vb = g.cvtx[((j + 1) % 3) * N + i];
+        const bool rev = (bits >> CMS_REV) & 1;
+        boundary_flux(g, o, S, c, bits, sd, fo.h, fo.u, fo.v, hi, dry_i, rev ? vb : va,
+                      rev ? va : vb, d0, d1, d2);
       }
+    };
+    side_flux(b0, s0, f0, n0, 0, c00, c01, c02);
+    side_flux(b1, s1, f1, n1, 1, c10, c11, c12);
+    side_flux(b2, s2, f2, n2, 2, c20, c21, c22);
+    // accumulate in the reference's order (invisc_fluxes: interior edges by
+    // number, walls, boundary routines)
+    double p0 = ZERO, p1 = ZERO, p2 = ZERO;
+#pragma unroll
+    for (int tt = 0; tt < 3; ++tt) {
+      const int j = (int)(meta >> (2 * tt)) & 3;
+      const double a0 = (j == 0) ? c00 : (j == 1) ? c10 : c20;
+      const double a1 = (j == 0) ? c01 : (j == 1) ? c11 : c21;
+      const double a2 = (j == 0) ? c02 : (j == 1) ? c12 : c22;
+      p0 = p0 + a0; p1 = p1 + a1; p2 = p2 + a2;
     }
-    const double ui = S.u[i], vi = S.v[i];  // stage-input velocities (find_dry_cells)
-    double umag_in = ZERO;
-    if (!dry_i) umag_in = sqrt(ui * ui + vi * vi);
     if (o.friction == 1) {
       // fric_terms.f90:24-98 (explicit Manning), after the edge fluxes
-      double r = dry_i ? ZERO : G * g.cd[i] * g.cd[i] / pow(hi, ONE_THIRD);
+      double r = dry_i ? ZERO : G * si.cd * si.cd / pow(hi, ONE_THIRD);
       r = -r * umag_in;
-      p1 = p1 + r * S.up[i] * g.area[i];
-      p2 = p2 + r * S.vp[i] * g.area[i];
+      p1 = p1 + r * S.up[i] * area;
+      p2 = p2 + r * S.vp[i] * area;
     }
     if (!dry_i) {
-      // bed_slope_STD.f90:23-70 with adjustZb.f90:41-70
-      const double dhx = S.grad[i], dhy = S.grad[N + i];
+      // bed_slope_STD.f90:23-70 with adjustZb.f90:41-70 and bed_slope_VTX.f90:32-33
       double hp[3];
       bool adjust = false;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        hp[k] = hi + dhx * g.vdx[k * N + i] + dhy * g.vdy[k * N + i];
+        hp[k] = hi + dhx * (tr.x[k] - xci) + dhy * (tr.y[k] - yci);
         if (hp[k] < ZERO) adjust = true;
       }
       double gx, gy;
       if (adjust) {
-        double zp[3], xx[3], yy[3];
+        double zp[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const int v = g.cvtx[k * N + i];
           hp[k] = dmin(ZERO, hp[k]);
-          zp[k] = g.zv3[k * N + i] + hp[k];
-          xx[k] = g.xv[v]; yy[k] = g.yv[v];
+          zp[k] = tr.zv[k] + hp[k];
         }
-        const double b0 = yy[1] - yy[2], c0 = xx[2] - xx[1];
-        const double b1 = yy[2] - yy[0], c1 = xx[0] - xx[2];
-        const double b2 = yy[0] - yy[1], c2 = xx[1] - xx[0];
-        const double a = HALF / g.area[i];
-        gx = a * (b0 * zp[0] + b1 * zp[1] + b2 * zp[2]);
-        gy = a * (c0 * zp[0] + c1 * zp[1] + c2 * zp[2]);
+        const double bb0 = tr.y[1] - tr.y[2], cc0 = tr.x[2] - tr.x[1];
+        const double bb1 = tr.y[2] - tr.y[0], cc1 = tr.x[0] - tr.x[2];
+        const double bb2 = tr.y[0] - tr.y[1], cc2 = tr.x[1] - tr.x[0];
+        const double a = HALF / area;
+        gx = a * (bb0 * zp[0] + bb1 * zp[1] + bb2 * zp[2]);
+        gy = a * (cc0 * zp[0] + cc1 * zp[1] + cc2 * zp[2]);
       } else {
-        gx = g.delzx[i];
-        gy = g.delzy[i];
+        gx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                     tr.zv[2] * (tr.y[0] - tr.y[1])) / area;
+        gy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                     tr.zv[2] * (tr.x[1] - tr.x[0])) / area;
       }
-      p1 = p1 - G * hi * gx * g.area[i];
-      p2 = p2 - G * hi * gy * g.area[i];
+      p1 = p1 - G * hi * gx * area;
+      p2 = p2 - G * hi * gy * area;
     }
+    const double perim = s0.len + s1.len + s2.len;
     // solverFVT.f90:259-283
     double delt = c->dt;
     if (o.steady) {
       const double wc = dry_i ? ZERO : sqrt(G * hi);
       double d = umag_in + wc;
       if (d < SMALL) d = 1.044;
-      delt = o.cfl * g.area[i] / (d * g.perim[i]);
+      delt = o.cfl * area / (d * perim);
     }
-    const double dt = delt / g.area[i];
+    const double dt = delt / area;
     double qh = ZERO, qu = ZERO, qv = ZERO;
     for (int jj = 0; jj < st.s; ++jj) {
       const double a = st.alpha[jj];
@@ -818,10 +1021,9 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       st.out[i] = qh;
       st.out[N + i] = qu;
       st.out[2 * N + i] = qv;
-      S.u[i] = un;
-      S.v[i] = vn;
       double um;
-      S.dry_out[i] = find_dry(hn, un, vn, h_dry, h_wet, &um);
+      const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um);
+      st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
     } else {
       // find_dry_cells + fric_terms_impl (solverFVT.f90:335-336)
       double um;
@@ -830,7 +1032,7 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       double rough = ZERO;
       if (!dpost) {
         const double fdt = o.steady ? c->delt_last : c->dt;
-        rough = G * g.cd[i] * g.cd[i] / pow(hn, ONE_THIRD);
+        rough = G * si.cd * si.cd / pow(hn, ONE_THIRD);
         rough = hn / (hn + fdt * rough * um);
       }
       if (o.friction == 0) {
@@ -842,21 +1044,15 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       const double dh = hn - h0n;
       top = dh * dh;
       bot = h0n * h0n;
-      // hvalve (solverFVT.f90:357-370)
+      // hvalve (solverFVT.f90:357-370): outward normal of the cell's side
       if (o.hvalve) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const uint32_t inf = g.cinfo[k * N + i];
-          if (!(inf & CI_HBAR)) continue;
-          const double dot = un * g.enx[k * N + i] + vn * g.eny[k * N + i];
-          if (dot < 0) { un = ZERO; vn = ZERO; }
-        }
+        if ((b0 >> CMS_HBAR) & 1) { if (un * s0.nx + vn * s0.ny < 0) { un = ZERO; vn = ZERO; } }
+        if ((b1 >> CMS_HBAR) & 1) { if (un * s1.nx + vn * s1.ny < 0) { un = ZERO; vn = ZERO; } }
+        if ((b2 >> CMS_HBAR) & 1) { if (un * s2.nx + vn * s2.ny < 0) { un = ZERO; vn = ZERO; } }
       }
       st.out[i] = qh;
       st.out[N + i] = qu;
       st.out[2 * N + i] = qv;
-      S.u[i] = un;
-      S.v[i] = vn;
       // phi reset at dry cells (solverFVT.f90:376-386)
       if (dpost) { p0 = ZERO; p1 = ZERO; p2 = ZERO; }
       S.phi[i] = p0; S.phi[N + i] = p1; S.phi[2 * N + i] = p2;
@@ -864,9 +1060,9 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       // stage-1 dry flags and CFL candidate of the next step
       double um2;
       const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um2);
-      S.dry_out[i] = dn;
+      st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
       const double wc = dn ? ZERO : sqrt(G * hn);
-      cand = o.cfl * g.area[i] / ((um2 + wc + SMALL) * g.perim[i]);
+      cand = o.cfl * area / ((um2 + wc + SMALL) * perim);
     }
   }
   if (!st.last || done) return;
@@ -900,13 +1096,13 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
 
 // to_vertices' cell pass alone (solverFVT.f90:483 -> to_vertices.f90:44-55),
 // after k_vertex, for the final state
-__global__ void k_final_cells(Geo g, const uint8_t* dry, const uint8_t* vwet, double* u, double* v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_final_cells(Geo g, const uint8_t* dry, const uint8_t* vwet, CellD* d) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= g.N) return;
-  if (dry[i]) return;
+  if (dry ? (dry[i] != 0) : (d[i].dry != ZERO)) return;
   const int N = g.N;
   const bool flag = vwet[g.cvtx[i]] && vwet[g.cvtx[N + i]] && vwet[g.cvtx[2 * N + i]];
-  if (!flag) { u[i] = ZERO; v[i] = ZERO; }
+  if (!flag) { d[i].u = ZERO; d[i].v = ZERO; }
 }
 
 // ---------------------------------------------------------------- host side
@@ -957,7 +1153,7 @@ struct storm_fvt {
   State S{};
   double* slots[4] = {nullptr, nullptr, nullptr, nullptr};  // physical slot buffers
   double *d_h0 = nullptr, *d_u0 = nullptr, *d_v0 = nullptr;
-  uint8_t *dry_a = nullptr, *dry_b = nullptr;
+  CellD *d_a = nullptr, *d_b = nullptr;  // stage state, double-buffered
   int* d_out_type = nullptr;
   int *d_qts_ptr = nullptr, *d_hts_ptr = nullptr;
   double *d_qts_t = nullptr, *d_qts_v = nullptr, *d_hts_t = nullptr, *d_hts_v = nullptr;
@@ -969,9 +1165,9 @@ struct storm_fvt {
   double last_run_s = 0.0;
   // RK order 1 keeps two physical slots and swaps them every step
   int r1_flip = 0;
-  // h input of the last stage run: to_vertices after the loop works on the
-  // water surface formed at the start of that stage (solverFVT.f90:169,483)
-  const double* last_hin = nullptr;
+  // input state of the last stage run: to_vertices after the loop works on
+  // the water surface formed at the start of that stage (solverFVT.f90:169,483)
+  const CellD* last_din = nullptr;
 
   ~storm_fvt() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -1049,32 +1245,24 @@ void setup_device(storm_fvt* C) {
   Geo& g = C->g;
   g.N = P.N; g.P = P.P; g.R = P.R;
   g.cvtx = dupload(own, P.cvtx); g.cnbr = dupload(own, P.cnbr);
-  g.ep1 = dupload(own, P.ep1); g.ep2 = dupload(own, P.ep2);
-  g.cinfo = dupload(own, P.cinfo); g.corder = dupload(own, P.corder);
-  g.rc = dupload(own, P.rc);
-  g.enx = dupload(own, P.enx); g.eny = dupload(own, P.eny);
-  g.etx = dupload(own, P.etx); g.ety = dupload(own, P.ety);
-  g.elen = dupload(own, P.elen); g.ezmid = dupload(own, P.ezmid);
-  g.area = dupload(own, P.area); g.perim = dupload(own, P.perim);
-  g.z = dupload(own, P.z); g.cd = dupload(own, P.cd);
-  g.delzx = dupload(own, P.delzx); g.delzy = dupload(own, P.delzy);
-  g.zhigh = dupload(own, P.zhigh);
-  g.vdx = dupload(own, P.vdx); g.vdy = dupload(own, P.vdy); g.zv3 = dupload(own, P.zv3);
+  g.cmeta = dupload(own, P.cmeta);
+  {
+    std::vector<CellS> cs(P.N);
+    for (int i = 0; i < P.N; ++i) cs[i] = CellS{P.xc[i], P.yc[i], P.z[i], P.cd[i]};
+    g.cs = dupload(own, cs);
+  }
+  g.vxyz = reinterpret_cast<const double4*>(dupload(own, P.vxyz));
   g.mptr = dupload(own, P.mptr); g.mcell = dupload(own, P.mcell);
-  g.mdx = dupload(own, P.mdx); g.mdy = dupload(own, P.mdy); g.mw = dupload(own, P.mw);
-  g.ai = dupload(own, P.ai); g.bi = dupload(own, P.bi); g.ci = dupload(own, P.ci);
-  g.det = dupload(own, P.det);
   g.gconn = dupload(own, P.gconn);
   g.gcx = dupload(own, P.gcx); g.gcy = dupload(own, P.gcy); g.gwt = dupload(own, P.gwt);
-  g.vptr = dupload(own, P.vptr); g.vcell = dupload(own, P.vcell);
-  g.vbc = dupload(own, P.vbc); g.vw = dupload(own, P.vw); g.zvtx = dupload(own, P.zvtx);
+  g.vptr = dupload(own, P.vptr); g.vcell = dupload(own, P.vcell); g.vbc = dupload(own, P.vbc);
   g.qs_ptr = dupload(own, P.qs_ptr); g.qs_cell = dupload(own, P.qs_cell);
   g.qs_slot = dupload(own, P.qs_slot);
+  g.qs_rcx = dupload(own, P.qs_rcx); g.qs_rcy = dupload(own, P.qs_rcy);
+  g.qs_nx = dupload(own, P.qs_nx); g.qs_ny = dupload(own, P.qs_ny);
+  g.qs_len = dupload(own, P.qs_len);
   g.in_type = dupload(own, P.in_type);
   g.n_in = P.n_in; g.n_out = P.n_out; g.last_cell = P.last_cell; g.bch_max = P.bch_max;
-  // node coordinates in internal vertex order (adjustZb's partially dry branch)
-  g.xv = dupload(own, P.xv_int);
-  g.yv = dupload(own, P.yv_int);
   upload_tables(C);
   Opts& o = C->o;
   const storm_fvt_opts_t& q = P.o;
@@ -1092,9 +1280,9 @@ void setup_device(storm_fvt* C) {
     throw StormErr(STORM_E_ARG, "block must be 64, 128 or 256");
   C->nblk_upd = (int)((N + UPD_THREADS - 1) / UPD_THREADS);
   State& S = C->S;
-  S.u = dalloc<double>(own, N); S.v = dalloc<double>(own, N);
-  S.up = dalloc<double>(own, N); S.vp = dalloc<double>(own, N);
-  S.grad = dalloc<double>(own, 6 * N);
+  S.up = dalloc<double>(own, q.friction == 1 ? N : 1);
+  S.vp = dalloc<double>(own, q.friction == 1 ? N : 1);
+  S.f = dalloc<CellF>(own, N);
   S.gwork = (q.gradient == STORM_GRAD_GAUSS) ? dalloc<double>(own, 6 * N) : nullptr;
   S.hvtx = dalloc<double>(own, P.P);
   S.zetavtx = dalloc<double>(own, P.P);
@@ -1102,8 +1290,8 @@ void setup_device(storm_fvt* C) {
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
   S.part = dalloc<double>(own, 4 * (size_t)C->nblk_upd);
-  C->dry_a = dalloc<uint8_t>(own, N);
-  C->dry_b = dalloc<uint8_t>(own, N);
+  C->d_a = dalloc<CellD>(own, N);
+  C->d_b = dalloc<CellD>(own, N);
   S.ctrl = dalloc<Ctrl>(own, 1);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
 }
@@ -1116,13 +1304,13 @@ void start_run(storm_fvt* C) {
   host_time_bcs(P, 0.0, hc.qin, hc.hbc);
   HIPCHK(hipMemcpyAsync(C->S.ctrl, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, C->stream));
   C->r1_flip = 0;
-  C->last_hin = C->slot_ptr(0);
   const int nb = (P.N + 255) / 256;
   hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, C->stream, C->g, C->o, C->slot_ptr(0),
-                     C->d_h0, C->d_u0, C->d_v0, C->S.u, C->S.v, C->dry_a, C->S.part);
+                     C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.part);
   HIPCHK(hipGetLastError());
-  C->S.dry_in = C->dry_a;
-  C->S.dry_out = C->dry_b;
+  C->S.din = C->d_a;
+  C->S.dout = C->d_b;
+  C->last_din = C->d_a;
   C->iter_host = 0;
   C->started = true;
   C->finalized = false;
@@ -1166,18 +1354,21 @@ void launch_step(storm_fvt* C) {
     st.beta = b[s - 1];
     st.s = s;
     st.last = (s == R);
-    hipLaunchKernelGGL(k_vertex, dim3(nbv), dim3(bs), 0, C->stream, C->g, C->o, st.hin,
-                       C->S.dry_in, C->S.hvtx, C->S.vwet, (double*)nullptr, C->S.ctrl);
+    hipLaunchKernelGGL(k_vertex, dim3(nbv), dim3(bs), 0, C->stream, C->g, C->o, C->S.din,
+                       (const uint8_t*)nullptr, C->S.hvtx, C->S.vwet, (double*)nullptr,
+                       C->S.ctrl);
     if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
-      hipLaunchKernelGGL(k_gauss1, dim3(nbc), dim3(bs), 0, C->stream, C->g, st.hin, C->S.u,
-                         C->S.v, C->S.gwork, C->S.ctrl);
+      hipLaunchKernelGGL(k_gauss1, dim3(nbc), dim3(bs), 0, C->stream, C->g, C->S.din,
+                         C->S.gwork, C->S.ctrl);
     hipLaunchKernelGGL(k_grad, dim3(nbc), dim3(bs), 0, C->stream, C->g, C->o, st, C->S);
     if (P.qs_cell.size() > 0)
       hipLaunchKernelGGL(k_bcq, dim3(P.n_in), dim3(BCQ_THREADS), 0, C->stream, C->g, C->o, st, C->S);
     hipLaunchKernelGGL(k_update, dim3(C->nblk_upd), dim3(UPD_THREADS), 0, C->stream, C->g, C->o,
                        st, C->S, C->S.part);
-    std::swap(C->S.dry_in, C->S.dry_out);
-    C->last_hin = st.hin;
+    C->last_din = C->S.din;
+    CellD* next = C->S.dout;
+    C->S.dout = const_cast<CellD*>(C->S.din);
+    C->S.din = next;
   }
   if (R == 1) C->r1_flip ^= 1;
 }
@@ -1306,11 +1497,13 @@ int storm_fvt_finalize(storm_fvt_t* C) {
     // to_vertices after the loop (solverFVT.f90:483): the water surface of
     // the last stage's start and the dry flags of the post-step
     // find_dry_cells (:335)
-    const uint8_t* dry = (C->iter_host > 0) ? C->S.dry_post : C->S.dry_in;
+    const bool stepped = C->iter_host > 0;
     hipLaunchKernelGGL(k_vertex, dim3((P.P + bs - 1) / bs), dim3(bs), 0, C->stream, C->g, C->o,
-                       C->last_hin, dry, C->S.hvtx, C->S.vwet, C->S.zetavtx, C->S.ctrl);
+                       C->last_din, stepped ? C->S.dry_post : (const uint8_t*)nullptr,
+                       C->S.hvtx, C->S.vwet, C->S.zetavtx, C->S.ctrl);
     hipLaunchKernelGGL(k_final_cells, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g,
-                       dry, C->S.vwet, C->S.u, C->S.v);
+                       stepped ? C->S.dry_post : (const uint8_t*)nullptr, C->S.vwet,
+                       const_cast<CellD*>(C->S.din));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(C->stream));
     C->finalized = true;
@@ -1337,16 +1530,26 @@ int storm_fvt_get_state(storm_fvt_t* C, double* h, double* u, double* v, double*
     Prep& P = C->prep;
     const size_t N = P.N;
     if (h) copy_cells(C, C->slot_ptr(0), h);
-    if (u) copy_cells(C, C->S.u, u);
-    if (v) copy_cells(C, C->S.v, v);
+    std::vector<CellD> d;
+    if (u || v || (drycell && C->iter_host == 0)) {
+      d.resize(N);
+      HIPCHK(hipMemcpy(d.data(), C->S.din, sizeof(CellD) * N, hipMemcpyDeviceToHost));
+    }
+    if (u)
+      for (size_t ii = 0; ii < N; ++ii) u[P.perm[ii]] = d[ii].u;
+    if (v)
+      for (size_t ii = 0; ii < N; ++ii) v[P.perm[ii]] = d[ii].v;
     if (phi)
       for (int c = 0; c < 3; ++c) copy_cells(C, C->S.phi + c * N, phi + c * N);
     if (drycell) {
       // the flags of the post-step find_dry_cells (solverFVT.f90:335)
-      std::vector<uint8_t> tmp(N);
-      HIPCHK(hipMemcpy(tmp.data(), (C->iter_host > 0) ? C->S.dry_post : C->S.dry_in, N,
-                       hipMemcpyDeviceToHost));
-      for (size_t ii = 0; ii < N; ++ii) drycell[P.perm[ii]] = tmp[ii];
+      if (C->iter_host > 0) {
+        std::vector<uint8_t> tmp(N);
+        HIPCHK(hipMemcpy(tmp.data(), C->S.dry_post, N, hipMemcpyDeviceToHost));
+        for (size_t ii = 0; ii < N; ++ii) drycell[P.perm[ii]] = tmp[ii];
+      } else {
+        for (size_t ii = 0; ii < N; ++ii) drycell[P.perm[ii]] = d[ii].dry != 0.0;
+      }
     }
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);

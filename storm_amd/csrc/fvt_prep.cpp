@@ -335,6 +335,9 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     if (chain_edges(m->in_nodes, m->in_ptr[ch], m->in_ptr[ch + 1], ce)) return STORM_E_MESH;
     in_edges[ch] = ce;
     nchain_edges += (int)ce.size();
+    if ((vt == 1 || vt == 5) && ce.size() > 512) {
+      S.err = "discharge (vtype 1/5) strings are limited to 512 edges"; return STORM_E_ARG;
+    }
     int kind = (vt == 1) ? SK_IN_Q1 : (vt == 5) ? SK_IN_Q5 : (vt == 4) ? SK_IN_H : SK_NONE;
     if (vt < 0 || vt > 5) { S.err = "invalid value in vtype"; return STORM_E_ARG; }
     if (vt == 2) { S.err = "VTYPE = 2: option not implemented (enforce_bcs.f90:151-155)"; return STORM_E_ARG; }
@@ -524,6 +527,17 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     dzy[i] = HALF * (z1 * (x3 - x2) + z2 * (x1 - x3) + z3 * (x2 - x1)) / area[i];
   }
 
+  // consistent orientation: every interior edge runs opposite ways in its two cells
+  for (int k = 0; k < E; ++k) {
+    if (cnt[k] != 2) continue;
+    int e1 = ee[2 * k], e2 = ee[2 * k + 1], s1 = 0, s2 = 0;
+    for (int l = 0; l < 3; ++l) {
+      if (gedge[3 * e1 + l] == k) s1 = gsign[3 * e1 + l];
+      if (gedge[3 * e2 + l] == k) s2 = gsign[3 * e2 + l];
+    }
+    if (s1 != -s2) { S.err = "inconsistently oriented triangles"; return STORM_E_MESH; }
+  }
+
   // ---- cell ordering --------------------------------------------------
   std::vector<int> order;
   if (o->reorder) order = rcm_order(N, t2t3);
@@ -544,112 +558,86 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     if (S.viperm[p] < 0) { S.viperm[p] = (int)S.vperm.size(); S.vperm.push_back(p); }
   const std::vector<int>& ip = S.iperm;
   const std::vector<int>& vip = S.viperm;
-  auto N3 = 3 * (size_t)N;
-  S.cvtx.assign(N3, 0); S.cnbr.assign(N3, -1); S.ep1.assign(N3, 0); S.ep2.assign(N3, 0);
-  S.cinfo.assign(N3, 0); S.corder.assign(N, 0);
-  S.rc.assign(6 * (size_t)N, 0);
-  S.enx.assign(N3, 0); S.eny.assign(N3, 0); S.etx.assign(N3, 0); S.ety.assign(N3, 0);
-  S.elen.assign(N3, 0); S.ezmid.assign(N3, 0);
-  S.area.resize(N); S.perim.resize(N); S.z.resize(N); S.cd.resize(N); S.delzx.resize(N);
-  S.delzy.resize(N); S.zhigh.resize(N);
-  S.vdx.assign(N3, 0); S.vdy.assign(N3, 0); S.zv3.assign(N3, 0);
+  const size_t N3 = 3 * (size_t)N;
+  S.cvtx.assign(N3, 0); S.cnbr.assign(N3, -1); S.cmeta.assign(N, 0);
+  S.xc.resize(N); S.yc.resize(N); S.z.resize(N); S.cd.resize(N);
   S.gconn.assign(N3, 0); S.gcx.assign(N3, 0); S.gcy.assign(N3, 0); S.gwt.assign(N3, 0);
   S.mptr.assign(N + 1, 0);
-  S.ai.resize(N); S.bi.resize(N); S.ci.resize(N); S.det.resize(N);
   S.h0.resize(N); S.u0.resize(N); S.v0.resize(N);
   for (int ii = 0; ii < N; ++ii) S.mptr[ii + 1] = S.mptr[ii] + (t2t_ptr[order[ii] + 1] - t2t_ptr[order[ii]]);
-  S.mcell.resize(S.mptr[N]); S.mdx.resize(S.mptr[N]); S.mdy.resize(S.mptr[N]); S.mw.resize(S.mptr[N]);
+  S.mcell.resize(S.mptr[N]);
   for (int ii = 0; ii < N; ++ii) {
     const int i = order[ii];
-    S.area[ii] = area[i]; S.perim[ii] = perim[i]; S.z[ii] = z[i]; S.cd[ii] = cd[i];
-    S.delzx[ii] = dzx[i]; S.delzy[ii] = dzy[i]; S.zhigh[ii] = zhigh[i];
-    S.ai[ii] = ai[i]; S.bi[ii] = bi[i]; S.ci[ii] = ci[i]; S.det[ii] = det[i];
+    S.xc[ii] = xc[i]; S.yc[ii] = yc[i]; S.z[ii] = z[i]; S.cd[ii] = cd[i];
     S.h0[ii] = h[i]; S.u0[ii] = u[i]; S.v0[ii] = v[i];
-    for (int t = 0; t < t2t_ptr[i + 1] - t2t_ptr[i]; ++t) {
-      int src = t2t_ptr[i] + t, dst = S.mptr[ii] + t;
-      S.mcell[dst] = ip[t2t[src]]; S.mdx[dst] = mdx[src]; S.mdy[dst] = mdy[src]; S.mw[dst] = mw[src];
-    }
+    for (int t = 0; t < t2t_ptr[i + 1] - t2t_ptr[i]; ++t)
+      S.mcell[S.mptr[ii] + t] = ip[t2t[t2t_ptr[i] + t]];
     uint64_t keys[3];
+    uint64_t meta = 0;
     for (int j = 0; j < 3; ++j) {
       const size_t s = (size_t)j * N + ii;
       const int k = gedge[3 * i + j];
-      const int vtxid = tri[3 * i + j];
-      S.cvtx[s] = vip[vtxid];
-      S.vdx[s] = x[vtxid] - xc[i];
-      S.vdy[s] = y[vtxid] - yc[i];
-      S.zv3[s] = zvtx[vtxid];
-      S.rc[(size_t)(2 * j) * N + ii] = rcx[3 * i + j];
-      S.rc[(size_t)(2 * j + 1) * N + ii] = rcy[3 * i + j];
+      S.cvtx[s] = vip[tri[3 * i + j]];
       S.gconn[s] = ip[gconn[3 * i + j]];
       S.gcx[s] = gcx[3 * i + j]; S.gcy[s] = gcy[3 * i + j]; S.gwt[s] = gwt[3 * i + j];
-      S.ep1[s] = vip[ep[2 * k]]; S.ep2[s] = vip[ep[2 * k + 1]];
-      S.elen[s] = len[k];
-      S.ezmid[s] = HALF * (zvtx[ep[2 * k]] + zvtx[ep[2 * k + 1]]);
-      // the normal/tangent the reference uses for edge k: that of edges(k)
-      // flipped when e(1) holds it with negative orientation
       const int e1 = ee[2 * k];
-      int m1 = -1;
-      for (int l = 0; l < 3; ++l) if (gedge[3 * e1 + l] == k) { m1 = l; break; }
-      const double sg = (gsign[3 * e1 + m1] < 0) ? -1.0 : 1.0;
-      S.enx[s] = sg < 0 ? -nrm[2 * k] : nrm[2 * k];
-      S.eny[s] = sg < 0 ? -nrm[2 * k + 1] : nrm[2 * k + 1];
-      S.etx[s] = sg < 0 ? -tng[2 * k] : tng[2 * k];
-      S.ety[s] = sg < 0 ? -tng[2 * k + 1] : tng[2 * k + 1];
-      uint32_t info = 0;
+      uint64_t slot = 0;
+      if (gsign[3 * i + j] < 0) slot |= 1ull << CMS_REV;
       if (cnt[k] == 2) {
         int nb = t2t3[3 * i + j];
         S.cnbr[s] = ip[nb];
         int nl = -1;
         for (int l = 0; l < 3; ++l) if (gedge[3 * nb + l] == k) { nl = l; break; }
-        info = SK_INTERIOR | ((uint32_t)nl << CI_NBRLOC_SHIFT);
-        if (e1 == i) info |= CI_LEFT;
+        slot |= (uint64_t)SK_INTERIOR | ((uint64_t)nl << CMS_NL);
+        if (e1 == i) slot |= 1ull << CMS_LEFT;
         keys[j] = ((uint64_t)0 << 60) | (uint64_t)k;
       } else if (wall[k]) {
-        info = SK_WALL | CI_LEFT;
+        slot |= (uint64_t)SK_WALL | (1ull << CMS_LEFT);
         keys[j] = ((uint64_t)1 << 60) | (uint64_t)k;
       } else {
-        uint32_t kind = (uint32_t)e_kind[k];
+        uint64_t kind = (uint64_t)e_kind[k];
         // reference routine order: inflowQbyH (vtype 1 strings, then 5),
         // inflowH, outflowbyH, crit_outflow, free_flow
         uint64_t rank = (kind == SK_IN_Q1) ? 0 : (kind == SK_IN_Q5) ? 1 : (kind == SK_IN_H) ? 2
                       : (kind == SK_OUT_H) ? 3 : (kind == SK_OUT_CRIT) ? 4 : (kind == SK_OUT_FREE) ? 5 : 6;
-        info = kind | CI_LEFT | ((uint32_t)e_str[k] << CI_CHAIN_SHIFT);
+        slot |= kind | (1ull << CMS_LEFT) | ((uint64_t)e_str[k] << CMS_CHAIN);
         keys[j] = ((uint64_t)2 << 60) | (rank << 52) | ((uint64_t)e_str[k] << 32) | (uint64_t)e_pos[k];
       }
-      if (hbar_e[k] && cnt[k] == 1) info |= CI_HBAR;
-      S.cinfo[s] = info;
+      if (hbar_e[k] && cnt[k] == 1) slot |= 1ull << CMS_HBAR;
+      meta |= slot << (CM_SLOT0 + CM_SLOT_BITS * j);
     }
     int ord[3] = {0, 1, 2};
     std::sort(ord, ord + 3, [&](int a, int b) { return keys[a] < keys[b]; });
-    S.corder[ii] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4));
+    meta |= (uint64_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4));
+    S.cmeta[ii] = meta;
   }
-  // vertices: inverse-distance weights over the cells in input order
+  // vertices: coordinates and bed, vertex -> cells in input cell order
   S.vptr.assign(P + 1, 0);
   for (int pp = 0; pp < P; ++pp) {
     int p = S.vperm[pp];
     S.vptr[pp + 1] = S.vptr[pp] + (n2t_ptr[p + 1] - n2t_ptr[p]);
   }
-  S.vcell.resize(S.vptr[P]); S.vw.resize(S.vptr[P]); S.zvtx.resize(P); S.vbc.assign(P, -1);
-  S.xv_int.resize(P); S.yv_int.resize(P);
+  S.vcell.resize(S.vptr[P]); S.vbc.assign(P, -1); S.vxyz.assign(4 * (size_t)P, 0.0);
   for (int pp = 0; pp < P; ++pp) {
     int p = S.vperm[pp];
-    S.zvtx[pp] = zvtx[p];
-    S.xv_int[pp] = x[p];
-    S.yv_int[pp] = y[p];
-    for (int t = 0; t < n2t_ptr[p + 1] - n2t_ptr[p]; ++t) {
+    S.vxyz[4 * (size_t)pp] = x[p];
+    S.vxyz[4 * (size_t)pp + 1] = y[p];
+    S.vxyz[4 * (size_t)pp + 2] = zvtx[p];
+    for (int t = 0; t < n2t_ptr[p + 1] - n2t_ptr[p]; ++t)
       S.vcell[S.vptr[pp] + t] = ip[n2t[n2t_ptr[p] + t]];
-      S.vw[S.vptr[pp] + t] = c2vwid[n2t_ptr[p] + t];
-    }
   }
+  (void)c2vwid;
   // enforce_bcs.f90:40-56: nodes of stage strings (last string wins, as the
   // reference loops over strings in order)
   for (int ch = 0; ch < m->n_out; ++ch) {
     if (m->out_type[ch] != 1) continue;
     for (int t = m->out_ptr[ch]; t < m->out_ptr[ch + 1]; ++t) S.vbc[vip[m->out_nodes[t]]] = ch;
   }
-  // inflowQbyH strings: (cell, slot) in string order
+  // inflowQbyH strings: (cell, local edge) in string order, with the edge's
+  // rc vector, unit normal (as the reference orients it) and length
   S.qs_ptr.assign(m->n_in + 1, 0);
   S.qs_cell.clear(); S.qs_slot.clear();
+  S.qs_rcx.clear(); S.qs_rcy.clear(); S.qs_nx.clear(); S.qs_ny.clear(); S.qs_len.clear();
   for (int ch = 0; ch < m->n_in; ++ch) {
     int vt = m->in_type[ch];
     if (vt == 1 || vt == 5) {
@@ -659,6 +647,12 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
         for (int t = 0; t < 3; ++t) if (gedge[3 * i + t] == k) { l = t; break; }
         S.qs_cell.push_back(ip[i]);
         S.qs_slot.push_back(l);
+        S.qs_rcx.push_back(rcx[3 * i + l]);
+        S.qs_rcy.push_back(rcy[3 * i + l]);
+        const double sg = (gsign[3 * i + l] < 0) ? -1.0 : 1.0;
+        S.qs_nx.push_back(sg < 0 ? -nrm[2 * k] : nrm[2 * k]);
+        S.qs_ny.push_back(sg < 0 ? -nrm[2 * k + 1] : nrm[2 * k + 1]);
+        S.qs_len.push_back(len[k]);
       }
     }
     S.qs_ptr[ch + 1] = (int)S.qs_cell.size();

@@ -11,8 +11,8 @@
 
 namespace storm {
 
-// Cell-edge slot kinds (cinfo bits 0-3).  The boundary kinds name the
-// reference routine that owns the edge (invisc_fluxes.f90:195-238).
+// Cell-edge slot kinds.  The boundary kinds name the reference routine that
+// owns the edge (invisc_fluxes.f90:195-238).
 enum SlotKind : uint32_t {
   SK_INTERIOR = 0,
   SK_WALL = 1,      // wenslp3_fluxes / weslp2_fluxes
@@ -25,12 +25,17 @@ enum SlotKind : uint32_t {
   SK_NONE = 8       // boundary edge with no flux routine (inflow strings of other vtypes)
 };
 
-// cinfo layout
-constexpr uint32_t CI_KIND_MASK = 0xF;
-constexpr uint32_t CI_NBRLOC_SHIFT = 4;   // 2 bits: neighbour's local index of the edge
-constexpr uint32_t CI_LEFT = 1u << 6;     // this cell is the edge's e(1)
-constexpr uint32_t CI_HBAR = 1u << 7;     // edge of a stage (outflow) string
-constexpr uint32_t CI_CHAIN_SHIFT = 8;    // 8 bits: boundary string index
+// Per-cell metadata word (uint64):
+//   bits 0-5   processing order of the three local edges, 2 bits each: the
+//              reference's phi accumulation order (interior edges by edge
+//              number, then walls, then the boundary routines in call order)
+//   slot j at bit 6 + 18*j (j = 0..2, local edge j runs from vertex j to j+1):
+//     +0  kind (4 bits)        +4  neighbour's local index of the edge (2)
+//     +6  cell is e(1) (1)     +7  edge of a stage string, hvalve (1)
+//     +8  edges%p runs from vertex j+1 to j (1)   +9 boundary string (8)
+constexpr int CM_SLOT0 = 6, CM_SLOT_BITS = 18;
+constexpr uint64_t CMS_KIND = 0xF;
+constexpr int CMS_NL = 4, CMS_LEFT = 6, CMS_HBAR = 7, CMS_REV = 8, CMS_CHAIN = 9;
 
 constexpr int MAX_STRINGS = 64;
 
@@ -40,34 +45,28 @@ struct Prep {
   std::vector<int> perm, iperm;     // cells: internal -> input, input -> internal
   std::vector<int> vperm, viperm;   // vertices
   // per cell (internal order); [3][N] arrays are slot-major: a[j*N + i]
-  std::vector<int> cvtx, cnbr, ep1, ep2;
-  std::vector<uint32_t> cinfo;
-  std::vector<uint8_t> corder;      // processing order of the 3 slots, 2 bits each
-  std::vector<double> rc;           // [6][N]: (rcx_j, rcy_j) rows for j = 0..2
-  std::vector<double> enx, eny, etx, ety, elen, ezmid;  // [3][N]
-  std::vector<double> area, perim, z, cd, delzx, delzy, zhigh;
-  std::vector<double> vdx, vdy, zv3;  // [3][N] vertex offsets from the centroid, vertex beds
+  std::vector<int> cvtx, cnbr;
+  std::vector<uint64_t> cmeta;
+  std::vector<double> xc, yc, z, cd;
   std::vector<int> mptr, mcell;       // least-squares molecule (CSR)
-  std::vector<double> mdx, mdy, mw, ai, bi, ci, det;
   std::vector<int> gconn;             // [3][N] Green-Gauss neighbours
   std::vector<double> gcx, gcy, gwt;  // [3][N]
   // vertices (internal order)
+  std::vector<double> vxyz;           // [4][P] interleaved: x, y, zvtx, 0
   std::vector<int> vptr, vcell;       // vertex -> cells (input cell order kept)
-  std::vector<double> vw;             // inverse-distance weights (c2v_weights_id)
-  std::vector<double> zvtx, xv_int, yv_int;  // bed and coordinates per vertex
   std::vector<int> vbc;               // stage string index of htype-1 string nodes, else -1
   // boundary strings
   int n_in = 0, n_out = 0;
   std::vector<int> in_type, out_type;
   std::vector<int> qts_ptr, hts_ptr;
   std::vector<double> qts_t, qts_v, hts_t, hts_v;
-  // inflowQbyH strings (vtype 1 and 5): edge list per string in string order
-  std::vector<int> qs_ptr, qs_cell, qs_slot;  // qs_ptr over n_in (empty range = other vtype)
+  // inflowQbyH strings (vtype 1 and 5): edges in string order with their cell,
+  // local edge and geometry
+  std::vector<int> qs_ptr, qs_cell, qs_slot;
+  std::vector<double> qs_rcx, qs_rcy, qs_nx, qs_ny, qs_len;
   double bch_max = 0.0;
   int last_cell = 0;                  // internal id of input cell N-1
-  // initial state (internal order)
-  std::vector<double> h0, u0, v0;
-  // input-order copies needed for output (input vertex ids of cells)
+  std::vector<double> h0, u0, v0;     // initial state (internal order)
   std::string err;
 };
 
