@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--block", type=int, default=0,
+                    help="stage-kernel tile capacity (256, 512, 1024; 0 = library default)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
@@ -86,7 +88,7 @@ def main():
     case = cases.meander()
     workload = "meander-2M-rk3"
     t0 = time.perf_counter()
-    solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph))
+    solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph, block=args.block))
     setup_s = time.perf_counter() - t0
     solver.run(args.warmup)
     t0 = time.perf_counter()

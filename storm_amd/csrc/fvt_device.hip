@@ -1,21 +1,25 @@
 // MI355X (gfx950) FP64 kernels of the SToRM FVT time step and the C ABI
 // declared in include/storm_fvt.h.
 //
-// Per RK stage (solverFVT.f90:158-331) four launches:
-//   k_vertex  to_vertices' vertex pass (to_vertices.f90:26-41, enforce_bcs
-//             stage nodes): IDW water surface over wet cells -> hvtx + wet flag
-//   k_grad    find_dry_cells' results + cell_grads (LSQ/Gauss, Barth-Jespersen)
-//             + hgradfix2 + the cell pass of to_vertices (:44-55)
-//   k_bcq     inflowQbyH's fixed point, one block per discharge string
-//   k_update  every flux of the cell's three edges, gathered by the cell in
-//             the reference's accumulation order (no atomics), bed slope,
-//             explicit friction if chosen, SSP-RK update, clip and wet/dry
-//             recovery; in the last stage also implicit friction, hvalve,
-//             the residual partial sums and the next step's CFL candidate
-// and per step one single-block k_control (dt, time, BC tables, stopping).
+// Per RK stage (solverFVT.f90:158-331) one launch, k_stage, one block per
+// tile (a run of Hilbert-ordered cells, its halo of edge neighbours and the
+// vertices of both):
+//   - the IDW water surface and wet flags of the tile's vertices
+//     (to_vertices.f90:26-41 with the stage strings' nodes of enforce_bcs);
+//   - find_dry_cells' flags, cell_grads (LSQ/Gauss + Barth-Jespersen),
+//     hgradfix2 and to_vertices' cell pass for the tile and halo cells, whose
+//     face states stay in shared memory;
+//   - inflowQbyH's fixed point for the strings the tile touches;
+//   - every flux of each tile cell's three sides, gathered by the cell in the
+//     reference's accumulation order (no atomics), bed slope, explicit
+//     friction if chosen, SSP-RK update, clip and wet/dry recovery; in the
+//     last stage also implicit friction, hvalve, the residual partial sums
+//     and the next step's CFL candidate.
+// Per step one single-block k_control (dt, time, BC tables, stopping).  The
+// Green-Gauss option adds its first pass (k_gauss1) before each stage.
 //
-// Cells are in Reverse Cuthill-McKee order; all arrays are structure-of-
-// arrays FP64/int32; per-cell edge data are stored per (slot, cell).
+// Per-cell records are 32-byte structs (static, stage state); per-cell edge
+// data are stored per (slot, cell).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +39,7 @@ using namespace storm;
 namespace {
 
 constexpr int CTRL_THREADS = 1024;
+constexpr int DEFAULT_CAP = 256;  // cells + halo cells per tile (threads per stage block)
 
 struct Ctrl {
   double dt;          // dt of the current step
@@ -42,18 +47,12 @@ struct Ctrl {
   double ttime, stime, rmax, sumtop, sumbot;
   int iter, done, pending, pad;
   double qin[MAX_STRINGS], hbc[MAX_STRINGS];
-  double atotal[MAX_STRINGS];
-  int null_area[MAX_STRINGS], qskip[MAX_STRINGS];
 };
 
-// Cell records, read and written whole with 16-byte loads: static data,
-// stage state (double-buffered), and the face record k_grad writes for
-// k_update (one 32-byte slot per side, so a neighbour fetches the state at
-// the shared side with one load).
+// Cell records, read whole with 16-byte loads: static data and the stage
+// state (double-buffered between stages).
 struct alignas(32) CellS { double xc, yc, z, cd; };
-struct alignas(32) CellD { double h, u, v, dry; };        // dry: 1 or 0
-struct alignas(32) FSlot { double h, u, v, z; };          // face state + the cell's bed
-struct alignas(32) CellF { FSlot s[3]; double dhx, dhy, umag, h; };  // umag < 0: dry cell
+struct alignas(32) CellD { double h, u, v, dry; };  // dry: 1 or 0
 
 struct Geo {
   int N, P, R;
@@ -89,11 +88,24 @@ struct Stage {
 struct State {
   const CellD* din;   // state at the start of the stage
   CellD* dout;        // state for the next stage
-  CellF* f;
-  double *up, *vp;    // modified velocities, explicit-friction mode only
   double *gwork, *hvtx, *zetavtx, *phi, *part;
   uint8_t *dry_post, *vwet;
   Ctrl* ctrl;
+};
+
+// Tiles of the stage kernel: tile b owns cells [tptr[b], tptr[b+1]) (a run of
+// the Hilbert-ordered cells), its halo of edge neighbours outside the run is
+// hcell[hptr[b]..hptr[b+1]), and the vertices of both are
+// tvert[tvptr[b]..tvptr[b+1]).  Local ids are 16 bits, three to a word (cell
+// slot j at bit 16*j): tile cells 0..T-1, then halo cells; vertices in list
+// order.
+struct Tiles {
+  const int* order;             // block -> tile
+  const int *tptr, *hptr, *hcell, *tvptr, *tvert;
+  const uint64_t *clv, *cln;    // per cell: local vertex ids, local neighbour ids
+  const uint64_t* hlv;          // per halo entry: local vertex ids
+  const uint64_t* qmask;        // per tile: inflowQbyH strings it touches
+  int nt;
 };
 
 __device__ __forceinline__ CellD ld(const CellD* p) {
@@ -104,14 +116,7 @@ __device__ __forceinline__ CellS ld(const CellS* p) {
   const double4 q = *reinterpret_cast<const double4*>(p);
   return CellS{q.x, q.y, q.z, q.w};
 }
-__device__ __forceinline__ FSlot ld(const FSlot* p) {
-  const double4 q = *reinterpret_cast<const double4*>(p);
-  return FSlot{q.x, q.y, q.z, q.w};
-}
 __device__ __forceinline__ void st4(CellD* p, double a, double b, double c, double d) {
-  *reinterpret_cast<double4*>(p) = make_double4(a, b, c, d);
-}
-__device__ __forceinline__ void st4(double* p, double a, double b, double c, double d) {
   *reinterpret_cast<double4*>(p) = make_double4(a, b, c, d);
 }
 
@@ -160,21 +165,26 @@ __device__ __forceinline__ double tri_area(const Tri& t) {
 struct Side {
   double len, nx, ny, tx, ty, mx, my, zm;
 };
+__device__ __forceinline__ Side side_of(double xa, double ya, double za, double xb, double yb,
+                                       double zb) {
+  Side s;
+  const double dx = xa - xb, dy = ya - yb;
+  s.len = sqrt(dx * dx + dy * dy);
+  s.nx = (yb - ya) / s.len;
+  s.ny = (xa - xb) / s.len;
+  s.tx = (xb - xa) / s.len;
+  s.ty = (yb - ya) / s.len;
+  s.mx = HALF * (xa + xb);  // dataprepFVT.f90:168-169
+  s.my = HALF * (ya + yb);
+  s.zm = HALF * (za + zb);  // invisc_fluxes.f90:89
+  return s;
+}
 template <int J>
 __device__ __forceinline__ Side side(const Tri& t) {
   constexpr int A = J, B = (J + 1) % 3;
-  Side s;
-  const double dx = t.x[A] - t.x[B], dy = t.y[A] - t.y[B];
-  s.len = sqrt(dx * dx + dy * dy);
-  s.nx = (t.y[B] - t.y[A]) / s.len;
-  s.ny = (t.x[A] - t.x[B]) / s.len;
-  s.tx = (t.x[B] - t.x[A]) / s.len;
-  s.ty = (t.y[B] - t.y[A]) / s.len;
-  s.mx = HALF * (t.x[A] + t.x[B]);    // dataprepFVT.f90:168-169
-  s.my = HALF * (t.y[A] + t.y[B]);
-  s.zm = HALF * (t.zv[A] + t.zv[B]);  // invisc_fluxes.f90:89
-  return s;
+  return side_of(t.x[A], t.y[A], t.zv[A], t.x[B], t.y[B], t.zv[B]);
 }
+
 
 __device__ __forceinline__ uint32_t slot_bits(uint64_t meta, int j) {
   return (uint32_t)(meta >> (CM_SLOT0 + CM_SLOT_BITS * j)) & ((1u << CM_SLOT_BITS) - 1);
@@ -192,14 +202,19 @@ __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_
 }
 
 // ---------------------------------------------------------------- init
-__global__ void __launch_bounds__(256)
-k_init(Geo g, Opts o, double* slot0, const double* h0, const double* u0, const double* v0,
-       CellD* d, double* part) {
-  // solverFVT.f90:123-127 and the first find_dry_cells; the CFL candidates of
-  // step 1 go to the per-block partials (slot 3), reduced by k_control
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+template <int CAP>
+__global__ void __launch_bounds__(CAP)
+k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u0,
+       const double* v0, CellD* d, double* part) {
+  // solverFVT.f90:123-127 and the first find_dry_cells, one block per tile;
+  // the CFL candidates of step 1 go to the per-tile partials (slot 3),
+  // reduced by k_control
+  __shared__ double s_c[CAP];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int c0 = tl.tptr[b], T = tl.tptr[b + 1] - c0;
   double cand = DBL_MAX;
-  if (i < g.N) {
+  if (t < T) {
+    const int i = c0 + t;
     const double h = h0[i], uu = u0[i], vv = v0[i];
     slot0[g.N + i] = h * uu;
     slot0[2 * g.N + i] = h * vv;
@@ -208,25 +223,23 @@ k_init(Geo g, Opts o, double* slot0, const double* h0, const double* u0, const d
     const bool dr = find_dry(h, uu, vv, o.h_dry, o.h_wet, &um);
     st4(&d[i], h, uu, vv, dr ? ONE : ZERO);
     const double wc = dr ? ZERO : sqrt(G * h);
-    const Tri t = load_tri(g, i);
-    const double area = tri_area(t);
-    const double perim = side<0>(t).len + side<1>(t).len + side<2>(t).len;
-    double dd = um + wc + SMALL;
+    const Tri tr = load_tri(g, i);
+    const double area = tri_area(tr);
+    const double perim = side<0>(tr).len + side<1>(tr).len + side<2>(tr).len;
+    const double dd = um + wc + SMALL;
     cand = o.cfl * area / (dd * perim);
   }
-  __shared__ double s_c[256];
-  const int t = threadIdx.x;
   s_c[t] = cand;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = CAP / 2; w > 0; w >>= 1) {
     if (t < w) s_c[t] = dmin(s_c[t], s_c[t + w]);
     __syncthreads();
   }
   if (t == 0) {
-    part[4 * blockIdx.x] = ZERO;
-    part[4 * blockIdx.x + 1] = ZERO;
-    part[4 * blockIdx.x + 2] = ZERO;
-    part[4 * blockIdx.x + 3] = s_c[0];
+    part[4 * b] = ZERO;
+    part[4 * b + 1] = ZERO;
+    part[4 * b + 2] = ZERO;
+    part[4 * b + 3] = s_c[0];
   }
 }
 
@@ -333,43 +346,72 @@ k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, con
 
 // ---------------------------------------------------------------- vertex pass
 // to_vertices.f90:26-41 with ctr2vtxWSE.f90:89-105 (inverse-distance weights
-// of c2v_setup.f90:106-113) and positivity_vtx.f90, then the stage strings'
-// nodes of enforce_bcs.f90:40-56.  `dry` overrides the dry flags of `d` (the
-// post-loop call uses the flags of the post-step find_dry_cells).
+// of c2v_setup.f90:106-113) and positivity_vtx.f90 for vertex p, then the
+// stage strings' nodes of enforce_bcs.f90:40-56.  `dry` overrides the dry
+// flags of `d` when given (the post-loop call uses the flags of the
+// post-step find_dry_cells).
+struct Vtx {
+  double h, zeta;
+  bool wet;
+};
+
+__device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const CellD* d,
+                                          const uint8_t* dry, int p, const double4& q,
+                                          const Ctrl* c) {
+  double id = ZERO, a = ZERO, zr = q.z;
+  int counter = 0;
+  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
+  for (int t = t0; t < t1; t += 4) {
+    // four cells per round, all loads issued before the sums
+    int k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = (t + u < t1) ? g.vcell[t + u] : -1;
+    CellD dk[4];
+    CellS sk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k[u] < 0 ? k[0] : k[u];
+      dk[u] = ld(&d[kk]);
+      sk[u] = ld(&g.cs[kk]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k[u] < 0) continue;
+      const bool kd = dry ? (dry[k[u]] != 0) : (dk[u].dry != ZERO);
+      if (kd) continue;
+      counter++;
+      const double xk = sk[u].xc - q.x, yk = sk[u].yc - q.y;
+      const double w = ONE / sqrt(xk * xk + yk * yk);
+      const double zk = dk[u].h + sk[u].z;
+      id = id + w;
+      a = a + w * zk;
+      zr = dmax(zk, zr);
+    }
+  }
+  double zeta = (counter > 0) ? a / id : q.z;
+  zeta = dmin(zeta, zr);
+  double hv = zeta - q.z;
+  if (hv < ZERO) { hv = ZERO; zeta = q.z; }
+  Vtx v;
+  v.wet = hv > o.h_dry;
+  const int b = g.vbc[p];
+  if (b >= 0) hv = dmax(ZERO, c->hbc[b] - q.z);
+  v.h = hv;
+  v.zeta = zeta;
+  return v;
+}
+
+// All vertices: the pass after the time loop (solverFVT.f90:483), which also
+// forms zetavtx.
 __global__ void __launch_bounds__(256)
 k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_t* vwet,
          double* zetavtx, const Ctrl* c) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
-  const double4 q = g.vxyz[p];
-  const double zv = q.z;
-  double id = ZERO, a = ZERO, zr = zv;
-  int counter = 0;
-  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
-  for (int t = t0; t < t1; ++t) {
-    const int k = g.vcell[t];
-    const CellD dk = ld(&d[k]);
-    const CellS sk = ld(&g.cs[k]);
-    const bool dk_dry = dry ? (dry[k] != 0) : (dk.dry != ZERO);
-    if (dk_dry) continue;
-    counter++;
-    const double xk = sk.xc - q.x, yk = sk.yc - q.y;
-    const double w = ONE / sqrt(xk * xk + yk * yk);
-    const double zk = dk.h + sk.z;
-    id = id + w;
-    a = a + w * zk;
-    zr = dmax(zk, zr);
-  }
-  if (c->done) return;
-  double zeta = (counter > 0) ? a / id : zv;
-  zeta = dmin(zeta, zr);
-  double hv = zeta - zv;
-  if (hv < ZERO) { hv = ZERO; zeta = zv; }
-  vwet[p] = hv > o.h_dry;
-  const int b = g.vbc[p];
-  if (b >= 0) hv = dmax(ZERO, c->hbc[b] - zv);
-  hvtx[p] = hv;
-  if (zetavtx) zetavtx[p] = zeta;
+  const Vtx v = vertex_idw(g, o, d, dry, p, g.vxyz[p], c);
+  vwet[p] = v.wet;
+  hvtx[p] = v.h;
+  zetavtx[p] = v.zeta;
 }
 
 // ---------------------------------------------------------------- Green-Gauss pass 1
@@ -412,33 +454,39 @@ __device__ __forceinline__ void bj_limit(double q0, const double qn[3], const do
   gy = ph * gy;
 }
 
-// ---------------------------------------------------------------- gradients
-// cell_grads.f90:24-75 (least squares of lsq_grad_array.f90:60-73 with the
-// weights and normal-equation sums of lsq_setup.f90:105-145 formed on the fly),
-// hgradfix2.f90:46-98, the cell pass of to_vertices.f90:44-55, and the face
-// states of invisc_fluxes.f90:53-81 that k_update consumes.
-__global__ void __launch_bounds__(256)
-k_grad(Geo g, Opts o, Stage st, State S) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
-  if (i >= g.N) return;
+// ---------------------------------------------------------------- face states
+// What the flux loop needs of one cell: its face states at the three sides
+// (interior sides as invisc_fluxes.f90:53-81 forms them -- zero for a dry
+// cell, depth clipped at 0 -- boundary sides unclipped), the depth gradient
+// for the bed-slope term, the modified velocities and the stage input.
+struct Faces {
+  double fh[3], fu[3], fv[3];
+  double dhx, dhy, up, vp, h, umag;  // umag: |u| of the stage input, 0 when dry
+  bool dry;
+  bool fix;  // hgradfix2 set the gradient: rk hu, hv of slot s-1 are zeroed
+};
+
+// find_dry_cells' flags (carried in the state), cell_grads.f90:24-75 (least
+// squares of lsq_grad_array.f90:60-73 with the weights and normal-equation
+// sums of lsq_setup.f90:105-145 formed on the fly, or Green-Gauss),
+// hgradfix2.f90:46-98 and the cell pass of to_vertices.f90:44-55, for cell i.
+// vw: the cell's three vertices are wet.
+__device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const CellD* din,
+                                           const double* gwork, int i, const Tri& tr, bool vw,
+                                           Faces& F) {
   const int N = g.N;
-  // first level of loads: own records and neighbour ids
   const uint64_t meta = g.cmeta[i];
   const int nb[3] = {g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i]};
-  const CellD di = ld(&S.din[i]);
+  const CellD di = ld(&din[i]);
   const CellS si = ld(&g.cs[i]);
-  const Tri tr = load_tri(g, i);
-  // second level: the edge neighbours' records and the vertex wet flags
   CellD dn[3];
   CellS sn[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const int k = nb[j] < 0 ? i : nb[j];
-    dn[j] = ld(&S.din[k]);
+    dn[j] = ld(&din[k]);
     sn[j] = ld(&g.cs[k]);
   }
-  const bool vw = S.vwet[g.cvtx[i]] && S.vwet[g.cvtx[N + i]] && S.vwet[g.cvtx[2 * N + i]];
-  if (S.ctrl->done) return;
   const double hi = di.h, zi = si.z, ui = di.u, vi = di.v;
   const double xci = si.xc, yci = si.yc;
   const bool dry_i = di.dry != ZERO;
@@ -450,7 +498,7 @@ k_grad(Geo g, Opts o, Stage st, State S) {
   if (o.limiter != 0) {
     if (o.gradient == STORM_GRAD_GAUSS) {
       // gaussgrad.f90:45-52
-      const double* w = S.gwork;
+      const double* w = gwork;
       const int n0 = g.gconn[i], n1 = g.gconn[N + i], n2 = g.gconn[2 * N + i];
       const double w0 = g.gwt[i], w1 = g.gwt[N + i], w2 = g.gwt[2 * N + i];
       gzx = HALF * (w[i] + w0 * w[n0] + w1 * w[n1] + w2 * w[n2]);
@@ -492,7 +540,7 @@ k_grad(Geo g, Opts o, Stage st, State S) {
         const int t0 = g.mptr[i], t1 = g.mptr[i + 1];
         for (int t = t0; t < t1; ++t) {
           const int k = g.mcell[t];
-          const CellD dk = ld(&S.din[k]);
+          const CellD dk = ld(&din[k]);
           const CellS sk = ld(&g.cs[k]);
           member(sk.xc, sk.yc, dk.h + sk.z, dk.u, dk.v);
         }
@@ -503,9 +551,12 @@ k_grad(Geo g, Opts o, Stage st, State S) {
       gvx = (cc * dv - b * ev) / d; gvy = (a * ev - dv * b) / d;
     }
   }
-  const Side s0 = side<0>(tr), s1 = side<1>(tr), s2 = side<2>(tr);
-  const double rcx[3] = {s0.mx - xci, s1.mx - xci, s2.mx - xci};
-  const double rcy[3] = {s0.my - yci, s1.my - yci, s2.my - yci};
+  double rcx[3], rcy[3];
+  {
+    const Side s0 = side<0>(tr), s1 = side<1>(tr), s2 = side<2>(tr);
+    rcx[0] = s0.mx - xci; rcx[1] = s1.mx - xci; rcx[2] = s2.mx - xci;
+    rcy[0] = s0.my - yci; rcy[1] = s1.my - yci; rcy[2] = s2.my - yci;
+  }
   if (o.limiter == 2) {
     double qz[3], qu[3], qv[3];
 #pragma unroll
@@ -526,6 +577,7 @@ k_grad(Geo g, Opts o, Stage st, State S) {
                              tr.zv[2] * (tr.x[1] - tr.x[0])) / area;
   double dhx = gzx - dzx, dhy = gzy - dzy;
   double up = ui, vp = vi;
+  bool fix = false;
   if (!dry_i) {
     // hgradfix2.f90:49-96 (zvtx of the highest vertex: dataprepFVT.f90:293-306)
     const double zhigh = dmax(dmax(tr.zv[0], tr.zv[1]), tr.zv[2]);
@@ -538,16 +590,12 @@ k_grad(Geo g, Opts o, Stage st, State S) {
     if (setgrad) {
       dhx = -dzx; dhy = -dzy;
       up = ZERO; vp = ZERO;
-      st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1)
-      st.hin[2 * N + i] = ZERO;
+      fix = true;
       gux = ZERO; guy = ZERO; gvx = ZERO; gvy = ZERO;
     }
     // to_vertices.f90:44-55
     if (!vw) { up = ZERO; vp = ZERO; }
   }
-  // face records: interior sides as invisc_fluxes.f90:53-81 forms them (zero
-  // for a dry cell, depth clipped at 0), boundary sides unclipped
-  CellF* F = &S.f[i];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     double fh = hi + dhx * rcx[j] + dhy * rcy[j];
@@ -557,137 +605,37 @@ k_grad(Geo g, Opts o, Stage st, State S) {
       if (dry_i) { fh = ZERO; fu = ZERO; fv = ZERO; }
       else if (fh < ZERO) fh = ZERO;
     }
-    st4(reinterpret_cast<double*>(&F->s[j]), fh, fu, fv, zi);
+    F.fh[j] = fh; F.fu[j] = fu; F.fv[j] = fv;
   }
-  const double umag = dry_i ? -ONE : sqrt(ui * ui + vi * vi);
-  st4(&F->dhx, dhx, dhy, umag, hi);
-  if (o.friction == 1) {
-    S.up[i] = up;
-    S.vp[i] = vp;
-  }
-  if (o.steady && st.last && i == g.last_cell) {
-    // the local dt of the reference's last cell, which fric_terms_impl uses
-    // (solverFVT.f90:260-265,336)
-    double umg = ZERO;
-    bool d = find_dry(hi, ui, vi, o.h_dry, o.h_wet, &umg);
-    const double wc = d ? ZERO : sqrt(G * hi);
-    double dd = umg + wc;
-    if (dd < SMALL) dd = 1.044;
-    const double perim = s0.len + s1.len + s2.len;
-    S.ctrl->delt_last = o.cfl * area / (dd * perim);
-  }
+  F.dhx = dhx; F.dhy = dhy; F.up = up; F.vp = vp; F.h = hi;
+  F.umag = dry_i ? ZERO : sqrt(ui * ui + vi * vi);
+  F.dry = dry_i;
+  F.fix = fix;
 }
 
-// ---------------------------------------------------------------- inflowQbyH
-// inflowQbyH.f90:31-126 (vtype 1) / :221-316 (vtype 5): the fixed point on
-// the string's wetted length, one wavefront per string.  Each lane holds the
-// face state of its edges; every sum runs over the edges in string order,
-// read lane by lane, so it is the reference's sequential sum.
-constexpr int BCQ_THREADS = 64;
-constexpr int BCQ_MAX = 8;  // edges per lane: strings of up to 512 edges
-
-__device__ __forceinline__ double lane_read(double x, int k) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), k);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+// The local dt of the reference's last cell from the stage input, which the
+// steady-mode implicit friction uses (solverFVT.f90:260-265,336).
+__device__ double last_cell_dt(const Geo& g, const Opts& o, const CellD* din) {
+  const int i = g.last_cell;
+  const CellD d = ld(&din[i]);
+  const Tri tr = load_tri(g, i);
+  double umg = ZERO;
+  const bool dr = find_dry(d.h, d.u, d.v, o.h_dry, o.h_wet, &umg);
+  const double wc = dr ? ZERO : sqrt(G * d.h);
+  double dd = umg + wc;
+  if (dd < SMALL) dd = 1.044;
+  const double perim = side<0>(tr).len + side<1>(tr).len + side<2>(tr).len;
+  return o.cfl * tri_area(tr) / (dd * perim);
 }
-
-__global__ void __launch_bounds__(BCQ_THREADS)
-k_bcq(Geo g, Opts o, Stage st, State S) {
-  const int ch = blockIdx.x;
-  Ctrl* c = S.ctrl;
-  const int vt = g.in_type[ch];
-  if (vt != 1 && vt != 5) return;
-  if (c->done) return;
-  const int t = threadIdx.x;
-  const int e0 = g.qs_ptr[ch], n = g.qs_ptr[ch + 1] - e0;
-  const double qin = c->qin[ch];
-  if (qin < DBL_MIN) { if (t == 0) c->qskip[ch] = 1; return; }
-  const double h_dry = o.h_dry;
-  __shared__ double s_a[BCQ_THREADS * BCQ_MAX], s_q[BCQ_THREADS * BCQ_MAX];
-  __shared__ int s_ok[BCQ_THREADS * BCQ_MAX];
-  double fh[BCQ_MAX], fu[BCQ_MAX], fv[BCQ_MAX], nx[BCQ_MAX], ny[BCQ_MAX], L[BCQ_MAX];
-  bool act[BCQ_MAX];
-#pragma unroll
-  for (int r = 0; r < BCQ_MAX; ++r) {
-    const int l = t + 64 * r;
-    act[r] = false; fh[r] = fu[r] = fv[r] = nx[r] = ny[r] = L[r] = ZERO;
-    if (l < n) {
-      const int e = e0 + l;
-      const int i = g.qs_cell[e], m = g.qs_slot[e];
-      const FSlot fs = ld(&S.f[i].s[m]);
-      const bool d = S.f[i].umag < ZERO;
-      act[r] = !(vt == 5 && d);
-      fh[r] = fs.h; fu[r] = fs.u; fv[r] = fs.v;
-      nx[r] = g.qs_nx[e]; ny[r] = g.qs_ny[e]; L[r] = g.qs_len[e];
-      const double hl = d ? ZERO : dmax(fs.h, ZERO);
-      s_a[l] = hl * L[r];
-      s_ok[l] = act[r];
-    }
-  }
-  __syncthreads();
-  // initial wetted length (inflowQbyH.f90:40-56), summed in string order
-  double atotal = ZERO;
-  for (int k = 0; k < n; ++k)
-    if (s_ok[k]) atotal = atotal + s_a[k];
-  const bool null_area = equals(atotal, ZERO);
-  int counter = 0;
-  double newatotal, qtotal;
-  for (;;) {
-    __syncthreads();
-    const double qfrac = qin / atotal;
-#pragma unroll
-    for (int r = 0; r < BCQ_MAX; ++r) {
-      const int l = t + 64 * r;
-      if (l >= n) continue;
-      int ok = 0;
-      if (act[r]) {
-        double hl = fh[r], ue, ve;
-        if (null_area) hl = g.bch_max;
-        if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
-        else { ue = fu[r]; ve = fv[r]; }
-        double ur = -qfrac;
-        const double ul = ue * nx[r] + ve * ny[r];
-        const double sq = sqrt(G * hl);
-        double hr = 2.0 * sq + ul - ur;
-        hr = hr * hr / 4.0 / G;
-        ur = -qfrac * nx[r];
-        const double vr = -qfrac * ny[r];
-        if (!(hr < h_dry)) {
-          s_a[l] = hr * L[r];
-          s_q[l] = (ur * nx[r] + vr * ny[r]) * hr * L[r];
-          ok = 1;
-        }
-      }
-      s_ok[l] = ok;
-    }
-    __syncthreads();
-    newatotal = ZERO;
-    qtotal = ZERO;
-    for (int k = 0; k < n; ++k)
-      if (s_ok[k]) { newatotal = newatotal + s_a[k]; qtotal = qtotal + s_q[k]; }
-    qtotal = fabs(qtotal);
-    if (fabs(qtotal - qin) / qin < 0.005 || counter > 50) break;
-    counter++;
-    atotal = newatotal;
-  }
-  if (t == 0) {
-    c->atotal[ch] = newatotal;
-    c->null_area[ch] = null_area;
-    c->qskip[ch] = 0;
-  }
-}
-
-// ---------------------------------------------------------------- update
-constexpr int UPD_THREADS = 256;
 
 // Flux through one boundary side of cell i (the routines of
-// invisc_fluxes.f90:195-238), as the change of phi(i,:).
-__device__ void boundary_flux(const Geo& g, const Opts& o, const State& S, const Ctrl* c,
-                              uint32_t bits, const Side& sd, double fh, double fu, double fv,
-                              double hi, bool dry_i, int p1, int p2, double& d0, double& d1,
-                              double& d2) {
+// invisc_fluxes.f90:195-238), as the change of phi(i,:).  h1, h2: vertex
+// depths at the ends of the edge in the reference's edge direction; q_*:
+// the inflowQbyH results of the strings.
+__device__ void boundary_flux(const Geo& g, const Opts& o, const Ctrl* c, const double* q_at,
+                              const int* q_null, const int* q_skip, uint32_t bits, const Side& sd,
+                              double fh, double fu, double fv, double hi, bool dry_i, double h1,
+                              double h2, double& d0, double& d1, double& d2) {
   const uint32_t kind = bits & CMS_KIND;
   const double h_dry = o.h_dry;
   const double nx = sd.nx, ny = sd.ny, L = sd.len;
@@ -696,7 +644,6 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const State& S, const
     if (dry_i) return;
     if (o.btype == 0) {
       // wenslp3_fluxes.f90:40-61
-      const double h1 = S.hvtx[p1], h2 = S.hvtx[p2];
       double he = sqrt((h1 * h1 + h1 * h2 + h2 * h2) * ONE_THIRD);
       if (he < ZERO) he = ZERO;
       if (he < h_dry) return;
@@ -718,12 +665,12 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const State& S, const
   const int ch = (bits >> CMS_CHAIN) & 0xFF;
   if (kind == SK_IN_Q1 || kind == SK_IN_Q5) {
     // inflowQbyH.f90:131-210 (vtype 1) / :321-400 (vtype 5)
-    if (c->qskip[ch]) return;
+    if (q_skip[ch]) return;
     if (kind == SK_IN_Q5 && dry_i) return;
     const double qin = c->qin[ch];
-    const double atotal = c->atotal[ch];
+    const double atotal = q_at[ch];
     double hl = fh, ue, ve;
-    if (c->null_area[ch]) hl = g.bch_max;
+    if (q_null[ch]) hl = g.bch_max;
     if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
     else { ue = fu; ve = fv; }
     const double qfrac = qin / atotal;
@@ -823,22 +770,12 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const State& S, const
   }
 }
 
-// Neighbour data an interior side needs, gathered before any flux work: its
-// face state at the shared side with its bed, its depth and dry flag.
+// Neighbour data an interior side needs: its face state at the shared side,
+// its bed, stage-input depth and dry flag.
 struct Nbr {
-  FSlot f;
-  double h;
+  double fh, fu, fv, z, h;
   bool dry;
 };
-
-__device__ __forceinline__ Nbr load_nbr(const State& S, int nb, int nl) {
-  Nbr n;
-  n.f = ld(&S.f[nb].s[nl]);
-  const double2 t = *reinterpret_cast<const double2*>(&S.f[nb].umag);
-  n.dry = t.x < ZERO;
-  n.h = t.y;
-  return n;
-}
 
 // invisc_fluxes.f90:39-166 for one interior side of cell i: the flux of the
 // edge computed exactly as the reference does for it (left state = its e(1)
@@ -851,10 +788,10 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
   if (dry_i && nb.dry) return;
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   const bool left = (bits >> CMS_LEFT) & 1;
-  double hl = left ? fho : nb.f.h, ul = left ? fuo : nb.f.u, vl = left ? fvo : nb.f.v;
-  double hr = left ? nb.f.h : fho, ur = left ? nb.f.u : fuo, vr = left ? nb.f.v : fvo;
+  double hl = left ? fho : nb.fh, ul = left ? fuo : nb.fu, vl = left ? fvo : nb.fv;
+  double hr = left ? nb.fh : fho, ur = left ? nb.fu : fuo, vr = left ? nb.fv : fvo;
   const bool dL = left ? dry_i : nb.dry, dR = left ? nb.dry : dry_i;
-  const double zL = left ? zi : nb.f.z, zR = left ? nb.f.z : zi;
+  const double zL = left ? zi : nb.z, zR = left ? nb.z : zi;
   const double wL = left ? hi : nb.h, wR = left ? nb.h : hi;
   const double nx = left ? sd.nx : -sd.nx, ny = left ? sd.ny : -sd.ny;
   if (hl < h_dry && hr < h_dry) return;
@@ -885,69 +822,272 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
   else { d0 = f[0] * L; d1 = f[1] * L; d2 = f[2] * L; }
 }
 
-__global__ void __launch_bounds__(UPD_THREADS)
-k_update(Geo g, Opts o, Stage st, State S, double* part) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
-  const Ctrl* c = S.ctrl;
-  const bool done = c->done;
-  const int N = g.N;
-  double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX;
-  const double h_dry = o.h_dry, h_wet = o.h_wet;
-  if (i < N && !done) {
-    // own records
-    const uint64_t meta = g.cmeta[i];
-    const int nb0 = g.cnbr[i], nb1 = g.cnbr[N + i], nb2 = g.cnbr[2 * N + i];
-    const CellF* F = &S.f[i];
-    const FSlot f0 = ld(&F->s[0]), f1 = ld(&F->s[1]), f2 = ld(&F->s[2]);
-    const double4 tail = *reinterpret_cast<const double4*>(&F->dhx);
-    const CellS si = ld(&g.cs[i]);
+// ---------------------------------------------------------------- stage kernel
+// Shared-memory layout of one tile (CAP cells + halo, CAP vertices) and the
+// inflow-string buffers.
+constexpr int QMAX = 512;  // edges of one inflowQbyH string
+
+struct Lds {
+  double *fh, *fu, *fv;          // [3][CAP] face states, side-major
+  double *z, *h;                 // [CAP] bed and stage-input depth
+  double *vx, *vy, *vz, *vh;     // [CAP] tile vertices: x, y, bed, depth
+  double *qa, *qq;               // [QMAX] string edges: area and discharge terms
+  uint8_t *dry, *vw, *qok;
+};
+
+__host__ __device__ constexpr size_t lds_bytes(int cap) {
+  return sizeof(double) * (15 * (size_t)cap + 2 * (size_t)QMAX) + 2 * (size_t)cap + (size_t)QMAX;
+}
+
+__device__ __forceinline__ Lds lds_layout(double* base, int cap) {
+  Lds L;
+  L.fh = base; L.fu = L.fh + 3 * cap; L.fv = L.fu + 3 * cap;
+  L.z = L.fv + 3 * cap; L.h = L.z + cap;
+  L.vx = L.h + cap; L.vy = L.vx + cap; L.vz = L.vy + cap; L.vh = L.vz + cap;
+  L.qa = L.vh + cap; L.qq = L.qa + QMAX;
+  uint8_t* b = reinterpret_cast<uint8_t*>(L.qq + QMAX);
+  L.dry = b; L.vw = b + cap; L.qok = b + 2 * cap;
+  return L;
+}
+
+__device__ __forceinline__ int lid(uint64_t w, int j) { return (int)(w >> (16 * j)) & 0xFFFF; }
+
+// side j (known only at run time) of the cell with local vertex ids lv
+__device__ __forceinline__ Side lds_side(const Lds& L, uint64_t lv, int j) {
+  const int a = lid(lv, j), b = lid(lv, j == 2 ? 0 : j + 1);
+  return side_of(L.vx[a], L.vy[a], L.vz[a], L.vx[b], L.vy[b], L.vz[b]);
+}
+
+__device__ __forceinline__ Tri lds_tri(const Lds& L, uint64_t lv) {
+  Tri t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int a = lid(lv, k);
+    t.x[k] = L.vx[a]; t.y[k] = L.vy[a]; t.zv[k] = L.vz[a];
+  }
+  return t;
+}
+
+// inflowQbyH.f90:31-126 (vtype 1) / :221-316 (vtype 5): the fixed point on
+// the wetted length of string ch, by the whole block (edge l of the string
+// is held by thread l % CAP).  The face state of each string edge's cell is
+// formed here from the stage input, as the cell's own tile forms it.  Every
+// sum runs over the edges in string order (the reference's sequential sum);
+// all threads form the sums from shared memory, so the loop exit is uniform
+// across the block.
+template <int CAP>
+__device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const double* gwork,
+                           const Ctrl* c, int ch, const Lds& L, double* s_at, int* s_null,
+                           int* s_skip) {
+  constexpr int QPT = (QMAX + CAP - 1) / CAP;
+  const int t = threadIdx.x;
+  const int vt = g.in_type[ch];
+  const int e0 = g.qs_ptr[ch], n = g.qs_ptr[ch + 1] - e0;
+  const double qin = c->qin[ch];
+  if (vt != 1 && vt != 5) return;
+  if (qin < DBL_MIN) {
+    if (t == 0) s_skip[ch] = 1;
+    __syncthreads();
+    return;
+  }
+  const double h_dry = o.h_dry;
+  double qh[QPT], qu[QPT], qv[QPT];
+  bool qact[QPT];
+#pragma unroll
+  for (int r = 0; r < QPT; ++r) {
+    const int l = t + CAP * r;
+    qh[r] = ZERO; qu[r] = ZERO; qv[r] = ZERO; qact[r] = false;
+    if (l >= n) continue;
+    const int e = e0 + l;
+    const int i = g.qs_cell[e], m = g.qs_slot[e];
     const Tri tr = load_tri(g, i);
-    // neighbour data of the interior sides, gathered up front
-    const uint32_t b0 = slot_bits(meta, 0), b1 = slot_bits(meta, 1), b2 = slot_bits(meta, 2);
-    Nbr n0{}, n1{}, n2{};
-    if (nb0 >= 0) n0 = load_nbr(S, nb0, (b0 >> CMS_NL) & 3);
-    if (nb1 >= 0) n1 = load_nbr(S, nb1, (b1 >> CMS_NL) & 3);
-    if (nb2 >= 0) n2 = load_nbr(S, nb2, (b2 >> CMS_NL) & 3);
-    const double dhx = tail.x, dhy = tail.y;
-    const bool dry_i = tail.z < ZERO;
-    const double umag_in = dry_i ? ZERO : tail.z;  // |u| of the stage input (find_dry_cells)
-    const double hi = tail.w, zi = f0.z;
-    const double xci = si.xc, yci = si.yc;
-    const Side s0 = side<0>(tr), s1 = side<1>(tr), s2 = side<2>(tr);
+    bool vw = true;
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+      const int p = g.cvtx[k * g.N + i];
+      vw = vertex_idw(g, o, din, nullptr, p, g.vxyz[p], c).wet && vw;
+    }
+    Faces F;
+    cell_faces(g, o, din, gwork, i, tr, vw, F);
+    qh[r] = (m == 0) ? F.fh[0] : (m == 1) ? F.fh[1] : F.fh[2];
+    qu[r] = (m == 0) ? F.fu[0] : (m == 1) ? F.fu[1] : F.fu[2];
+    qv[r] = (m == 0) ? F.fv[0] : (m == 1) ? F.fv[1] : F.fv[2];
+    qact[r] = !(vt == 5 && F.dry);
+    // initial wetted length (inflowQbyH.f90:40-56)
+    const double hl = F.dry ? ZERO : dmax(qh[r], ZERO);
+    L.qa[l] = hl * g.qs_len[e];
+    L.qok[l] = qact[r];
+  }
+  __syncthreads();
+  double atotal = ZERO;
+  for (int k = 0; k < n; ++k)
+    if (L.qok[k]) atotal = atotal + L.qa[k];
+  const bool null_area = equals(atotal, ZERO);
+  int counter = 0;
+  double newatotal, qtotal;
+  for (;;) {
+    __syncthreads();
+    const double qfrac = qin / atotal;
+#pragma unroll
+    for (int r = 0; r < QPT; ++r) {
+      const int l = t + CAP * r;
+      if (l >= n) continue;
+      const int e = e0 + l;
+      int ok = 0;
+      if (qact[r]) {
+        double hl = qh[r], ue, ve;
+        if (null_area) hl = g.bch_max;
+        if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
+        else { ue = qu[r]; ve = qv[r]; }
+        const double nx = g.qs_nx[e], ny = g.qs_ny[e], len = g.qs_len[e];
+        double ur = -qfrac;
+        const double ul = ue * nx + ve * ny;
+        const double sq = sqrt(G * hl);
+        double hr = 2.0 * sq + ul - ur;
+        hr = hr * hr / 4.0 / G;
+        ur = -qfrac * nx;
+        const double vr = -qfrac * ny;
+        if (!(hr < h_dry)) {
+          L.qa[l] = hr * len;
+          L.qq[l] = (ur * nx + vr * ny) * hr * len;
+          ok = 1;
+        }
+      }
+      L.qok[l] = ok;
+    }
+    __syncthreads();
+    newatotal = ZERO;
+    qtotal = ZERO;
+    for (int k = 0; k < n; ++k)
+      if (L.qok[k]) { newatotal = newatotal + L.qa[k]; qtotal = qtotal + L.qq[k]; }
+    qtotal = fabs(qtotal);
+    if (fabs(qtotal - qin) / qin < 0.005 || counter > 50) break;
+    counter++;
+    atotal = newatotal;
+  }
+  if (t == 0) {
+    s_at[ch] = newatotal;
+    s_null[ch] = null_area;
+    s_skip[ch] = 0;
+  }
+  __syncthreads();
+}
+
+// One RK stage of one tile (solverFVT.f90:158-331):
+//   1. the IDW water surface and wet flags of the tile's vertices
+//      (to_vertices.f90:26-41, enforce_bcs stage nodes);
+//   2. the face states of the tile and halo cells (find_dry_cells' flags,
+//      cell_grads, hgradfix2, to_vertices' cell pass) into shared memory;
+//   3. the inflowQbyH fixed points of the strings the tile touches;
+//   4. per tile cell: every flux of its three sides, gathered in the
+//      reference's accumulation order (no atomics), bed slope, explicit
+//      friction if chosen, SSP-RK update, clip and wet/dry recovery; in the
+//      last stage also implicit friction, hvalve, the residual partial sums
+//      and the next step's CFL candidate (per-tile partials, reduced in a
+//      fixed order by k_control).
+template <int CAP>
+__global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
+  extern __shared__ double lds_dyn[];
+  __shared__ double s_at[MAX_STRINGS];
+  __shared__ int s_null[MAX_STRINGS], s_skip[MAX_STRINGS];
+  __shared__ double s_dlast;
+  const Ctrl* c = S.ctrl;
+  if (c->done) return;
+  const int b = tl.order[blockIdx.x];
+  const int t = threadIdx.x;
+  const int N = g.N;
+  const int c0 = tl.tptr[b], T = tl.tptr[b + 1] - c0;
+  const int hb = tl.hptr[b], H = tl.hptr[b + 1] - hb;
+  const int vb = tl.tvptr[b], NV = tl.tvptr[b + 1] - vb;
+  const Lds L = lds_layout(lds_dyn, CAP);
+  const double h_dry = o.h_dry, h_wet = o.h_wet;
+
+  // 1. vertices
+  if (t < NV) {
+    const int p = tl.tvert[vb + t];
+    const double4 q = g.vxyz[p];
+    const Vtx v = vertex_idw(g, o, S.din, nullptr, p, q, c);
+    L.vx[t] = q.x; L.vy[t] = q.y; L.vz[t] = q.z; L.vh[t] = v.h;
+    L.vw[t] = v.wet;
+  }
+  if (t == 0 && o.steady && st.last) {
+    s_dlast = last_cell_dt(g, o, S.din);
+    if (blockIdx.x == 0) S.ctrl->delt_last = s_dlast;  // k_control advances ttime by it
+  }
+  __syncthreads();
+
+  // 2. face states of the tile and halo cells
+  Faces F;
+  if (t < T + H) {
+    const bool own = t < T;
+    const int i = own ? c0 + t : tl.hcell[hb + t - T];
+    const uint64_t lv = own ? tl.clv[i] : tl.hlv[hb + t - T];
+    const Tri tr = lds_tri(L, lv);
+    const bool vw = L.vw[lid(lv, 0)] && L.vw[lid(lv, 1)] && L.vw[lid(lv, 2)];
+    cell_faces(g, o, S.din, S.gwork, i, tr, vw, F);
+    if (own && F.fix) {
+      st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1) (hgradfix2.f90:80-81)
+      st.hin[2 * N + i] = ZERO;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      L.fh[j * CAP + t] = F.fh[j]; L.fu[j * CAP + t] = F.fu[j]; L.fv[j * CAP + t] = F.fv[j];
+    }
+    L.z[t] = g.cs[i].z;
+    L.h[t] = F.h;
+    L.dry[t] = F.dry;
+  }
+  __syncthreads();
+
+  // 3. inflow discharge strings
+  for (uint64_t qm = tl.qmask[b]; qm; qm &= qm - 1)
+    bcq_string<CAP>(g, o, S.din, S.gwork, c, __builtin_ctzll(qm), L, s_at, s_null, s_skip);
+
+  // 4. update of the tile cells
+  double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX;
+  if (t < T) {
+    const int i = c0 + t;
+    const uint64_t meta = g.cmeta[i];
+    const uint64_t lv = tl.clv[i], ln = tl.cln[i];
+    const CellS si = ld(&g.cs[i]);
+    const Tri tr = lds_tri(L, lv);
+    const double dhx = F.dhx, dhy = F.dhy, hi = F.h, umag_in = F.umag;
+    const bool dry_i = F.dry;
+    const double zi = si.z, xci = si.xc, yci = si.yc;
     const double area = tri_area(tr);
-    // per-side changes of phi
-    double c00, c01, c02, c10, c11, c12, c20, c21, c22;
-    auto side_flux = [&](uint32_t bits, const Side& sd, const FSlot& fo, const Nbr& nb, int j,
-                         double& d0, double& d1, double& d2) {
+    auto side_flux = [&](uint32_t bits, const Side& sd, int j, double& d0, double& d1,
+                         double& d2) {
+      const double fho = L.fh[j * CAP + t], fuo = L.fu[j * CAP + t], fvo = L.fv[j * CAP + t];
       if ((bits & CMS_KIND) == SK_INTERIOR) {
-        interior_flux(o, bits, sd, dry_i, hi, zi, fo.h, fo.u, fo.v, nb, d0, d1, d2);
+        const int k = lid(ln, j), nl = (bits >> CMS_NL) & 3;
+        Nbr nb;
+        nb.fh = L.fh[nl * CAP + k]; nb.fu = L.fu[nl * CAP + k]; nb.fv = L.fv[nl * CAP + k];
+        nb.z = L.z[k]; nb.h = L.h[k]; nb.dry = L.dry[k];
+        interior_flux(o, bits, sd, dry_i, hi, zi, fho, fuo, fvo, nb, d0, d1, d2);
       } else {
-        const int va = g.cvtx[j * N + i], vb = g.cvtx[((j + 1) % 3) * N + i];
+        const double ha = L.vh[lid(lv, j)], hb2 = L.vh[lid(lv, (j + 1) % 3)];
         const bool rev = (bits >> CMS_REV) & 1;
-        boundary_flux(g, o, S, c, bits, sd, fo.h, fo.u, fo.v, hi, dry_i, rev ? vb : va,
-                      rev ? va : vb, d0, d1, d2);
+        boundary_flux(g, o, c, s_at, s_null, s_skip, bits, sd, fho, fuo, fvo, hi, dry_i,
+                      rev ? hb2 : ha, rev ? ha : hb2, d0, d1, d2);
       }
     };
-    side_flux(b0, s0, f0, n0, 0, c00, c01, c02);
-    side_flux(b1, s1, f1, n1, 1, c10, c11, c12);
-    side_flux(b2, s2, f2, n2, 2, c20, c21, c22);
-    // accumulate in the reference's order (invisc_fluxes: interior edges by
-    // number, walls, boundary routines)
+    // the sides in the reference's accumulation order (invisc_fluxes:
+    // interior edges by number, walls, boundary routines), each added to phi
+    // as it is formed
     double p0 = ZERO, p1 = ZERO, p2 = ZERO;
 #pragma unroll
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
-      const double a0 = (j == 0) ? c00 : (j == 1) ? c10 : c20;
-      const double a1 = (j == 0) ? c01 : (j == 1) ? c11 : c21;
-      const double a2 = (j == 0) ? c02 : (j == 1) ? c12 : c22;
-      p0 = p0 + a0; p1 = p1 + a1; p2 = p2 + a2;
+      double d0, d1, d2;
+      side_flux(slot_bits(meta, j), lds_side(L, lv, j), j, d0, d1, d2);
+      p0 = p0 + d0; p1 = p1 + d1; p2 = p2 + d2;
     }
     if (o.friction == 1) {
       // fric_terms.f90:24-98 (explicit Manning), after the edge fluxes
       double r = dry_i ? ZERO : G * si.cd * si.cd / pow(hi, ONE_THIRD);
       r = -r * umag_in;
-      p1 = p1 + r * S.up[i] * area;
-      p2 = p2 + r * S.vp[i] * area;
+      p1 = p1 + r * F.up * area;
+      p2 = p2 + r * F.vp * area;
     }
     if (!dry_i) {
       // bed_slope_STD.f90:23-70 with adjustZb.f90:41-70 and bed_slope_VTX.f90:32-33
@@ -981,7 +1121,7 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       p1 = p1 - G * hi * gx * area;
       p2 = p2 - G * hi * gy * area;
     }
-    const double perim = s0.len + s1.len + s2.len;
+    const double perim = side<0>(tr).len + side<1>(tr).len + side<2>(tr).len;
     // solverFVT.f90:259-283
     double delt = c->dt;
     if (o.steady) {
@@ -1031,7 +1171,7 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       S.dry_post[i] = dpost;
       double rough = ZERO;
       if (!dpost) {
-        const double fdt = o.steady ? c->delt_last : c->dt;
+        const double fdt = o.steady ? s_dlast : c->dt;
         rough = G * si.cd * si.cd / pow(hn, ONE_THIRD);
         rough = hn / (hn + fdt * rough * um);
       }
@@ -1046,9 +1186,18 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       bot = h0n * h0n;
       // hvalve (solverFVT.f90:357-370): outward normal of the cell's side
       if (o.hvalve) {
-        if ((b0 >> CMS_HBAR) & 1) { if (un * s0.nx + vn * s0.ny < 0) { un = ZERO; vn = ZERO; } }
-        if ((b1 >> CMS_HBAR) & 1) { if (un * s1.nx + vn * s1.ny < 0) { un = ZERO; vn = ZERO; } }
-        if ((b2 >> CMS_HBAR) & 1) { if (un * s2.nx + vn * s2.ny < 0) { un = ZERO; vn = ZERO; } }
+        if ((slot_bits(meta, 0) >> CMS_HBAR) & 1) {
+          const Side sd = side<0>(tr);
+          if (un * sd.nx + vn * sd.ny < 0) { un = ZERO; vn = ZERO; }
+        }
+        if ((slot_bits(meta, 1) >> CMS_HBAR) & 1) {
+          const Side sd = side<1>(tr);
+          if (un * sd.nx + vn * sd.ny < 0) { un = ZERO; vn = ZERO; }
+        }
+        if ((slot_bits(meta, 2) >> CMS_HBAR) & 1) {
+          const Side sd = side<2>(tr);
+          if (un * sd.nx + vn * sd.ny < 0) { un = ZERO; vn = ZERO; }
+        }
       }
       st.out[i] = qh;
       st.out[N + i] = qu;
@@ -1065,33 +1214,35 @@ k_update(Geo g, Opts o, Stage st, State S, double* part) {
       cand = o.cfl * area / ((um2 + wc + SMALL) * perim);
     }
   }
-  if (!st.last || done) return;
-  // block partials: deterministic tree over the block (fixed shape)
-  __shared__ double s_top[UPD_THREADS], s_bot[UPD_THREADS], s_mx[UPD_THREADS];
-  const int t = threadIdx.x;
-  s_top[t] = top; s_bot[t] = bot; s_mx[t] = mxphi;
+  if (!st.last) return;
+  // per-tile partials: fixed tree over the block
   __syncthreads();
-  for (int w = UPD_THREADS / 2; w > 0; w >>= 1) {
+  double* r0 = L.fh;
+  double* r1 = L.fh + CAP;
+  double* r2 = L.fh + 2 * CAP;
+  r0[t] = top; r1[t] = bot; r2[t] = mxphi;
+  __syncthreads();
+  for (int w = CAP / 2; w > 0; w >>= 1) {
     if (t < w) {
-      s_top[t] = s_top[t] + s_top[t + w];
-      s_bot[t] = s_bot[t] + s_bot[t + w];
-      s_mx[t] = dmax(s_mx[t], s_mx[t + w]);
+      r0[t] = r0[t] + r0[t + w];
+      r1[t] = r1[t] + r1[t + w];
+      r2[t] = dmax(r2[t], r2[t + w]);
     }
     __syncthreads();
   }
   if (t == 0) {
-    part[4 * blockIdx.x] = s_top[0];
-    part[4 * blockIdx.x + 1] = s_bot[0];
-    part[4 * blockIdx.x + 2] = s_mx[0];
+    S.part[4 * b] = r0[0];
+    S.part[4 * b + 1] = r1[0];
+    S.part[4 * b + 2] = r2[0];
   }
   __syncthreads();
-  s_top[t] = cand;
+  r0[t] = cand;
   __syncthreads();
-  for (int w = UPD_THREADS / 2; w > 0; w >>= 1) {
-    if (t < w) s_top[t] = dmin(s_top[t], s_top[t + w]);
+  for (int w = CAP / 2; w > 0; w >>= 1) {
+    if (t < w) r0[t] = dmin(r0[t], r0[t + w]);
     __syncthreads();
   }
-  if (t == 0) part[4 * blockIdx.x + 3] = s_top[0];
+  if (t == 0) S.part[4 * b + 3] = r0[0];
 }
 
 // to_vertices' cell pass alone (solverFVT.f90:483 -> to_vertices.f90:44-55),
@@ -1157,8 +1308,8 @@ struct storm_fvt {
   int* d_out_type = nullptr;
   int *d_qts_ptr = nullptr, *d_hts_ptr = nullptr;
   double *d_qts_t = nullptr, *d_qts_v = nullptr, *d_hts_t = nullptr, *d_hts_v = nullptr;
-  int nblk_upd = 0;
-  int block = 256;
+  Tiles tl{};
+  int cap = 0;
   int iter_host = 0;
   bool started = false;
   bool finalized = false;
@@ -1232,6 +1383,13 @@ void upload_tables(storm_fvt* C) {
   C->d_hts_v = dupload(own, P.hts_v);
 }
 
+void set_lds_limit(int cap) {
+  const void* f = (cap == 256) ? reinterpret_cast<const void*>(&k_stage<256>)
+                : (cap == 512) ? reinterpret_cast<const void*>(&k_stage<512>)
+                               : reinterpret_cast<const void*>(&k_stage<1024>);
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(cap)));
+}
+
 void setup_device(storm_fvt* C) {
   Prep& P = C->prep;
   auto& own = C->owned;
@@ -1275,25 +1433,44 @@ void setup_device(storm_fvt* C) {
   const int nslots = (R == 1) ? 2 : R + 1;
   for (int s = 0; s < nslots && s < 4; ++s)
     if (R == 1 || s >= 1) C->slots[s] = dalloc<double>(own, 3 * N);
-  C->block = q.block > 0 ? q.block : 256;
-  if (C->block != 64 && C->block != 128 && C->block != 256)
-    throw StormErr(STORM_E_ARG, "block must be 64, 128 or 256");
-  C->nblk_upd = (int)((N + UPD_THREADS - 1) / UPD_THREADS);
+  // tiles of the stage kernel
+  C->cap = q.block > 0 ? q.block : DEFAULT_CAP;
+  if (C->cap != 256 && C->cap != 512 && C->cap != 1024)
+    throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256, 512 or 1024");
+  if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
+  Tiles& tl = C->tl;
+  tl.order = dupload(own, P.t_order);
+  tl.tptr = dupload(own, P.t_ptr); tl.hptr = dupload(own, P.t_hptr);
+  tl.hcell = dupload(own, P.t_hcell); tl.tvptr = dupload(own, P.t_vptr);
+  tl.tvert = dupload(own, P.t_vert);
+  tl.clv = dupload(own, P.t_clv); tl.cln = dupload(own, P.t_cln);
+  tl.hlv = dupload(own, P.t_hlv); tl.qmask = dupload(own, P.t_qmask);
+  tl.nt = P.nt;
+  set_lds_limit(C->cap);
   State& S = C->S;
-  S.up = dalloc<double>(own, q.friction == 1 ? N : 1);
-  S.vp = dalloc<double>(own, q.friction == 1 ? N : 1);
-  S.f = dalloc<CellF>(own, N);
   S.gwork = (q.gradient == STORM_GRAD_GAUSS) ? dalloc<double>(own, 6 * N) : nullptr;
   S.hvtx = dalloc<double>(own, P.P);
   S.zetavtx = dalloc<double>(own, P.P);
   S.vwet = dalloc<uint8_t>(own, P.P);
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
-  S.part = dalloc<double>(own, 4 * (size_t)C->nblk_upd);
+  S.part = dalloc<double>(own, 4 * (size_t)P.nt);
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
   S.ctrl = dalloc<Ctrl>(own, 1);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
+}
+
+template <int CAP>
+void launch_init(storm_fvt* C) {
+  hipLaunchKernelGGL(k_init<CAP>, dim3(C->tl.nt), dim3(CAP), 0, C->stream, C->g, C->o, C->tl,
+                     C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.part);
+}
+
+template <int CAP>
+void launch_stage(storm_fvt* C, const Stage& st) {
+  hipLaunchKernelGGL(k_stage<CAP>, dim3(C->tl.nt), dim3(CAP), lds_bytes(CAP), C->stream, C->g,
+                     C->o, C->tl, st, C->S);
 }
 
 void start_run(storm_fvt* C) {
@@ -1304,9 +1481,11 @@ void start_run(storm_fvt* C) {
   host_time_bcs(P, 0.0, hc.qin, hc.hbc);
   HIPCHK(hipMemcpyAsync(C->S.ctrl, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, C->stream));
   C->r1_flip = 0;
-  const int nb = (P.N + 255) / 256;
-  hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, C->stream, C->g, C->o, C->slot_ptr(0),
-                     C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.part);
+  switch (C->cap) {
+    case 256: launch_init<256>(C); break;
+    case 512: launch_init<512>(C); break;
+    default: launch_init<1024>(C); break;
+  }
   HIPCHK(hipGetLastError());
   C->S.din = C->d_a;
   C->S.dout = C->d_b;
@@ -1332,7 +1511,7 @@ void rk_table(int R, double a[3][3], double b[3]) {
 
 void launch_control(storm_fvt* C, int begin) {
   hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_THREADS), 0, C->stream, C->g, C->o, C->S.ctrl,
-                     C->S.part, C->nblk_upd, begin, C->d_out_type, C->d_qts_ptr, C->d_qts_t,
+                     C->S.part, C->tl.nt, begin, C->d_out_type, C->d_qts_ptr, C->d_qts_t,
                      C->d_qts_v, C->d_hts_ptr, C->d_hts_t, C->d_hts_v);
 }
 
@@ -1343,9 +1522,6 @@ void launch_step(storm_fvt* C) {
   double a[3][3], b[3];
   rk_table(R, a, b);
   launch_control(C, 1);
-  const int bs = C->block;
-  const int nbc = (N + bs - 1) / bs;
-  const int nbv = (P.P + bs - 1) / bs;
   for (int s = 1; s <= R; ++s) {
     Stage st{};
     for (int j = 0; j < s; ++j) { st.slot[j] = C->slot_ptr(j); st.alpha[j] = a[s - 1][j]; }
@@ -1354,17 +1530,14 @@ void launch_step(storm_fvt* C) {
     st.beta = b[s - 1];
     st.s = s;
     st.last = (s == R);
-    hipLaunchKernelGGL(k_vertex, dim3(nbv), dim3(bs), 0, C->stream, C->g, C->o, C->S.din,
-                       (const uint8_t*)nullptr, C->S.hvtx, C->S.vwet, (double*)nullptr,
-                       C->S.ctrl);
     if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
-      hipLaunchKernelGGL(k_gauss1, dim3(nbc), dim3(bs), 0, C->stream, C->g, C->S.din,
-                         C->S.gwork, C->S.ctrl);
-    hipLaunchKernelGGL(k_grad, dim3(nbc), dim3(bs), 0, C->stream, C->g, C->o, st, C->S);
-    if (P.qs_cell.size() > 0)
-      hipLaunchKernelGGL(k_bcq, dim3(P.n_in), dim3(BCQ_THREADS), 0, C->stream, C->g, C->o, st, C->S);
-    hipLaunchKernelGGL(k_update, dim3(C->nblk_upd), dim3(UPD_THREADS), 0, C->stream, C->g, C->o,
-                       st, C->S, C->S.part);
+      hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
+                         C->S.din, C->S.gwork, C->S.ctrl);
+    switch (C->cap) {
+      case 256: launch_stage<256>(C, st); break;
+      case 512: launch_stage<512>(C, st); break;
+      default: launch_stage<1024>(C, st); break;
+    }
     C->last_din = C->S.din;
     CellD* next = C->S.dout;
     C->S.dout = const_cast<CellD*>(C->S.din);

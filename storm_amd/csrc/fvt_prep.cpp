@@ -11,7 +11,6 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
-#include <queue>
 #include <unordered_map>
 
 namespace storm {
@@ -127,60 +126,41 @@ int edge_table(const storm_mesh_t* m, std::vector<int>& ep, std::vector<int>& ee
   return STORM_OK;
 }
 
-// Reverse Cuthill-McKee over the cell adjacency (edge neighbours), each
-// connected component started from a pseudo-peripheral cell.
-std::vector<int> rcm_order(int N, const std::vector<int>& nb3) {
-  std::vector<int> deg(N, 0);
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j < 3; ++j) deg[i] += nb3[3 * i + j] >= 0;
-  std::vector<int> order;
-  order.reserve(N);
-  std::vector<int> level(N, -1);
-  std::vector<char> placed(N, 0);
-  auto bfs_far = [&](int s, std::vector<int>& touched) {
-    std::queue<int> q;
-    q.push(s);
-    level[s] = 0;
-    touched.push_back(s);
-    int far = s;
-    while (!q.empty()) {
-      int c = q.front(); q.pop();
-      if (level[c] > level[far] || (level[c] == level[far] && deg[c] < deg[far])) far = c;
-      for (int j = 0; j < 3; ++j) {
-        int n = nb3[3 * c + j];
-        if (n >= 0 && level[n] < 0) { level[n] = level[c] + 1; touched.push_back(n); q.push(n); }
-      }
+// Cell order along a Hilbert curve over the centroids (one scale for both
+// axes, 2^16 cells a side): runs of consecutive cells are compact patches,
+// which is what the stage kernel's tiles are.
+uint64_t hilbert_index(uint32_t x, uint32_t y) {
+  const uint32_t n = 1u << 16;
+  uint64_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
+      std::swap(x, y);
     }
-    return far;
-  };
-  for (int seed = 0; seed < N; ++seed) {
-    if (placed[seed]) continue;
-    std::vector<int> touched;
-    int s = seed;
-    for (int it = 0; it < 2; ++it) {
-      int f = bfs_far(s, touched);
-      for (int t : touched) level[t] = -1;
-      touched.clear();
-      s = f;
-    }
-    std::vector<int> comp;
-    std::queue<int> q;
-    q.push(s);
-    placed[s] = 1;
-    while (!q.empty()) {
-      int c = q.front(); q.pop();
-      comp.push_back(c);
-      int nbs[3], k = 0;
-      for (int j = 0; j < 3; ++j) {
-        int n = nb3[3 * c + j];
-        if (n >= 0 && !placed[n]) nbs[k++] = n;
-      }
-      std::sort(nbs, nbs + k, [&](int a, int b) { return deg[a] < deg[b] || (deg[a] == deg[b] && a < b); });
-      for (int t = 0; t < k; ++t) { placed[nbs[t]] = 1; q.push(nbs[t]); }
-    }
-    std::reverse(comp.begin(), comp.end());
-    order.insert(order.end(), comp.begin(), comp.end());
   }
+  return d;
+}
+
+std::vector<int> hilbert_order(int N, const std::vector<double>& xc, const std::vector<double>& yc) {
+  double x0 = xc[0], x1 = xc[0], y0 = yc[0], y1 = yc[0];
+  for (int i = 1; i < N; ++i) {
+    x0 = std::min(x0, xc[i]); x1 = std::max(x1, xc[i]);
+    y0 = std::min(y0, yc[i]); y1 = std::max(y1, yc[i]);
+  }
+  double span = std::max(x1 - x0, y1 - y0);
+  if (!(span > 0.0)) span = 1.0;
+  const double f = 65535.0 / span;
+  std::vector<std::pair<uint64_t, int>> key(N);
+  for (int i = 0; i < N; ++i) {
+    const uint32_t xi = (uint32_t)std::min(65535.0, std::max(0.0, (xc[i] - x0) * f));
+    const uint32_t yi = (uint32_t)std::min(65535.0, std::max(0.0, (yc[i] - y0) * f));
+    key[i] = {hilbert_index(xi, yi), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> order(N);
+  for (int i = 0; i < N; ++i) order[i] = key[i].second;
   return order;
 }
 
@@ -540,7 +520,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
 
   // ---- cell ordering --------------------------------------------------
   std::vector<int> order;
-  if (o->reorder) order = rcm_order(N, t2t3);
+  if (o->reorder) order = hilbert_order(N, xc, yc);
   else { order.resize(N); std::iota(order.begin(), order.end(), 0); }
   S.perm = order;
   S.iperm.assign(N, 0);
@@ -658,6 +638,118 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     S.qs_ptr[ch + 1] = (int)S.qs_cell.size();
   }
   S.last_cell = ip[N - 1];
+  return STORM_OK;
+}
+
+int build_tiles(Prep& S, int cap) {
+  const int N = S.N, P = S.P;
+  if (cap < 16 || cap > 0xFFFF) { S.err = "bad tile capacity"; return STORM_E_ARG; }
+  const int* cv = S.cvtx.data();
+  const int* cn = S.cnbr.data();
+  // greedy runs of consecutive cells: grow while the run, its halo of edge
+  // neighbours and the vertices of both fit the capacity
+  std::vector<int> hstamp(N, -1), vstamp(P, -1);
+  S.t_ptr.assign(1, 0);
+  int start = 0, tile = 0;
+  while (start < N) {
+    int T = 0, H = 0, NV = 0;
+    while (start + T < N) {
+      const int c = start + T;
+      int newv[12], nnv = 0, newh[3], nnh = 0;
+      auto addv = [&](int v) {
+        if (vstamp[v] == tile) return;
+        for (int k = 0; k < nnv; ++k) if (newv[k] == v) return;
+        newv[nnv++] = v;
+      };
+      for (int j = 0; j < 3; ++j) addv(cv[(size_t)j * N + c]);
+      const bool was_halo = hstamp[c] == tile;
+      for (int j = 0; j < 3; ++j) {
+        const int nb = cn[(size_t)j * N + c];
+        if (nb < 0 || (nb >= start && nb <= c) || hstamp[nb] == tile) continue;
+        bool dup = false;
+        for (int k = 0; k < nnh; ++k) dup = dup || newh[k] == nb;
+        if (dup) continue;
+        newh[nnh++] = nb;
+        for (int k = 0; k < 3; ++k) addv(cv[(size_t)k * N + nb]);
+      }
+      const int T2 = T + 1, H2 = H + nnh - (was_halo ? 1 : 0), NV2 = NV + nnv;
+      if (T2 + H2 > cap || NV2 > cap) break;
+      for (int k = 0; k < nnv; ++k) vstamp[newv[k]] = tile;
+      for (int k = 0; k < nnh; ++k) hstamp[newh[k]] = tile;
+      if (was_halo) hstamp[c] = -1;
+      T = T2; H = H2; NV = NV2;
+    }
+    if (T == 0) { S.err = "tile capacity too small for the mesh"; return STORM_E_ARG; }
+    start += T;
+    S.t_ptr.push_back(start);
+    ++tile;
+  }
+  const int nt = tile;
+  S.nt = nt;
+  S.cap = cap;
+  // per tile: halo and vertex lists (ascending ids), 16-bit local ids
+  S.t_hptr.assign(1, 0); S.t_hcell.clear(); S.t_hlv.clear();
+  S.t_vptr.assign(1, 0); S.t_vert.clear();
+  S.t_clv.assign(N, 0); S.t_cln.assign(N, 0);
+  S.t_qmask.assign(nt, 0);
+  std::vector<int> vloc(P, -1), hloc(N, -1);
+  std::vector<int> hl, vl;
+  for (int b = 0; b < nt; ++b) {
+    const int c0 = S.t_ptr[b], c1 = S.t_ptr[b + 1], T = c1 - c0;
+    hl.clear(); vl.clear();
+    for (int c = c0; c < c1; ++c)
+      for (int j = 0; j < 3; ++j) {
+        const int nb = cn[(size_t)j * N + c];
+        if (nb >= 0 && (nb < c0 || nb >= c1)) hl.push_back(nb);
+      }
+    std::sort(hl.begin(), hl.end());
+    hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+    for (int c = c0; c < c1; ++c)
+      for (int j = 0; j < 3; ++j) vl.push_back(cv[(size_t)j * N + c]);
+    for (int c : hl)
+      for (int j = 0; j < 3; ++j) vl.push_back(cv[(size_t)j * N + c]);
+    std::sort(vl.begin(), vl.end());
+    vl.erase(std::unique(vl.begin(), vl.end()), vl.end());
+    if (T + (int)hl.size() > cap || (int)vl.size() > cap) {
+      S.err = "tile construction exceeded its capacity"; return STORM_E_MESH;
+    }
+    for (int k = 0; k < (int)vl.size(); ++k) vloc[vl[k]] = k;
+    for (int k = 0; k < (int)hl.size(); ++k) hloc[hl[k]] = T + k;
+    auto lvw = [&](int c) {
+      uint64_t w = 0;
+      for (int j = 0; j < 3; ++j) w |= (uint64_t)vloc[cv[(size_t)j * N + c]] << (16 * j);
+      return w;
+    };
+    for (int c = c0; c < c1; ++c) {
+      S.t_clv[c] = lvw(c);
+      uint64_t w = 0;
+      for (int j = 0; j < 3; ++j) {
+        const int nb = cn[(size_t)j * N + c];
+        const uint64_t l = nb < 0 ? 0xFFFFu : (nb >= c0 && nb < c1) ? (uint64_t)(nb - c0) : (uint64_t)hloc[nb];
+        w |= l << (16 * j);
+        const uint32_t bits = (uint32_t)(S.cmeta[c] >> (CM_SLOT0 + CM_SLOT_BITS * j));
+        const uint32_t kind = bits & CMS_KIND;
+        if (kind == SK_IN_Q1 || kind == SK_IN_Q5) S.t_qmask[b] |= 1ull << ((bits >> CMS_CHAIN) & 0x3F);
+      }
+      S.t_cln[c] = w;
+    }
+    for (int c : hl) { S.t_hcell.push_back(c); S.t_hlv.push_back(lvw(c)); }
+    for (int v : vl) S.t_vert.push_back(v);
+    S.t_hptr.push_back((int)S.t_hcell.size());
+    S.t_vptr.push_back((int)S.t_vert.size());
+  }
+  // block -> tile: each of the 8 XCDs (blocks are dealt to them round-robin)
+  // takes a contiguous range of tiles, for L2 reuse of the shared halos; the
+  // tiles that run an inflow fixed point go first
+  const int NX = 8, q = nt / NX, r = nt % NX;
+  S.t_order.resize(nt);
+  for (int blk = 0; blk < nt; ++blk) {
+    const int x = blk % NX, k = blk / NX;
+    S.t_order[blk] = x * q + (x < r ? x : r) + k;
+  }
+  int front = 0;
+  for (int blk = 0; blk < nt; ++blk)
+    if (S.t_qmask[S.t_order[blk]]) std::swap(S.t_order[blk], S.t_order[front++]);
   return STORM_OK;
 }
 

@@ -1,7 +1,8 @@
 // Host-side set-up of the FVT step: the mesh tables of the reference's
 // find_edges/t_areas/walls/ibc_arrays/node_db/elemnt_db and the
-// coefficients of dataprepFVT, laid out for the device kernels (SoA, cells in
-// Reverse Cuthill-McKee order, per-cell edge slots in local order).
+// coefficients of dataprepFVT, laid out for the device kernels (cells in
+// Hilbert-curve order, per-cell edge slots in local order), and the tiles of
+// the stage kernel.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -67,12 +68,26 @@ struct Prep {
   double bch_max = 0.0;
   int last_cell = 0;                  // internal id of input cell N-1
   std::vector<double> h0, u0, v0;     // initial state (internal order)
+  // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
+  // (edge neighbours outside) and the vertices of both are listed per tile;
+  // local ids are 16 bits, three to a word (cell slot j at bit 16*j):
+  // tile cells 0..T-1, halo cells T.., vertices in list order
+  int cap = 0, nt = 0;
+  std::vector<int> t_order;           // block -> tile
+  std::vector<int> t_ptr, t_hptr, t_hcell, t_vptr, t_vert;
+  std::vector<uint64_t> t_clv, t_cln; // per cell: local vertex ids, local neighbour ids (0xFFFF: none)
+  std::vector<uint64_t> t_hlv;        // per halo entry: local vertex ids
+  std::vector<uint64_t> t_qmask;      // per tile: inflowQbyH strings it touches
   std::string err;
 };
 
 // Builds everything from the C-ABI mesh; returns STORM_OK or a STORM_E_* code
 // with P.err set.
 int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& P);
+
+// Tiles of at most `cap` cells + halo cells and `cap` vertices (needs the
+// cell tables of build_prep).
+int build_tiles(Prep& P, int cap);
 
 // Least-squares molecule only (elemnt_db.f90:42-128), input numbering.
 int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int>& cells,

@@ -1,0 +1,53 @@
+"""Host set-up of the stage kernel's tiles (no GPU): every tile fits its
+capacity and every local id the kernel dereferences names the right cell or
+vertex, for the small cases and the 2M-triangle meander reach."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from storm_amd import cases, fvt
+from cases_small import small_cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("probe") / "libprobe.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out,
+                           os.path.join(ROOT, "tests", "prep_probe.cpp"),
+                           os.path.join(ROOT, "storm_amd", "csrc", "fvt_prep.cpp")])
+    lib = C.CDLL(out)
+    lib.probe_tiles.restype = C.c_int
+    lib.probe_tiles.argtypes = [C.POINTER(fvt.Mesh), C.POINTER(fvt.Opts), C.c_int,
+                                C.POINTER(C.c_int), C.c_char_p, C.c_int]
+    return lib
+
+
+def run_probe(lib, case, cap):
+    keep = {}
+    m = fvt.mesh_struct(case, keep)
+    o = fvt.opts_struct(case.opts)
+    stats = (C.c_int * 4)()
+    msg = C.create_string_buffer(256)
+    rc = lib.probe_tiles(C.byref(m), C.byref(o), cap, stats, msg, 256)
+    assert rc == 0, msg.value.decode()
+    return list(stats)
+
+
+@pytest.mark.parametrize("cap", [256, 512, 1024])
+def test_small_case_tiles(probe, cap):
+    for name, (case, _) in small_cases().items():
+        nt, maxw, maxv, halo = run_probe(probe, case, cap)
+        assert 1 <= nt <= case.n_elems and maxw <= cap and maxv <= cap, name
+
+
+def test_meander_tiles(probe):
+    case = cases.meander()
+    nt, maxw, maxv, halo = run_probe(probe, case, 512)
+    # compact Hilbert patches: the halo stays a small share of the tile cells
+    assert halo < 0.3 * case.n_elems
+    assert maxw <= 512 and maxv <= 512
