@@ -1,11 +1,12 @@
 // MI355X (gfx950) FP64 kernels of the SToRM FVT time step and the C ABI
 // declared in include/storm_fvt.h.
 //
-// Per RK stage (solverFVT.f90:158-331) one launch, k_stage, one block per
-// tile (a run of Hilbert-ordered cells, its halo of edge neighbours and the
-// vertices of both):
-//   - the IDW water surface and wet flags of the tile's vertices
-//     (to_vertices.f90:26-41 with the stage strings' nodes of enforce_bcs);
+// Per RK stage (solverFVT.f90:158-331) two launches:
+//   k_vertex  the IDW water surface, depth and wet flag of every vertex
+//             (to_vertices.f90:26-41 with the stage strings' nodes of
+//             enforce_bcs);
+//   k_stage   one block per tile (a run of Hilbert-ordered cells and its
+//             halo of edge neighbours):
 //   - find_dry_cells' flags, cell_grads (LSQ/Gauss + Barth-Jespersen),
 //     hgradfix2 and to_vertices' cell pass for the tile and halo cells, whose
 //     face states stay in shared memory;
@@ -88,22 +89,20 @@ struct Stage {
 struct State {
   const CellD* din;   // state at the start of the stage
   CellD* dout;        // state for the next stage
+  double* zeta;       // water surface of the next stage's input, NaN at dry cells
   double *gwork, *hvtx, *zetavtx, *phi, *part;
   uint8_t *dry_post, *vwet;
   Ctrl* ctrl;
 };
 
 // Tiles of the stage kernel: tile b owns cells [tptr[b], tptr[b+1]) (a run of
-// the Hilbert-ordered cells), its halo of edge neighbours outside the run is
-// hcell[hptr[b]..hptr[b+1]), and the vertices of both are
-// tvert[tvptr[b]..tvptr[b+1]).  Local ids are 16 bits, three to a word (cell
-// slot j at bit 16*j): tile cells 0..T-1, then halo cells; vertices in list
-// order.
+// the Hilbert-ordered cells) and its halo of edge neighbours outside the run
+// is hcell[hptr[b]..hptr[b+1]).  Local cell ids are 16 bits, three to a word
+// (cell slot j at bit 16*j): tile cells 0..T-1, then halo cells.
 struct Tiles {
   const int* order;             // block -> tile
-  const int *tptr, *hptr, *hcell, *tvptr, *tvert;
-  const uint64_t *clv, *cln;    // per cell: local vertex ids, local neighbour ids
-  const uint64_t* hlv;          // per halo entry: local vertex ids
+  const int *tptr, *hptr, *hcell;
+  const uint64_t* cln;          // per cell: local neighbour ids (0xFFFF: none)
   const uint64_t* qmask;        // per tile: inflowQbyH strings it touches
   int nt;
 };
@@ -205,7 +204,7 @@ __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_
 template <int CAP>
 __global__ void __launch_bounds__(CAP)
 k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u0,
-       const double* v0, CellD* d, double* part) {
+       const double* v0, CellD* d, double* zeta, double* part) {
   // solverFVT.f90:123-127 and the first find_dry_cells, one block per tile;
   // the CFL candidates of step 1 go to the per-tile partials (slot 3),
   // reduced by k_control
@@ -222,6 +221,7 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
     double um;
     const bool dr = find_dry(h, uu, vv, o.h_dry, o.h_wet, &um);
     st4(&d[i], h, uu, vv, dr ? ONE : ZERO);
+    zeta[i] = dr ? __builtin_nan("") : h + g.cs[i].z;
     const double wc = dr ? ZERO : sqrt(G * h);
     const Tri tr = load_tri(g, i);
     const double area = tri_area(tr);
@@ -401,17 +401,61 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
   return v;
 }
 
-// All vertices: the pass after the time loop (solverFVT.f90:483), which also
-// forms zetavtx.
+// The vertex pass of every stage: depths for the wall fluxes and wet flags
+// for the cells, from the stage input's water surface (NaN marks a dry cell).
+__global__ void __launch_bounds__(256)
+k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* c) {
+  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
+  const double4 q = g.vxyz[p];
+  const int vb = g.vbc[p];
+  if (c->done) return;
+  double id = ZERO, a = ZERO, zr = q.z;
+  int counter = 0;
+  for (int t = t0; t < t1; t += 4) {
+    int k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = (t + u < t1) ? g.vcell[t + u] : -1;
+    double zk[4];
+    double2 ck[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k[u] < 0 ? k[0] : k[u];
+      zk[u] = zeta[kk];
+      ck[u] = *reinterpret_cast<const double2*>(&g.cs[kk]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k[u] < 0 || zk[u] != zk[u]) continue;
+      counter++;
+      const double xk = ck[u].x - q.x, yk = ck[u].y - q.y;
+      const double w = ONE / sqrt(xk * xk + yk * yk);
+      id = id + w;
+      a = a + w * zk[u];
+      zr = dmax(zk[u], zr);
+    }
+  }
+  double zeta_v = (counter > 0) ? a / id : q.z;
+  zeta_v = dmin(zeta_v, zr);
+  double hv = zeta_v - q.z;
+  if (hv < ZERO) hv = ZERO;
+  vwet[p] = hv > o.h_dry;
+  if (vb >= 0) hv = dmax(ZERO, c->hbc[vb] - q.z);
+  hvtx[p] = hv;
+}
+
+// All vertices after the time loop (solverFVT.f90:483), with zetavtx.
 __global__ void __launch_bounds__(256)
 k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_t* vwet,
          double* zetavtx, const Ctrl* c) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
+  if (c->done) return;
   const Vtx v = vertex_idw(g, o, d, dry, p, g.vxyz[p], c);
   vwet[p] = v.wet;
   hvtx[p] = v.h;
-  zetavtx[p] = v.zeta;
+  if (zetavtx) zetavtx[p] = v.zeta;
 }
 
 // ---------------------------------------------------------------- Green-Gauss pass 1
@@ -823,49 +867,44 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
 }
 
 // ---------------------------------------------------------------- stage kernel
-// Shared-memory layout of one tile (CAP cells + halo, CAP vertices) and the
-// inflow-string buffers.
+// Shared-memory layout of one tile (CAP cells + halo).  The inflowQbyH
+// buffers of a string (QMAX edges) overlay the cell arrays, which the fixed
+// point runs before.
 constexpr int QMAX = 512;  // edges of one inflowQbyH string
 
 struct Lds {
   double *fh, *fu, *fv;          // [3][CAP] face states, side-major
   double *z, *h;                 // [CAP] bed and stage-input depth
-  double *vx, *vy, *vz, *vh;     // [CAP] tile vertices: x, y, bed, depth
-  double *qa, *qq;               // [QMAX] string edges: area and discharge terms
-  uint8_t *dry, *vw, *qok;
+  double *vx, *vy, *vz;          // [3][CAP] vertex coordinates and bed of the tile cells
+  uint8_t* dry;                  // [CAP] dry flags
+  double *qh, *qu, *qv, *qa, *qq;  // [QMAX] string edges (overlay)
+  uint8_t* qok;                    // [QMAX] (overlay)
 };
 
 __host__ __device__ constexpr size_t lds_bytes(int cap) {
-  return sizeof(double) * (15 * (size_t)cap + 2 * (size_t)QMAX) + 2 * (size_t)cap + (size_t)QMAX;
+  return sizeof(double) * 20 * (size_t)cap + (size_t)cap;
 }
 
 __device__ __forceinline__ Lds lds_layout(double* base, int cap) {
   Lds L;
   L.fh = base; L.fu = L.fh + 3 * cap; L.fv = L.fu + 3 * cap;
   L.z = L.fv + 3 * cap; L.h = L.z + cap;
-  L.vx = L.h + cap; L.vy = L.vx + cap; L.vz = L.vy + cap; L.vh = L.vz + cap;
-  L.qa = L.vh + cap; L.qq = L.qa + QMAX;
-  uint8_t* b = reinterpret_cast<uint8_t*>(L.qq + QMAX);
-  L.dry = b; L.vw = b + cap; L.qok = b + 2 * cap;
+  L.vx = L.h + cap; L.vy = L.vx + 3 * cap; L.vz = L.vy + 3 * cap;
+  L.dry = reinterpret_cast<uint8_t*>(L.vz + 3 * cap);
+  // overlay of the 20*cap cell doubles: 5*QMAX doubles and QMAX flags
+  L.qh = base; L.qu = L.qh + QMAX; L.qv = L.qu + QMAX; L.qa = L.qv + QMAX; L.qq = L.qa + QMAX;
+  L.qok = reinterpret_cast<uint8_t*>(L.qq + QMAX);
   return L;
 }
+static_assert(5 * QMAX * 8 + QMAX <= 20 * 256 * 8, "string buffers must fit the cell arrays");
 
 __device__ __forceinline__ int lid(uint64_t w, int j) { return (int)(w >> (16 * j)) & 0xFFFF; }
 
-// side j (known only at run time) of the cell with local vertex ids lv
-__device__ __forceinline__ Side lds_side(const Lds& L, uint64_t lv, int j) {
-  const int a = lid(lv, j), b = lid(lv, j == 2 ? 0 : j + 1);
+// side j (known only at run time) of tile cell t
+template <int CAP>
+__device__ __forceinline__ Side side_dyn(const Lds& L, int t, int j) {
+  const int a = j * CAP + t, b = (j == 2 ? 0 : j + 1) * CAP + t;
   return side_of(L.vx[a], L.vy[a], L.vz[a], L.vx[b], L.vy[b], L.vz[b]);
-}
-
-__device__ __forceinline__ Tri lds_tri(const Lds& L, uint64_t lv) {
-  Tri t;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int a = lid(lv, k);
-    t.x[k] = L.vx[a]; t.y[k] = L.vy[a]; t.zv[k] = L.vz[a];
-  }
-  return t;
 }
 
 // inflowQbyH.f90:31-126 (vtype 1) / :221-316 (vtype 5): the fixed point on
@@ -877,9 +916,8 @@ __device__ __forceinline__ Tri lds_tri(const Lds& L, uint64_t lv) {
 // across the block.
 template <int CAP>
 __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const double* gwork,
-                           const Ctrl* c, int ch, const Lds& L, double* s_at, int* s_null,
-                           int* s_skip) {
-  constexpr int QPT = (QMAX + CAP - 1) / CAP;
+                           const uint8_t* vwet, const Ctrl* c, int ch, const Lds& L,
+                           double* s_at, int* s_null, int* s_skip) {
   const int t = threadIdx.x;
   const int vt = g.in_type[ch];
   const int e0 = g.qs_ptr[ch], n = g.qs_ptr[ch + 1] - e0;
@@ -891,54 +929,45 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
     return;
   }
   const double h_dry = o.h_dry;
-  double qh[QPT], qu[QPT], qv[QPT];
-  bool qact[QPT];
-#pragma unroll
-  for (int r = 0; r < QPT; ++r) {
-    const int l = t + CAP * r;
-    qh[r] = ZERO; qu[r] = ZERO; qv[r] = ZERO; qact[r] = false;
-    if (l >= n) continue;
+  // qok: bit 1 the edge takes part (vtype 5 skips dry cells), bit 0 its
+  // terms enter the current sums
+  for (int l = t; l < n; l += CAP) {
     const int e = e0 + l;
     const int i = g.qs_cell[e], m = g.qs_slot[e];
     const Tri tr = load_tri(g, i);
-    bool vw = true;
-#pragma unroll 1
-    for (int k = 0; k < 3; ++k) {
-      const int p = g.cvtx[k * g.N + i];
-      vw = vertex_idw(g, o, din, nullptr, p, g.vxyz[p], c).wet && vw;
-    }
+    const int N = g.N;
+    const bool vw = vwet[g.cvtx[i]] && vwet[g.cvtx[N + i]] && vwet[g.cvtx[2 * N + i]];
     Faces F;
     cell_faces(g, o, din, gwork, i, tr, vw, F);
-    qh[r] = (m == 0) ? F.fh[0] : (m == 1) ? F.fh[1] : F.fh[2];
-    qu[r] = (m == 0) ? F.fu[0] : (m == 1) ? F.fu[1] : F.fu[2];
-    qv[r] = (m == 0) ? F.fv[0] : (m == 1) ? F.fv[1] : F.fv[2];
-    qact[r] = !(vt == 5 && F.dry);
+    const double fh = (m == 0) ? F.fh[0] : (m == 1) ? F.fh[1] : F.fh[2];
+    L.qh[l] = fh;
+    L.qu[l] = (m == 0) ? F.fu[0] : (m == 1) ? F.fu[1] : F.fu[2];
+    L.qv[l] = (m == 0) ? F.fv[0] : (m == 1) ? F.fv[1] : F.fv[2];
+    const bool act = !(vt == 5 && F.dry);
     // initial wetted length (inflowQbyH.f90:40-56)
-    const double hl = F.dry ? ZERO : dmax(qh[r], ZERO);
+    const double hl = F.dry ? ZERO : dmax(fh, ZERO);
     L.qa[l] = hl * g.qs_len[e];
-    L.qok[l] = qact[r];
+    L.qok[l] = act ? 3 : 0;  // bit 1: takes part in every pass
   }
   __syncthreads();
   double atotal = ZERO;
   for (int k = 0; k < n; ++k)
-    if (L.qok[k]) atotal = atotal + L.qa[k];
+    if (L.qok[k] & 1) atotal = atotal + L.qa[k];
   const bool null_area = equals(atotal, ZERO);
   int counter = 0;
   double newatotal, qtotal;
   for (;;) {
     __syncthreads();
     const double qfrac = qin / atotal;
-#pragma unroll
-    for (int r = 0; r < QPT; ++r) {
-      const int l = t + CAP * r;
-      if (l >= n) continue;
+    for (int l = t; l < n; l += CAP) {
       const int e = e0 + l;
+      const int act = L.qok[l] & 2;
       int ok = 0;
-      if (qact[r]) {
-        double hl = qh[r], ue, ve;
+      if (act) {
+        double hl = L.qh[l], ue, ve;
         if (null_area) hl = g.bch_max;
         if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
-        else { ue = qu[r]; ve = qv[r]; }
+        else { ue = L.qu[l]; ve = L.qv[l]; }
         const double nx = g.qs_nx[e], ny = g.qs_ny[e], len = g.qs_len[e];
         double ur = -qfrac;
         const double ul = ue * nx + ve * ny;
@@ -953,13 +982,13 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
           ok = 1;
         }
       }
-      L.qok[l] = ok;
+      L.qok[l] = (uint8_t)(act | ok);
     }
     __syncthreads();
     newatotal = ZERO;
     qtotal = ZERO;
     for (int k = 0; k < n; ++k)
-      if (L.qok[k]) { newatotal = newatotal + L.qa[k]; qtotal = qtotal + L.qq[k]; }
+      if (L.qok[k] & 1) { newatotal = newatotal + L.qa[k]; qtotal = qtotal + L.qq[k]; }
     qtotal = fabs(qtotal);
     if (fabs(qtotal - qin) / qin < 0.005 || counter > 50) break;
     counter++;
@@ -973,12 +1002,11 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
   __syncthreads();
 }
 
-// One RK stage of one tile (solverFVT.f90:158-331):
-//   1. the IDW water surface and wet flags of the tile's vertices
-//      (to_vertices.f90:26-41, enforce_bcs stage nodes);
-//   2. the face states of the tile and halo cells (find_dry_cells' flags,
+// One RK stage of one tile (solverFVT.f90:158-331), after k_vertex has
+// formed the vertex depths and wet flags of the stage:
+//   2. the inflowQbyH fixed points of the strings the tile touches;
+//   3. the face states of the tile and halo cells (find_dry_cells' flags,
 //      cell_grads, hgradfix2, to_vertices' cell pass) into shared memory;
-//   3. the inflowQbyH fixed points of the strings the tile touches;
 //   4. per tile cell: every flux of its three sides, gathered in the
 //      reference's accumulation order (no atomics), bed slope, explicit
 //      friction if chosen, SSP-RK update, clip and wet/dry recovery; in the
@@ -998,33 +1026,32 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   const int N = g.N;
   const int c0 = tl.tptr[b], T = tl.tptr[b + 1] - c0;
   const int hb = tl.hptr[b], H = tl.hptr[b + 1] - hb;
-  const int vb = tl.tvptr[b], NV = tl.tvptr[b + 1] - vb;
   const Lds L = lds_layout(lds_dyn, CAP);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
-
-  // 1. vertices
-  if (t < NV) {
-    const int p = tl.tvert[vb + t];
-    const double4 q = g.vxyz[p];
-    const Vtx v = vertex_idw(g, o, S.din, nullptr, p, q, c);
-    L.vx[t] = q.x; L.vy[t] = q.y; L.vz[t] = q.z; L.vh[t] = v.h;
-    L.vw[t] = v.wet;
-  }
   if (t == 0 && o.steady && st.last) {
     s_dlast = last_cell_dt(g, o, S.din);
     if (blockIdx.x == 0) S.ctrl->delt_last = s_dlast;  // k_control advances ttime by it
   }
-  __syncthreads();
 
-  // 2. face states of the tile and halo cells
+  // 2. inflow discharge strings (their buffers overlay the cell arrays)
+  for (uint64_t qm = tl.qmask[b]; qm; qm &= qm - 1)
+    bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, c, __builtin_ctzll(qm), L, s_at, s_null,
+                    s_skip);
+
+  // 3. face states of the tile and halo cells
   Faces F;
   if (t < T + H) {
     const bool own = t < T;
     const int i = own ? c0 + t : tl.hcell[hb + t - T];
-    const uint64_t lv = own ? tl.clv[i] : tl.hlv[hb + t - T];
-    const Tri tr = lds_tri(L, lv);
-    const bool vw = L.vw[lid(lv, 0)] && L.vw[lid(lv, 1)] && L.vw[lid(lv, 2)];
+    const Tri tr = load_tri(g, i);
+    const bool vw = S.vwet[g.cvtx[i]] && S.vwet[g.cvtx[N + i]] && S.vwet[g.cvtx[2 * N + i]];
     cell_faces(g, o, S.din, S.gwork, i, tr, vw, F);
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        L.vx[k * CAP + t] = tr.x[k]; L.vy[k * CAP + t] = tr.y[k]; L.vz[k * CAP + t] = tr.zv[k];
+      }
+    }
     if (own && F.fix) {
       st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1) (hgradfix2.f90:80-81)
       st.hin[2 * N + i] = ZERO;
@@ -1039,18 +1066,18 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   }
   __syncthreads();
 
-  // 3. inflow discharge strings
-  for (uint64_t qm = tl.qmask[b]; qm; qm &= qm - 1)
-    bcq_string<CAP>(g, o, S.din, S.gwork, c, __builtin_ctzll(qm), L, s_at, s_null, s_skip);
-
   // 4. update of the tile cells
   double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX;
   if (t < T) {
     const int i = c0 + t;
     const uint64_t meta = g.cmeta[i];
-    const uint64_t lv = tl.clv[i], ln = tl.cln[i];
+    const uint64_t ln = tl.cln[i];
     const CellS si = ld(&g.cs[i]);
-    const Tri tr = lds_tri(L, lv);
+    Tri tr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      tr.x[k] = L.vx[k * CAP + t]; tr.y[k] = L.vy[k * CAP + t]; tr.zv[k] = L.vz[k * CAP + t];
+    }
     const double dhx = F.dhx, dhy = F.dhy, hi = F.h, umag_in = F.umag;
     const bool dry_i = F.dry;
     const double zi = si.z, xci = si.xc, yci = si.yc;
@@ -1065,7 +1092,8 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
         nb.z = L.z[k]; nb.h = L.h[k]; nb.dry = L.dry[k];
         interior_flux(o, bits, sd, dry_i, hi, zi, fho, fuo, fvo, nb, d0, d1, d2);
       } else {
-        const double ha = L.vh[lid(lv, j)], hb2 = L.vh[lid(lv, (j + 1) % 3)];
+        const int va = g.cvtx[j * N + i], vb = g.cvtx[(j == 2 ? 0 : j + 1) * N + i];
+        const double ha = S.hvtx[va], hb2 = S.hvtx[vb];
         const bool rev = (bits >> CMS_REV) & 1;
         boundary_flux(g, o, c, s_at, s_null, s_skip, bits, sd, fho, fuo, fvo, hi, dry_i,
                       rev ? hb2 : ha, rev ? ha : hb2, d0, d1, d2);
@@ -1079,7 +1107,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
       double d0, d1, d2;
-      side_flux(slot_bits(meta, j), lds_side(L, lv, j), j, d0, d1, d2);
+      side_flux(slot_bits(meta, j), side_dyn<CAP>(L, t, j), j, d0, d1, d2);
       p0 = p0 + d0; p1 = p1 + d1; p2 = p2 + d2;
     }
     if (o.friction == 1) {
@@ -1164,6 +1192,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
       double um;
       const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um);
       st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
+      S.zeta[i] = dn ? __builtin_nan("") : hn + zi;
     } else {
       // find_dry_cells + fric_terms_impl (solverFVT.f90:335-336)
       double um;
@@ -1210,6 +1239,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
       double um2;
       const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um2);
       st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
+      S.zeta[i] = dn ? __builtin_nan("") : hn + zi;
       const double wc = dn ? ZERO : sqrt(G * hn);
       cand = o.cfl * area / ((um2 + wc + SMALL) * perim);
     }
@@ -1385,8 +1415,7 @@ void upload_tables(storm_fvt* C) {
 
 void set_lds_limit(int cap) {
   const void* f = (cap == 256) ? reinterpret_cast<const void*>(&k_stage<256>)
-                : (cap == 512) ? reinterpret_cast<const void*>(&k_stage<512>)
-                               : reinterpret_cast<const void*>(&k_stage<1024>);
+                               : reinterpret_cast<const void*>(&k_stage<512>);
   HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(cap)));
 }
 
@@ -1435,16 +1464,15 @@ void setup_device(storm_fvt* C) {
     if (R == 1 || s >= 1) C->slots[s] = dalloc<double>(own, 3 * N);
   // tiles of the stage kernel
   C->cap = q.block > 0 ? q.block : DEFAULT_CAP;
-  if (C->cap != 256 && C->cap != 512 && C->cap != 1024)
-    throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256, 512 or 1024");
+  if (C->cap != 256 && C->cap != 512)
+    throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256 or 512");
   if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
   Tiles& tl = C->tl;
   tl.order = dupload(own, P.t_order);
   tl.tptr = dupload(own, P.t_ptr); tl.hptr = dupload(own, P.t_hptr);
-  tl.hcell = dupload(own, P.t_hcell); tl.tvptr = dupload(own, P.t_vptr);
-  tl.tvert = dupload(own, P.t_vert);
-  tl.clv = dupload(own, P.t_clv); tl.cln = dupload(own, P.t_cln);
-  tl.hlv = dupload(own, P.t_hlv); tl.qmask = dupload(own, P.t_qmask);
+  tl.hcell = dupload(own, P.t_hcell);
+  tl.cln = dupload(own, P.t_cln);
+  tl.qmask = dupload(own, P.t_qmask);
   tl.nt = P.nt;
   set_lds_limit(C->cap);
   State& S = C->S;
@@ -1454,6 +1482,7 @@ void setup_device(storm_fvt* C) {
   S.vwet = dalloc<uint8_t>(own, P.P);
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
+  S.zeta = dalloc<double>(own, N);
   S.part = dalloc<double>(own, 4 * (size_t)P.nt);
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
@@ -1464,7 +1493,7 @@ void setup_device(storm_fvt* C) {
 template <int CAP>
 void launch_init(storm_fvt* C) {
   hipLaunchKernelGGL(k_init<CAP>, dim3(C->tl.nt), dim3(CAP), 0, C->stream, C->g, C->o, C->tl,
-                     C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.part);
+                     C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.zeta, C->S.part);
 }
 
 template <int CAP>
@@ -1483,8 +1512,7 @@ void start_run(storm_fvt* C) {
   C->r1_flip = 0;
   switch (C->cap) {
     case 256: launch_init<256>(C); break;
-    case 512: launch_init<512>(C); break;
-    default: launch_init<1024>(C); break;
+    default: launch_init<512>(C); break;
   }
   HIPCHK(hipGetLastError());
   C->S.din = C->d_a;
@@ -1530,13 +1558,14 @@ void launch_step(storm_fvt* C) {
     st.beta = b[s - 1];
     st.s = s;
     st.last = (s == R);
+    hipLaunchKernelGGL(k_vertex_stage, dim3((P.P + 255) / 256), dim3(256), 0, C->stream, C->g,
+                       C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl);
     if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
       hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
                          C->S.din, C->S.gwork, C->S.ctrl);
     switch (C->cap) {
       case 256: launch_stage<256>(C, st); break;
-      case 512: launch_stage<512>(C, st); break;
-      default: launch_stage<1024>(C, st); break;
+      default: launch_stage<512>(C, st); break;
     }
     C->last_din = C->S.din;
     CellD* next = C->S.dout;

@@ -642,42 +642,32 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
 }
 
 int build_tiles(Prep& S, int cap) {
-  const int N = S.N, P = S.P;
+  const int N = S.N;
   if (cap < 16 || cap > 0xFFFF) { S.err = "bad tile capacity"; return STORM_E_ARG; }
-  const int* cv = S.cvtx.data();
   const int* cn = S.cnbr.data();
-  // greedy runs of consecutive cells: grow while the run, its halo of edge
-  // neighbours and the vertices of both fit the capacity
-  std::vector<int> hstamp(N, -1), vstamp(P, -1);
+  // greedy runs of consecutive cells: grow while the run and its halo of
+  // edge neighbours fit the capacity
+  std::vector<int> hstamp(N, -1);
   S.t_ptr.assign(1, 0);
   int start = 0, tile = 0;
   while (start < N) {
-    int T = 0, H = 0, NV = 0;
+    int T = 0, H = 0;
     while (start + T < N) {
       const int c = start + T;
-      int newv[12], nnv = 0, newh[3], nnh = 0;
-      auto addv = [&](int v) {
-        if (vstamp[v] == tile) return;
-        for (int k = 0; k < nnv; ++k) if (newv[k] == v) return;
-        newv[nnv++] = v;
-      };
-      for (int j = 0; j < 3; ++j) addv(cv[(size_t)j * N + c]);
+      int newh[3], nnh = 0;
       const bool was_halo = hstamp[c] == tile;
       for (int j = 0; j < 3; ++j) {
         const int nb = cn[(size_t)j * N + c];
         if (nb < 0 || (nb >= start && nb <= c) || hstamp[nb] == tile) continue;
         bool dup = false;
         for (int k = 0; k < nnh; ++k) dup = dup || newh[k] == nb;
-        if (dup) continue;
-        newh[nnh++] = nb;
-        for (int k = 0; k < 3; ++k) addv(cv[(size_t)k * N + nb]);
+        if (!dup) newh[nnh++] = nb;
       }
-      const int T2 = T + 1, H2 = H + nnh - (was_halo ? 1 : 0), NV2 = NV + nnv;
-      if (T2 + H2 > cap || NV2 > cap) break;
-      for (int k = 0; k < nnv; ++k) vstamp[newv[k]] = tile;
+      const int T2 = T + 1, H2 = H + nnh - (was_halo ? 1 : 0);
+      if (T2 + H2 > cap) break;
       for (int k = 0; k < nnh; ++k) hstamp[newh[k]] = tile;
       if (was_halo) hstamp[c] = -1;
-      T = T2; H = H2; NV = NV2;
+      T = T2; H = H2;
     }
     if (T == 0) { S.err = "tile capacity too small for the mesh"; return STORM_E_ARG; }
     start += T;
@@ -687,16 +677,15 @@ int build_tiles(Prep& S, int cap) {
   const int nt = tile;
   S.nt = nt;
   S.cap = cap;
-  // per tile: halo and vertex lists (ascending ids), 16-bit local ids
-  S.t_hptr.assign(1, 0); S.t_hcell.clear(); S.t_hlv.clear();
-  S.t_vptr.assign(1, 0); S.t_vert.clear();
-  S.t_clv.assign(N, 0); S.t_cln.assign(N, 0);
+  // per tile: halo list (ascending ids) and 16-bit local neighbour ids
+  S.t_hptr.assign(1, 0); S.t_hcell.clear();
+  S.t_cln.assign(N, 0);
   S.t_qmask.assign(nt, 0);
-  std::vector<int> vloc(P, -1), hloc(N, -1);
-  std::vector<int> hl, vl;
+  std::vector<int> hloc(N, -1);
+  std::vector<int> hl;
   for (int b = 0; b < nt; ++b) {
     const int c0 = S.t_ptr[b], c1 = S.t_ptr[b + 1], T = c1 - c0;
-    hl.clear(); vl.clear();
+    hl.clear();
     for (int c = c0; c < c1; ++c)
       for (int j = 0; j < 3; ++j) {
         const int nb = cn[(size_t)j * N + c];
@@ -704,24 +693,9 @@ int build_tiles(Prep& S, int cap) {
       }
     std::sort(hl.begin(), hl.end());
     hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
-    for (int c = c0; c < c1; ++c)
-      for (int j = 0; j < 3; ++j) vl.push_back(cv[(size_t)j * N + c]);
-    for (int c : hl)
-      for (int j = 0; j < 3; ++j) vl.push_back(cv[(size_t)j * N + c]);
-    std::sort(vl.begin(), vl.end());
-    vl.erase(std::unique(vl.begin(), vl.end()), vl.end());
-    if (T + (int)hl.size() > cap || (int)vl.size() > cap) {
-      S.err = "tile construction exceeded its capacity"; return STORM_E_MESH;
-    }
-    for (int k = 0; k < (int)vl.size(); ++k) vloc[vl[k]] = k;
+    if (T + (int)hl.size() > cap) { S.err = "tile construction exceeded its capacity"; return STORM_E_MESH; }
     for (int k = 0; k < (int)hl.size(); ++k) hloc[hl[k]] = T + k;
-    auto lvw = [&](int c) {
-      uint64_t w = 0;
-      for (int j = 0; j < 3; ++j) w |= (uint64_t)vloc[cv[(size_t)j * N + c]] << (16 * j);
-      return w;
-    };
     for (int c = c0; c < c1; ++c) {
-      S.t_clv[c] = lvw(c);
       uint64_t w = 0;
       for (int j = 0; j < 3; ++j) {
         const int nb = cn[(size_t)j * N + c];
@@ -733,10 +707,8 @@ int build_tiles(Prep& S, int cap) {
       }
       S.t_cln[c] = w;
     }
-    for (int c : hl) { S.t_hcell.push_back(c); S.t_hlv.push_back(lvw(c)); }
-    for (int v : vl) S.t_vert.push_back(v);
+    for (int c : hl) S.t_hcell.push_back(c);
     S.t_hptr.push_back((int)S.t_hcell.size());
-    S.t_vptr.push_back((int)S.t_vert.size());
   }
   // block -> tile: each of the 8 XCDs (blocks are dealt to them round-robin)
   // takes a contiguous range of tiles, for L2 reuse of the shared halos; the

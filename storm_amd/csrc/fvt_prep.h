@@ -69,14 +69,13 @@ struct Prep {
   int last_cell = 0;                  // internal id of input cell N-1
   std::vector<double> h0, u0, v0;     // initial state (internal order)
   // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
-  // (edge neighbours outside) and the vertices of both are listed per tile;
-  // local ids are 16 bits, three to a word (cell slot j at bit 16*j):
-  // tile cells 0..T-1, halo cells T.., vertices in list order
+  // (edge neighbours outside) is listed per tile; local cell ids are 16
+  // bits, three to a word (cell slot j at bit 16*j): tile cells 0..T-1, then
+  // halo cells in list order
   int cap = 0, nt = 0;
   std::vector<int> t_order;           // block -> tile
-  std::vector<int> t_ptr, t_hptr, t_hcell, t_vptr, t_vert;
-  std::vector<uint64_t> t_clv, t_cln; // per cell: local vertex ids, local neighbour ids (0xFFFF: none)
-  std::vector<uint64_t> t_hlv;        // per halo entry: local vertex ids
+  std::vector<int> t_ptr, t_hptr, t_hcell;
+  std::vector<uint64_t> t_cln;        // per cell: local neighbour ids (0xFFFF: none)
   std::vector<uint64_t> t_qmask;      // per tile: inflowQbyH strings it touches
   std::string err;
 };
@@ -85,8 +84,8 @@ struct Prep {
 // with P.err set.
 int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& P);
 
-// Tiles of at most `cap` cells + halo cells and `cap` vertices (needs the
-// cell tables of build_prep).
+// Tiles of at most `cap` cells + halo cells (needs the cell tables of
+// build_prep).
 int build_tiles(Prep& P, int cap);
 
 // Least-squares molecule only (elemnt_db.f90:42-128), input numbering.

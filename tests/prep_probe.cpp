@@ -29,27 +29,17 @@ extern "C" int probe_tiles(const storm_mesh_t* m, const storm_fvt_opts_t* o, int
     const int t = P.t_order[b];
     if (t < 0 || t >= nt || seen[t]++) return fail("block order is not a permutation");
   }
-  int maxw = 0, maxv = 0;
+  int maxw = 0;
   long halo = 0;
   for (int b = 0; b < nt; ++b) {
     const int c0 = P.t_ptr[b], T = P.t_ptr[b + 1] - c0;
     const int hb = P.t_hptr[b], H = P.t_hptr[b + 1] - hb;
-    const int vb = P.t_vptr[b], NV = P.t_vptr[b + 1] - vb;
     if (T <= 0) return fail("empty tile");
-    if (T + H > cap || NV > cap) return fail("tile over capacity");
+    if (T + H > cap) return fail("tile over capacity");
     maxw = std::max(maxw, T + H);
-    maxv = std::max(maxv, NV);
     halo += H;
-    auto vcheck = [&](int c, uint64_t w) {
-      for (int j = 0; j < 3; ++j) {
-        const int k = (int)(w >> (16 * j)) & 0xFFFF;
-        if (k >= NV || P.t_vert[vb + k] != P.cvtx[(size_t)j * N + c]) return false;
-      }
-      return true;
-    };
     uint64_t qm = 0;
     for (int c = c0; c < c0 + T; ++c) {
-      if (!vcheck(c, P.t_clv[c])) return fail("bad local vertex id of a tile cell");
       for (int j = 0; j < 3; ++j) {
         const int nb = P.cnbr[(size_t)j * N + c];
         const int k = (int)(P.t_cln[c] >> (16 * j)) & 0xFFFF;
@@ -69,9 +59,8 @@ extern "C" int probe_tiles(const storm_mesh_t* m, const storm_fvt_opts_t* o, int
     for (int k = 0; k < H; ++k) {
       const int c = P.t_hcell[hb + k];
       if (c >= c0 && c < c0 + T) return fail("halo cell inside the tile");
-      if (!vcheck(c, P.t_hlv[hb + k])) return fail("bad local vertex id of a halo cell");
     }
   }
-  stats[0] = nt; stats[1] = maxw; stats[2] = maxv; stats[3] = (int)halo;
+  stats[0] = nt; stats[1] = maxw; stats[2] = 0; stats[3] = (int)halo;
   return 0;
 }
