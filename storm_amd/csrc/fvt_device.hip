@@ -100,10 +100,10 @@ struct State {
 // is hcell[hptr[b]..hptr[b+1]).  Local cell ids are 16 bits, three to a word
 // (cell slot j at bit 16*j): tile cells 0..T-1, then halo cells.
 struct Tiles {
-  const int* order;             // block -> tile
-  const int *tptr, *hptr, *hcell;
-  const uint64_t* cln;          // per cell: local neighbour ids (0xFFFF: none)
-  const uint64_t* qmask;        // per tile: inflowQbyH strings it touches
+  const int4* hdr;              // per block, two int4: {c0, T, hb, H}, {qmask lo, hi, tile, 0}
+  const int4* hrec;             // per halo entry, two int4: {cell, nbr0..2}, {vtx0..2, 0}
+  const uint64_t* hmeta;        // per halo entry: the cell's metadata word
+  const uint64_t* cln;          // per tile cell: local neighbour ids (0xFFFF: none)
   int nt;
 };
 
@@ -171,8 +171,8 @@ __device__ __forceinline__ Side side_of(double xa, double ya, double za, double 
   s.len = sqrt(dx * dx + dy * dy);
   s.nx = (yb - ya) / s.len;
   s.ny = (xa - xb) / s.len;
-  s.tx = (xb - xa) / s.len;
-  s.ty = (yb - ya) / s.len;
+  s.tx = -s.ny;  // (xb - xa) / len, the same value
+  s.ty = s.nx;   // (yb - ya) / len
   s.mx = HALF * (xa + xb);  // dataprepFVT.f90:168-169
   s.my = HALF * (ya + yb);
   s.zm = HALF * (za + zb);  // invisc_fluxes.f90:89
@@ -209,8 +209,9 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
   // the CFL candidates of step 1 go to the per-tile partials (slot 3),
   // reduced by k_control
   __shared__ double s_c[CAP];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const int c0 = tl.tptr[b], T = tl.tptr[b + 1] - c0;
+  const int t = threadIdx.x;
+  const int4 hd0 = tl.hdr[2 * blockIdx.x], hd1 = tl.hdr[2 * blockIdx.x + 1];
+  const int c0 = hd0.x, T = hd0.y, b = hd1.z;
   double cand = DBL_MAX;
   if (t < T) {
     const int i = c0 + t;
@@ -516,11 +517,10 @@ struct Faces {
 // hgradfix2.f90:46-98 and the cell pass of to_vertices.f90:44-55, for cell i.
 // vw: the cell's three vertices are wet.
 __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const CellD* din,
-                                           const double* gwork, int i, const Tri& tr, bool vw,
-                                           Faces& F) {
+                                           const double* gwork, int i, uint64_t meta, int nb0,
+                                           int nb1, int nb2, const Tri& tr, bool vw, Faces& F) {
   const int N = g.N;
-  const uint64_t meta = g.cmeta[i];
-  const int nb[3] = {g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i]};
+  const int nb[3] = {nb0, nb1, nb2};
   const CellD di = ld(&din[i]);
   const CellS si = ld(&g.cs[i]);
   CellD dn[3];
@@ -938,7 +938,8 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
     const int N = g.N;
     const bool vw = vwet[g.cvtx[i]] && vwet[g.cvtx[N + i]] && vwet[g.cvtx[2 * N + i]];
     Faces F;
-    cell_faces(g, o, din, gwork, i, tr, vw, F);
+    cell_faces(g, o, din, gwork, i, g.cmeta[i], g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i], tr,
+               vw, F);
     const double fh = (m == 0) ? F.fh[0] : (m == 1) ? F.fh[1] : F.fh[2];
     L.qh[l] = fh;
     L.qu[l] = (m == 0) ? F.fu[0] : (m == 1) ? F.fu[1] : F.fu[2];
@@ -1021,11 +1022,11 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   __shared__ double s_dlast;
   const Ctrl* c = S.ctrl;
   if (c->done) return;
-  const int b = tl.order[blockIdx.x];
+  const int4 hd0 = tl.hdr[2 * blockIdx.x], hd1 = tl.hdr[2 * blockIdx.x + 1];
+  const int c0 = hd0.x, T = hd0.y, hb = hd0.z, H = hd0.w, b = hd1.z;
+  const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
   const int t = threadIdx.x;
   const int N = g.N;
-  const int c0 = tl.tptr[b], T = tl.tptr[b + 1] - c0;
-  const int hb = tl.hptr[b], H = tl.hptr[b + 1] - hb;
   const Lds L = lds_layout(lds_dyn, CAP);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (t == 0 && o.steady && st.last) {
@@ -1034,7 +1035,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   }
 
   // 2. inflow discharge strings (their buffers overlay the cell arrays)
-  for (uint64_t qm = tl.qmask[b]; qm; qm &= qm - 1)
+  for (uint64_t qm = qmask; qm; qm &= qm - 1)
     bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, c, __builtin_ctzll(qm), L, s_at, s_null,
                     s_skip);
 
@@ -1042,10 +1043,28 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   Faces F;
   if (t < T + H) {
     const bool own = t < T;
-    const int i = own ? c0 + t : tl.hcell[hb + t - T];
-    const Tri tr = load_tri(g, i);
-    const bool vw = S.vwet[g.cvtx[i]] && S.vwet[g.cvtx[N + i]] && S.vwet[g.cvtx[2 * N + i]];
-    cell_faces(g, o, S.din, S.gwork, i, tr, vw, F);
+    int i, nb0, nb1, nb2, v0, v1, v2;
+    uint64_t meta;
+    if (own) {
+      i = c0 + t;
+      meta = g.cmeta[i];
+      nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
+      v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
+    } else {
+      const int4 r0 = tl.hrec[2 * (hb + t - T)], r1 = tl.hrec[2 * (hb + t - T) + 1];
+      i = r0.x; nb0 = r0.y; nb1 = r0.z; nb2 = r0.w;
+      v0 = r1.x; v1 = r1.y; v2 = r1.z;
+      meta = tl.hmeta[hb + t - T];
+    }
+    Tri tr;
+    {
+      const double4 q0 = g.vxyz[v0], q1 = g.vxyz[v1], q2 = g.vxyz[v2];
+      tr.x[0] = q0.x; tr.y[0] = q0.y; tr.zv[0] = q0.z;
+      tr.x[1] = q1.x; tr.y[1] = q1.y; tr.zv[1] = q1.z;
+      tr.x[2] = q2.x; tr.y[2] = q2.y; tr.zv[2] = q2.z;
+    }
+    const bool vw = S.vwet[v0] && S.vwet[v1] && S.vwet[v2];
+    cell_faces(g, o, S.din, S.gwork, i, meta, nb0, nb1, nb2, tr, vw, F);
     if (own) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -1468,11 +1487,10 @@ void setup_device(storm_fvt* C) {
     throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256 or 512");
   if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
   Tiles& tl = C->tl;
-  tl.order = dupload(own, P.t_order);
-  tl.tptr = dupload(own, P.t_ptr); tl.hptr = dupload(own, P.t_hptr);
-  tl.hcell = dupload(own, P.t_hcell);
+  tl.hdr = reinterpret_cast<const int4*>(dupload(own, P.t_hdr));
+  tl.hrec = reinterpret_cast<const int4*>(dupload(own, P.t_hrec));
+  tl.hmeta = dupload(own, P.t_hmeta);
   tl.cln = dupload(own, P.t_cln);
-  tl.qmask = dupload(own, P.t_qmask);
   tl.nt = P.nt;
   set_lds_limit(C->cap);
   State& S = C->S;

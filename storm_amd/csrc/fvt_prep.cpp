@@ -722,6 +722,30 @@ int build_tiles(Prep& S, int cap) {
   int front = 0;
   for (int blk = 0; blk < nt; ++blk)
     if (S.t_qmask[S.t_order[blk]]) std::swap(S.t_order[blk], S.t_order[front++]);
+  // what the kernel reads first: per block the tile header, per halo entry
+  // its cell's neighbours, vertices and metadata
+  S.t_hdr.assign(8 * (size_t)nt, 0);
+  for (int blk = 0; blk < nt; ++blk) {
+    const int b = S.t_order[blk];
+    int* h = &S.t_hdr[8 * (size_t)blk];
+    h[0] = S.t_ptr[b]; h[1] = S.t_ptr[b + 1] - S.t_ptr[b];
+    h[2] = S.t_hptr[b]; h[3] = S.t_hptr[b + 1] - S.t_hptr[b];
+    h[4] = (int)(uint32_t)(S.t_qmask[b] & 0xFFFFFFFFu); h[5] = (int)(uint32_t)(S.t_qmask[b] >> 32);
+    h[6] = b;
+  }
+  const size_t nh = S.t_hcell.size();
+  S.t_hrec.assign(8 * nh, 0);
+  S.t_hmeta.assign(nh, 0);
+  for (size_t k = 0; k < nh; ++k) {
+    const int c = S.t_hcell[k];
+    int* r = &S.t_hrec[8 * k];
+    r[0] = c;
+    for (int j = 0; j < 3; ++j) {
+      r[1 + j] = cn[(size_t)j * N + c];
+      r[4 + j] = S.cvtx[(size_t)j * N + c];
+    }
+    S.t_hmeta[k] = S.cmeta[c];
+  }
   return STORM_OK;
 }
 

@@ -77,6 +77,11 @@ struct Prep {
   std::vector<int> t_ptr, t_hptr, t_hcell;
   std::vector<uint64_t> t_cln;        // per cell: local neighbour ids (0xFFFF: none)
   std::vector<uint64_t> t_qmask;      // per tile: inflowQbyH strings it touches
+  // device copies in block order: 8 ints per block {c0, T, hb, H, qmask lo,
+  // qmask hi, tile, 0}; 8 ints per halo entry {cell, nbr0..2, vtx0..2, 0}
+  // and its metadata word
+  std::vector<int> t_hdr, t_hrec;
+  std::vector<uint64_t> t_hmeta;
   std::string err;
 };
 

@@ -61,6 +61,22 @@ extern "C" int probe_tiles(const storm_mesh_t* m, const storm_fvt_opts_t* o, int
       if (c >= c0 && c < c0 + T) return fail("halo cell inside the tile");
     }
   }
+  for (int blk = 0; blk < nt; ++blk) {
+    const int b = P.t_order[blk];
+    const int* h = &P.t_hdr[8 * (size_t)blk];
+    const uint64_t qm = (uint64_t)(uint32_t)h[4] | ((uint64_t)(uint32_t)h[5] << 32);
+    if (h[0] != P.t_ptr[b] || h[1] != P.t_ptr[b + 1] - P.t_ptr[b] || h[2] != P.t_hptr[b] ||
+        h[3] != P.t_hptr[b + 1] - P.t_hptr[b] || qm != P.t_qmask[b] || h[6] != b)
+      return fail("bad block header");
+  }
+  for (size_t k = 0; k < P.t_hcell.size(); ++k) {
+    const int c = P.t_hcell[k];
+    const int* r = &P.t_hrec[8 * k];
+    bool ok = r[0] == c && P.t_hmeta[k] == P.cmeta[c];
+    for (int j = 0; j < 3; ++j)
+      ok = ok && r[1 + j] == P.cnbr[(size_t)j * N + c] && r[4 + j] == P.cvtx[(size_t)j * N + c];
+    if (!ok) return fail("bad halo record");
+  }
   stats[0] = nt; stats[1] = maxw; stats[2] = 0; stats[3] = (int)halo;
   return 0;
 }
