@@ -170,7 +170,7 @@ def flume(nx: int = 200, ny: int = 50, length: float = 1000.0,
           width: float = 100.0, seed: int = 1001, slope: float = 0.001,
           rough_sd: float = 0.02, manning: float = 0.03, q_in: float = 200.0,
           h_out: float = 2.0, h0: Optional[float] = 2.0, name: str = "flume",
-          **opt_kw) -> Case:
+          in_kind: int = 1, out_kind: int = 1, **opt_kw) -> Case:
     """Straight flume [0,L] x [0,W].
 
     Random draws, in order: node x-jitter, node y-jitter (each U(-0.2,0.2)
@@ -178,7 +178,9 @@ def flume(nx: int = 200, ny: int = 50, length: float = 1000.0,
     N(0, rough_sd^2) per node.  Bed: slope*(L-x) + roughness.  Inflow string
     (vtype 1, Q constant) on x=0, outflow string (htype 1, stage constant) on
     x=L, walls on y=0 and y=W.  Initial state: h = h0 (or stage h_out when
-    h0 is None), u = v = 0.
+    h0 is None), u = v = 0.  in_kind 4 (stage inflow, q_in is the stage) or 5
+    (discharge into wet cells only) and out_kind 0 (free outflow) or 2
+    (critical depth) select the other boundary routines.
     """
     rng = np.random.default_rng(seed)
     dx, dy = length / nx, width / ny
@@ -201,13 +203,16 @@ def flume(nx: int = 200, ny: int = 50, length: float = 1000.0,
     v = np.zeros(n)
     inflow_nodes = np.arange(0, ny + 1, dtype=np.int32)            # i = 0
     outflow_nodes = (nx * (ny + 1) + np.arange(0, ny + 1)).astype(np.int32)
-    inflows = [Boundary(1, inflow_nodes, np.array([0.0]), np.array([q_in]))]
-    outflows = [Boundary(1, outflow_nodes, np.array([0.0]), np.array([h_out]))]
+    inflows = [Boundary(in_kind, inflow_nodes, np.array([0.0]), np.array([q_in]))]
+    if out_kind == 1:
+        outflows = [Boundary(1, outflow_nodes, np.array([0.0]), np.array([h_out]))]
+    else:
+        outflows = [Boundary(out_kind, outflow_nodes, np.zeros(0), np.zeros(0))]
     opts = default_opts(rkorder=2, cfl=0.9, dt_op=1, steady=False, istop=2000)
     opts.update(opt_kw)
     meta = dict(generator="flume", nx=nx, ny=ny, length=length, width=width,
                 seed=seed, slope=slope, rough_sd=rough_sd, manning=manning,
-                q_in=q_in, h_out=h_out)
+                q_in=q_in, h_out=h_out, in_kind=in_kind, out_kind=out_kind)
     return Case(name, x, y, tri, edge_p, edge_e, zvtx, cd, h, u, v, inflows,
                 outflows, opts, meta)
 

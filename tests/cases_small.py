@@ -24,4 +24,32 @@ def small_cases():
         "flume_zero_grad_hyst": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=14,
                                              dt_op=0, cfl=0.05, istop=25, q_in=15.0, limiter=0,
                                              c_threshold=3.0, h0=None, h_out=0.3), 25),
+        "flume_inH_free": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=21,
+                                       dt_op=0, cfl=0.05, istop=30, in_kind=4, out_kind=0,
+                                       q_in=2.3), 30),
+        "flume_q5_crit": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=21,
+                                      dt_op=0, cfl=0.05, istop=30, in_kind=5, out_kind=2,
+                                      q_in=12.0), 30),
+    }
+
+
+def option_cases():
+    """Cases for options whose reference routine is defective or not on the
+    reference's active path (so only the oracle, pinned by closed forms,
+    checks them): the corrected HLL and BGNVC fluxes, Rusanov, Green-Gauss
+    gradients, explicit Manning friction and the corrected CFL time step."""
+    return {
+        "flume_hll_cfl": (cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=31,
+                                      dt_op=1, cfl=0.9, istop=40, q_in=15.0, flux="hll"), 40),
+        "dambreak_rusanov_cfl": (cases.dambreak(nq=14, side=200.0, seed=32, x_dam=80.0,
+                                                h_res=2.0, flux="rusanov", dt_op=1, cfl=0.5,
+                                                istop=60, h_dry=1e-4), 60),
+        "flume_bgnvc": (cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=33, dt_op=0,
+                                    cfl=0.05, istop=30, q_in=15.0, flux="bgnvc"), 30),
+        "flume_gauss_rk3": (cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=34,
+                                        dt_op=1, cfl=0.5, istop=30, q_in=15.0, rkorder=3,
+                                        gradient="gauss"), 30),
+        "flume_explicit_friction": (cases.flume(nx=20, ny=6, length=100.0, width=30.0,
+                                                seed=35, dt_op=1, cfl=0.5, istop=30, q_in=15.0,
+                                                friction=1), 30),
     }

@@ -7,7 +7,7 @@ state (h, u, v, rk*(:,rkorder), phi, drycell, hvtx, zetavtx, iter, ttime) as
 .npz fixtures.  The fixtures are data: inputs are regenerated from the case
 generator and seed recorded with them.
 
-Usage (from the repository root):  python tests/golden/make_golden.py
+Usage (from the repository root):  python tests/golden/make_golden.py [name ...]
 """
 import os
 import sys
@@ -28,7 +28,10 @@ KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", 
 
 def main():
     out_dir = os.path.dirname(os.path.abspath(__file__))
+    only = set(sys.argv[1:])
     for name, (case, n) in small_cases().items():
+        if only and name not in only:
+            continue
         assert int(case.opts["istop"]) == n
         with tempfile.TemporaryDirectory() as wd:
             ref = O.run_reference(case, wd)
