@@ -121,9 +121,9 @@ int storm_fvt_get_state(storm_fvt_t *ctx, double *h, double *u, double *v, doubl
 int storm_fvt_get_conserved(storm_fvt_t *ctx, double *qh, double *qu, double *qv);
 /* Vertex depth and water-surface elevation of the last to_vertices. */
 int storm_fvt_get_vertex(storm_fvt_t *ctx, double *hvtx, double *zetavtx);
-/* Device pointers of the cell state in the library's internal (reordered)
- * order and the permutation perm[internal] = input id, for callers that keep
- * data resident on the GPU. */
+/* The cell count, the library's internal (Hilbert) cell order as the
+ * permutation perm[internal] = input id, and the context's hipStream_t, for
+ * callers that order their own GPU work after the time loop. */
 int storm_fvt_device_info(storm_fvt_t *ctx, int32_t *n_elems, int32_t *perm, void **stream);
 /* Seconds of device time of the last storm_fvt_run (HIP events on the
  * context's stream). */
