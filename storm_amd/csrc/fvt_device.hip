@@ -65,6 +65,9 @@ struct Geo {
   const int* gconn;
   const double *gcx, *gcy, *gwt;
   const int *vptr, *vcell, *vbc;
+  const int* vell;          // [vW][P] incident cells per vertex, n2t order
+  const double* vwt;        // [vW][P] inverse-distance weights
+  const double2* vzb;       // per vertex {zvtx, bits: string index | degree << 32}
   const int *qs_ptr, *qs_cell, *qs_slot;
   const double *qs_rcx, *qs_rcy, *qs_nx, *qs_ny, *qs_len;
   const int* in_type;
@@ -408,41 +411,40 @@ __global__ void __launch_bounds__(256)
 k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* c) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
-  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
-  const double4 q = g.vxyz[p];
-  const int vb = g.vbc[p];
+  const double2 zb = g.vzb[p];
   if (c->done) return;
-  double id = ZERO, a = ZERO, zr = q.z;
+  const double zv = zb.x;
+  const uint64_t bits = (uint64_t)__double_as_longlong(zb.y);
+  const int vb = (int)(uint32_t)bits, deg = (int)(bits >> 32);
+  const size_t P = (size_t)g.P;
+  double id = ZERO, a = ZERO, zr = zv;
   int counter = 0;
-  for (int t = t0; t < t1; t += 4) {
+  for (int t = 0; t < deg; t += 4) {
+    // four cells per round, all loads issued before the sums
     int k[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) k[u] = (t + u < t1) ? g.vcell[t + u] : -1;
-    double zk[4];
-    double2 ck[4];
+    double w[4], zk[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int kk = k[u] < 0 ? k[0] : k[u];
-      zk[u] = zeta[kk];
-      ck[u] = *reinterpret_cast<const double2*>(&g.cs[kk]);
+      k[u] = (t + u < deg) ? g.vell[(t + u) * P + p] : -1;
+      w[u] = (t + u < deg) ? g.vwt[(t + u) * P + p] : ZERO;
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) zk[u] = zeta[k[u] < 0 ? k[0] : k[u]];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (k[u] < 0 || zk[u] != zk[u]) continue;
       counter++;
-      const double xk = ck[u].x - q.x, yk = ck[u].y - q.y;
-      const double w = ONE / sqrt(xk * xk + yk * yk);
-      id = id + w;
-      a = a + w * zk[u];
+      id = id + w[u];
+      a = a + w[u] * zk[u];
       zr = dmax(zk[u], zr);
     }
   }
-  double zeta_v = (counter > 0) ? a / id : q.z;
+  double zeta_v = (counter > 0) ? a / id : zv;
   zeta_v = dmin(zeta_v, zr);
-  double hv = zeta_v - q.z;
+  double hv = zeta_v - zv;
   if (hv < ZERO) hv = ZERO;
   vwet[p] = hv > o.h_dry;
-  if (vb >= 0) hv = dmax(ZERO, c->hbc[vb] - q.z);
+  if (vb >= 0) hv = dmax(ZERO, c->hbc[vb] - zv);
   hvtx[p] = hv;
 }
 
@@ -1462,6 +1464,8 @@ void setup_device(storm_fvt* C) {
   g.gconn = dupload(own, P.gconn);
   g.gcx = dupload(own, P.gcx); g.gcy = dupload(own, P.gcy); g.gwt = dupload(own, P.gwt);
   g.vptr = dupload(own, P.vptr); g.vcell = dupload(own, P.vcell); g.vbc = dupload(own, P.vbc);
+  g.vell = dupload(own, P.vell); g.vwt = dupload(own, P.vwt);
+  g.vzb = reinterpret_cast<const double2*>(dupload(own, P.vzb));
   g.qs_ptr = dupload(own, P.qs_ptr); g.qs_cell = dupload(own, P.qs_cell);
   g.qs_slot = dupload(own, P.qs_slot);
   g.qs_rcx = dupload(own, P.qs_rcx); g.qs_rcy = dupload(own, P.qs_rcy);

@@ -606,12 +606,29 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     for (int t = 0; t < n2t_ptr[p + 1] - n2t_ptr[p]; ++t)
       S.vcell[S.vptr[pp] + t] = ip[n2t[n2t_ptr[p] + t]];
   }
-  (void)c2vwid;
   // enforce_bcs.f90:40-56: nodes of stage strings (last string wins, as the
   // reference loops over strings in order)
   for (int ch = 0; ch < m->n_out; ++ch) {
     if (m->out_type[ch] != 1) continue;
     for (int t = m->out_ptr[ch]; t < m->out_ptr[ch + 1]; ++t) S.vbc[vip[m->out_nodes[t]]] = ch;
+  }
+  // the vertex pass's tables: row r of vertex p at r*P + p (n2t order), the
+  // inverse-distance weights of c2v_setup.f90:106-113, and per vertex
+  // {zvtx, string index | degree << 32}
+  S.vW = 0;
+  for (int p = 0; p < P; ++p) S.vW = std::max(S.vW, n2t_ptr[p + 1] - n2t_ptr[p]);
+  S.vell.assign((size_t)S.vW * P, -1);
+  S.vwt.assign((size_t)S.vW * P, 0.0);
+  S.vzb.assign(2 * (size_t)P, 0.0);
+  for (int pp = 0; pp < P; ++pp) {
+    const int p = S.vperm[pp], deg = n2t_ptr[p + 1] - n2t_ptr[p];
+    for (int t = 0; t < deg; ++t) {
+      S.vell[(size_t)t * P + pp] = ip[n2t[n2t_ptr[p] + t]];
+      S.vwt[(size_t)t * P + pp] = c2vwid[n2t_ptr[p] + t];
+    }
+    const uint64_t bits = (uint64_t)(uint32_t)S.vbc[pp] | ((uint64_t)(uint32_t)deg << 32);
+    S.vzb[2 * (size_t)pp] = zvtx[p];
+    std::memcpy(&S.vzb[2 * (size_t)pp + 1], &bits, sizeof(bits));
   }
   // inflowQbyH strings: (cell, local edge) in string order, with the edge's
   // rc vector, unit normal (as the reference orients it) and length

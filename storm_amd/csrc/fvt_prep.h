@@ -56,6 +56,10 @@ struct Prep {
   std::vector<double> vxyz;           // [4][P] interleaved: x, y, zvtx, 0
   std::vector<int> vptr, vcell;       // vertex -> cells (input cell order kept)
   std::vector<int> vbc;               // stage string index of htype-1 string nodes, else -1
+  int vW = 0;                         // largest vertex degree
+  std::vector<int> vell;              // [vW][P] incident cells in n2t order, -1 past the degree
+  std::vector<double> vwt;            // [vW][P] their inverse-distance weights
+  std::vector<double> vzb;            // [2][P] interleaved: zvtx, bits (vbc | degree << 32)
   // boundary strings
   int n_in = 0, n_out = 0;
   std::vector<int> in_type, out_type;
