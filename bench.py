@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--block", type=int, default=0,
-                    help="stage-kernel tile capacity (256 or 512; 0 = library default)")
+                    help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")

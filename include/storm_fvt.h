@@ -91,7 +91,7 @@ typedef struct storm_fvt_opts {
   double cfl, rstop, h_dry, h_wet, vceiling;
   /* MI355X execution options (no reference counterpart) */
   int32_t reorder;   /* 1 = Hilbert-curve cell order (default), 0 = input order */
-  int32_t block;     /* stage-kernel tile capacity (threads per block): 256 or 512; 0 = default */
+  int32_t block;     /* stage-kernel tile capacity (threads per block): 256, 384, 512; 0 = default */
   int32_t graph;     /* 1 = replay captured HIP graphs of whole steps */
   int32_t reserved;
 } storm_fvt_opts_t;

@@ -42,6 +42,13 @@ namespace {
 constexpr int CTRL_THREADS = 1024;
 constexpr int DEFAULT_CAP = 256;  // cells + halo cells per tile (threads per stage block)
 
+// largest power of two not above n (the fixed tree shape of block reductions)
+__host__ __device__ constexpr int pow2_floor(int n) {
+  int p = 1;
+  while (2 * p <= n) p *= 2;
+  return p;
+}
+
 struct Ctrl {
   double dt;          // dt of the current step
   double delt_last;   // steady: local dt of the reference's last cell
@@ -235,7 +242,12 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
   }
   s_c[t] = cand;
   __syncthreads();
-  for (int w = CAP / 2; w > 0; w >>= 1) {
+  constexpr int P2 = pow2_floor(CAP);
+  if (P2 < CAP) {
+    if (t < CAP - P2) s_c[t] = dmin(s_c[t], s_c[t + P2]);
+    __syncthreads();
+  }
+  for (int w = P2 / 2; w > 0; w >>= 1) {
     if (t < w) s_c[t] = dmin(s_c[t], s_c[t + w]);
     __syncthreads();
   }
@@ -454,7 +466,6 @@ k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_
          double* zetavtx, const Ctrl* c) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
-  if (c->done) return;
   const Vtx v = vertex_idw(g, o, d, dry, p, g.vxyz[p], c);
   vwet[p] = v.wet;
   hvtx[p] = v.h;
@@ -1273,7 +1284,16 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   double* r2 = L.fh + 2 * CAP;
   r0[t] = top; r1[t] = bot; r2[t] = mxphi;
   __syncthreads();
-  for (int w = CAP / 2; w > 0; w >>= 1) {
+  constexpr int P2 = pow2_floor(CAP);
+  if (P2 < CAP) {
+    if (t < CAP - P2) {
+      r0[t] = r0[t] + r0[t + P2];
+      r1[t] = r1[t] + r1[t + P2];
+      r2[t] = dmax(r2[t], r2[t + P2]);
+    }
+    __syncthreads();
+  }
+  for (int w = P2 / 2; w > 0; w >>= 1) {
     if (t < w) {
       r0[t] = r0[t] + r0[t + w];
       r1[t] = r1[t] + r1[t + w];
@@ -1289,7 +1309,11 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   __syncthreads();
   r0[t] = cand;
   __syncthreads();
-  for (int w = CAP / 2; w > 0; w >>= 1) {
+  if (P2 < CAP) {
+    if (t < CAP - P2) r0[t] = dmin(r0[t], r0[t + P2]);
+    __syncthreads();
+  }
+  for (int w = P2 / 2; w > 0; w >>= 1) {
     if (t < w) r0[t] = dmin(r0[t], r0[t + w]);
     __syncthreads();
   }
@@ -1436,6 +1460,7 @@ void upload_tables(storm_fvt* C) {
 
 void set_lds_limit(int cap) {
   const void* f = (cap == 256) ? reinterpret_cast<const void*>(&k_stage<256>)
+                : (cap == 384) ? reinterpret_cast<const void*>(&k_stage<384>)
                                : reinterpret_cast<const void*>(&k_stage<512>);
   HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(cap)));
 }
@@ -1487,8 +1512,8 @@ void setup_device(storm_fvt* C) {
     if (R == 1 || s >= 1) C->slots[s] = dalloc<double>(own, 3 * N);
   // tiles of the stage kernel
   C->cap = q.block > 0 ? q.block : DEFAULT_CAP;
-  if (C->cap != 256 && C->cap != 512)
-    throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256 or 512");
+  if (C->cap != 256 && C->cap != 384 && C->cap != 512)
+    throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256, 384 or 512");
   if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
   Tiles& tl = C->tl;
   tl.hdr = reinterpret_cast<const int4*>(dupload(own, P.t_hdr));
@@ -1534,6 +1559,7 @@ void start_run(storm_fvt* C) {
   C->r1_flip = 0;
   switch (C->cap) {
     case 256: launch_init<256>(C); break;
+    case 384: launch_init<384>(C); break;
     default: launch_init<512>(C); break;
   }
   HIPCHK(hipGetLastError());
@@ -1587,6 +1613,7 @@ void launch_step(storm_fvt* C) {
                          C->S.din, C->S.gwork, C->S.ctrl);
     switch (C->cap) {
       case 256: launch_stage<256>(C, st); break;
+      case 384: launch_stage<384>(C, st); break;
       default: launch_stage<512>(C, st); break;
     }
     C->last_din = C->S.din;

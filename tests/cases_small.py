@@ -30,6 +30,10 @@ def small_cases():
         "flume_q5_crit": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=21,
                                       dt_op=0, cfl=0.05, istop=30, in_kind=5, out_kind=2,
                                       q_in=12.0), 30),
+        # steady local time stepping that converges (RMS dh < 1e-4) at step 390
+        "flume_steady_converged": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41,
+                                               dt_op=0, cfl=0.3, istop=600, q_in=15.0,
+                                               steady=True, stop_cr=1, rstop=1e-4), 600),
     }
 
 

@@ -36,3 +36,21 @@ def test_step_parity(name):
     for k in ("hvtx", "zetavtx", "phi1", "phi2", "phi3"):
         assert _cmp(so[k], sg[k]) <= 1e-9, (k, _cmp(so[k], sg[k]))
     assert np.array_equal(so["dry"], sg["dry"])
+
+
+@pytest.mark.parametrize("cap", [384, 512])
+def test_tile_capacity_leaves_results_unchanged(cap):
+    """The tiling (and so the block size) changes only the order of the
+    residual sums: the state is bit-identical."""
+    from storm_amd import fvt
+    case, n = small_cases()["meander_crit"]
+    print(f"[gpu tiles] meander_crit cap={cap}", flush=True)
+    out = []
+    for block in (256, cap):
+        s = fvt.FVTSolver(case, dict(block=block))
+        s.run(n)
+        s.finalize()
+        out.append(s.state())
+        s.close()
+    for k in ("h", "qh", "qu", "qv", "u", "v", "hvtx", "zetavtx", "phi1", "phi2", "phi3", "dry"):
+        assert np.array_equal(out[0][k], out[1][k]), k

@@ -38,7 +38,7 @@ def run_probe(lib, case, cap):
     return list(stats)
 
 
-@pytest.mark.parametrize("cap", [256, 512])
+@pytest.mark.parametrize("cap", [256, 384, 512])
 def test_small_case_tiles(probe, cap):
     for name, (case, _) in small_cases().items():
         nt, maxw, maxv, halo = run_probe(probe, case, cap)
