@@ -102,6 +102,20 @@ typedef struct storm_fvt storm_fvt_t;
 void storm_fvt_default_opts(storm_fvt_opts_t *o);
 
 int storm_fvt_create(const storm_mesh_t *mesh, const storm_fvt_opts_t *opts, storm_fvt_t **ctx);
+/* An ensemble of n independent reaches advanced together: every launch of a
+ * step covers all of them, and each reach keeps its own time step, time,
+ * residual and convergence (a converged steady reach stops while the others
+ * go on).  No reference counterpart: the reference runs one reach per
+ * process.  The arrays of every other entry point are the reaches' arrays
+ * laid end to end in reach order (cells, vertices, strings); iter/ttime/rmax
+ * of storm_fvt_run are reach 0's.  Results equal those of n separate
+ * contexts. */
+int storm_fvt_create_batch(int32_t n, const storm_mesh_t *meshes, const storm_fvt_opts_t *opts,
+                           storm_fvt_t **ctx);
+/* Per reach (arrays of n; any may be NULL): steps taken, time, residual of
+ * the last step, converged flag. */
+int storm_fvt_reach_status(storm_fvt_t *ctx, int32_t *iter, double *ttime, double *rmax,
+                           uint8_t *done);
 /* Cell-centred h, u, v; restarts the time loop (iter = 0). */
 int storm_fvt_set_state(storm_fvt_t *ctx, const double *h, const double *u, const double *v);
 /* Replace every string's table by one constant: qin[n_in] (Q, or stage for

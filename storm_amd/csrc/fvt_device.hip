@@ -24,7 +24,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -49,12 +51,19 @@ __host__ __device__ constexpr int pow2_floor(int n) {
   return p;
 }
 
+// Step control of one reach, kept on the device (k_control writes it).
 struct Ctrl {
   double dt;          // dt of the current step
   double delt_last;   // steady: local dt of the reference's last cell
   double ttime, stime, rmax, sumtop, sumbot;
   int iter, done, pending, pad;
-  double qin[MAX_STRINGS], hbc[MAX_STRINGS];
+};
+
+// Per reach: its strings, its tiles, the cell whose local dt drives steady
+// friction, and bch_max (dataprepFVT.f90:278-286).
+struct RInfo {
+  int in_base, n_in, out_base, n_out, tile0, tile1, last_cell, pad;
+  double bch_max, pad2;
 };
 
 // Cell records, read whole with 16-byte loads: static data and the stage
@@ -78,8 +87,8 @@ struct Geo {
   const int *qs_ptr, *qs_cell, *qs_slot;
   const double *qs_rcx, *qs_rcy, *qs_nx, *qs_ny, *qs_len;
   const int* in_type;
-  int n_in, n_out, last_cell;
-  double bch_max;
+  const RInfo* reach;
+  int n_in, n_out, nr;  // string totals, reaches
 };
 
 struct Opts {
@@ -102,7 +111,8 @@ struct State {
   double* zeta;       // water surface of the next stage's input, NaN at dry cells
   double *gwork, *hvtx, *zetavtx, *phi, *part;
   uint8_t *dry_post, *vwet;
-  Ctrl* ctrl;
+  double *sq, *sh;    // current Q (or stage) of every inflow string, stage of every outflow string
+  Ctrl* ctrl;         // one per reach
 };
 
 // Tiles of the stage kernel: tile b owns cells [tptr[b], tptr[b+1]) (a run of
@@ -260,38 +270,39 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
 }
 
 // ---------------------------------------------------------------- control
-__device__ void get_time_bcs(const Geo& g, const int* out_type, const int* qts_ptr,
+__device__ void get_time_bcs(const RInfo& ri, const int* out_type, const int* qts_ptr,
                              const double* qts_t, const double* qts_v, const int* hts_ptr,
-                             const double* hts_t, const double* hts_v, double time, Ctrl* c) {
-  // get_time_bcs.f90:31-77
-  for (int k = 0; k < g.n_out; ++k) {
+                             const double* hts_t, const double* hts_v, double time, double* sq,
+                             double* sh) {
+  // get_time_bcs.f90:31-77 for the strings of one reach
+  for (int k = ri.out_base; k < ri.out_base + ri.n_out; ++k) {
     if (out_type[k] != 1) continue;
     int a = hts_ptr[k], n = hts_ptr[k + 1] - a;
     if (n <= 0) continue;
     const double* tl = hts_t + a;
     const double* tv = hts_v + a;
-    if (time < tl[0]) c->hbc[k] = tv[0];
-    else if (time >= tl[n - 1]) c->hbc[k] = tv[n - 1];
+    if (time < tl[0]) sh[k] = tv[0];
+    else if (time >= tl[n - 1]) sh[k] = tv[n - 1];
     else
       for (int i = 0; i < n - 1; ++i)
         if (time >= tl[i] && time < tl[i + 1]) {
           double d = time - tl[i], d2 = tl[i + 1] - tl[i];
-          c->hbc[k] = tv[i] + d * (tv[i + 1] - tv[i]) / d2;
+          sh[k] = tv[i] + d * (tv[i + 1] - tv[i]) / d2;
           break;
         }
   }
-  for (int k = 0; k < g.n_in; ++k) {
+  for (int k = ri.in_base; k < ri.in_base + ri.n_in; ++k) {
     int a = qts_ptr[k], n = qts_ptr[k + 1] - a;
     if (n <= 0) continue;
     const double* tl = qts_t + a;
     const double* tv = qts_v + a;
-    if (time < tl[0]) c->qin[k] = tv[0];
-    else if (time >= tl[n - 1]) c->qin[k] = tv[n - 1];
+    if (time < tl[0]) sq[k] = tv[0];
+    else if (time >= tl[n - 1]) sq[k] = tv[n - 1];
     else
       for (int i = 0; i < n - 1; ++i)
         if (time >= tl[i] && time < tl[i + 1]) {
           double d = time - tl[i], d2 = tl[i + 1] - tl[i];
-          c->qin[k] = tv[i] + d * (tv[i + 1] - tv[i]) / d2;
+          sq[k] = tv[i] + d * (tv[i + 1] - tv[i]) / d2;
           break;
         }
   }
@@ -300,9 +311,12 @@ __device__ void get_time_bcs(const Geo& g, const int* out_type, const int* qts_p
 // Step bookkeeping of solverFVT.f90:132-155 (begin) and :338-472 (end of the
 // previous step: residuals, stopping test, time, boundary tables).
 __global__ void __launch_bounds__(CTRL_THREADS)
-k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, const int* out_type,
-          const int* qts_ptr, const double* qts_t, const double* qts_v, const int* hts_ptr,
-          const double* hts_t, const double* hts_v) {
+k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh, int begin,
+          const int* out_type, const int* qts_ptr, const double* qts_t, const double* qts_v,
+          const int* hts_ptr, const double* hts_t, const double* hts_v) {
+  // one block per reach
+  const RInfo ri = g.reach[blockIdx.x];
+  Ctrl* c = &ctrl[blockIdx.x];
   __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS],
       s_dt[CTRL_THREADS];
   const int t = threadIdx.x;
@@ -310,7 +324,7 @@ k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, con
   // fixed-order reduction of the per-block partials of the last stage
   // (residual sums, max|phi|) and of the next step's CFL candidates
   double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX;
-  for (int b = t; b < nparts; b += CTRL_THREADS) {
+  for (int b = ri.tile0 + t; b < ri.tile1; b += CTRL_THREADS) {
     top = top + part[4 * b];
     bot = bot + part[4 * b + 1];
     mx = dmax(mx, part[4 * b + 2]);
@@ -344,7 +358,7 @@ k_control(Geo g, Opts o, Ctrl* c, const double* part, int nparts, int begin, con
     } else {
       c->stime = c->stime + c->dt;
       c->rmax = s_max[0];
-      get_time_bcs(g, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, c->stime, c);
+      get_time_bcs(ri, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, c->stime, sq, sh);
       c->pending = 0;
     }
   }
@@ -373,7 +387,7 @@ struct Vtx {
 
 __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const CellD* d,
                                           const uint8_t* dry, int p, const double4& q,
-                                          const Ctrl* c) {
+                                          const double* sh) {
   double id = ZERO, a = ZERO, zr = q.z;
   int counter = 0;
   const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
@@ -411,7 +425,7 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
   Vtx v;
   v.wet = hv > o.h_dry;
   const int b = g.vbc[p];
-  if (b >= 0) hv = dmax(ZERO, c->hbc[b] - q.z);
+  if (b >= 0) hv = dmax(ZERO, sh[b] - q.z);
   v.h = hv;
   v.zeta = zeta;
   return v;
@@ -420,14 +434,15 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
 // The vertex pass of every stage: depths for the wall fluxes and wet flags
 // for the cells, from the stage input's water surface (NaN marks a dry cell).
 __global__ void __launch_bounds__(256)
-k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* c) {
+k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
+               const double* sh) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
   const double2 zb = g.vzb[p];
-  if (c->done) return;
   const double zv = zb.x;
   const uint64_t bits = (uint64_t)__double_as_longlong(zb.y);
-  const int vb = (int)(uint32_t)bits, deg = (int)(bits >> 32);
+  const int vb = (int)(uint32_t)bits, deg = (int)((bits >> 32) & 0xFF);
+  if (ctrl[(int)(bits >> 40)].done) return;
   const size_t P = (size_t)g.P;
   double id = ZERO, a = ZERO, zr = zv;
   int counter = 0;
@@ -456,27 +471,27 @@ k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, c
   double hv = zeta_v - zv;
   if (hv < ZERO) hv = ZERO;
   vwet[p] = hv > o.h_dry;
-  if (vb >= 0) hv = dmax(ZERO, c->hbc[vb] - zv);
+  if (vb >= 0) hv = dmax(ZERO, sh[vb] - zv);
   hvtx[p] = hv;
 }
 
 // All vertices after the time loop (solverFVT.f90:483), with zetavtx.
 __global__ void __launch_bounds__(256)
 k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_t* vwet,
-         double* zetavtx, const Ctrl* c) {
+         double* zetavtx, const double* sh) {
   const int p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= g.P) return;
-  const Vtx v = vertex_idw(g, o, d, dry, p, g.vxyz[p], c);
+  const Vtx v = vertex_idw(g, o, d, dry, p, g.vxyz[p], sh);
   vwet[p] = v.wet;
   hvtx[p] = v.h;
   if (zetavtx) zetavtx[p] = v.zeta;
 }
 
 // ---------------------------------------------------------------- Green-Gauss pass 1
-__global__ void k_gauss1(Geo g, const CellD* d, double* gwork, const Ctrl* c) {
+__global__ void k_gauss1(Geo g, const CellD* d, double* gwork) {
   // gaussgrad.f90:37-42 for zeta, u, v
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
-  if (i >= g.N || c->done) return;
+  if (i >= g.N) return;
   const int N = g.N;
   const int n0 = g.gconn[i], n1 = g.gconn[N + i], n2 = g.gconn[2 * N + i];
   const double cx0 = g.gcx[i], cx1 = g.gcx[N + i], cx2 = g.gcx[2 * N + i];
@@ -672,8 +687,7 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
 
 // The local dt of the reference's last cell from the stage input, which the
 // steady-mode implicit friction uses (solverFVT.f90:260-265,336).
-__device__ double last_cell_dt(const Geo& g, const Opts& o, const CellD* din) {
-  const int i = g.last_cell;
+__device__ double last_cell_dt(const Geo& g, const Opts& o, const CellD* din, int i) {
   const CellD d = ld(&din[i]);
   const Tri tr = load_tri(g, i);
   double umg = ZERO;
@@ -687,12 +701,14 @@ __device__ double last_cell_dt(const Geo& g, const Opts& o, const CellD* din) {
 
 // Flux through one boundary side of cell i (the routines of
 // invisc_fluxes.f90:195-238), as the change of phi(i,:).  h1, h2: vertex
-// depths at the ends of the edge in the reference's edge direction; q_*:
-// the inflowQbyH results of the strings.
-__device__ void boundary_flux(const Geo& g, const Opts& o, const Ctrl* c, const double* q_at,
-                              const int* q_null, const int* q_skip, uint32_t bits, const Side& sd,
-                              double fh, double fu, double fv, double hi, bool dry_i, double h1,
-                              double h2, double& d0, double& d1, double& d2) {
+// depths at the ends of the edge in the reference's edge direction; sq, sh:
+// the current values of every string; q_*: the inflowQbyH results of the
+// reach's strings (index local to the reach).
+__device__ void boundary_flux(const Opts& o, const RInfo& ri, const double* sq, const double* sh,
+                              const double* q_at, const int* q_null, const int* q_skip,
+                              uint32_t bits, const Side& sd, double fh, double fu, double fv,
+                              double hi, bool dry_i, double h1, double h2, double& d0, double& d1,
+                              double& d2) {
   const uint32_t kind = bits & CMS_KIND;
   const double h_dry = o.h_dry;
   const double nx = sd.nx, ny = sd.ny, L = sd.len;
@@ -724,10 +740,10 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const Ctrl* c, const 
     // inflowQbyH.f90:131-210 (vtype 1) / :321-400 (vtype 5)
     if (q_skip[ch]) return;
     if (kind == SK_IN_Q5 && dry_i) return;
-    const double qin = c->qin[ch];
+    const double qin = sq[ri.in_base + ch];
     const double atotal = q_at[ch];
     double hl = fh, ue, ve;
-    if (q_null[ch]) hl = g.bch_max;
+    if (q_null[ch]) hl = ri.bch_max;
     if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
     else { ue = fu; ve = fv; }
     const double qfrac = qin / atotal;
@@ -747,7 +763,7 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const Ctrl* c, const 
     // inflowH.f90:31-91
     double hl = dmax(fh, ZERO), ue, ve;
     if (hi < h_dry) { ue = ZERO; ve = ZERO; } else { ue = fu; ve = fv; }
-    const double hr = dmax(ZERO, c->qin[ch] - sd.zm);
+    const double hr = dmax(ZERO, sq[ri.in_base + ch] - sd.zm);
     const double ul = ue * nx + ve * ny;
     const double sl = sqrt(G * hl), sr = sqrt(G * hr);
     double urr = ul + (sl - sr);
@@ -762,7 +778,7 @@ __device__ void boundary_flux(const Geo& g, const Opts& o, const Ctrl* c, const 
     double hl = fh, ue, ve;
     if (dry_i || hl < h_dry) { ue = ZERO; ve = ZERO; hl = dmax(hl, ZERO); }
     else { ue = fu; ve = fv; }
-    double hr = dmax(ZERO, c->hbc[ch] - sd.zm);
+    double hr = dmax(ZERO, sh[ri.out_base + ch] - sd.zm);
     if (hl < h_dry && hr < h_dry) return;
     double ul, vl, urr, vrr;
     if (hr < h_dry) { urr = ZERO; ul = ue; vrr = ZERO; vl = ve; }
@@ -929,12 +945,13 @@ __device__ __forceinline__ Side side_dyn(const Lds& L, int t, int j) {
 // across the block.
 template <int CAP>
 __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const double* gwork,
-                           const uint8_t* vwet, const Ctrl* c, int ch, const Lds& L,
-                           double* s_at, int* s_null, int* s_skip) {
+                           const uint8_t* vwet, const double* sq, const RInfo& ri, int ch,
+                           const Lds& L, double* s_at, int* s_null, int* s_skip) {
   const int t = threadIdx.x;
-  const int vt = g.in_type[ch];
-  const int e0 = g.qs_ptr[ch], n = g.qs_ptr[ch + 1] - e0;
-  const double qin = c->qin[ch];
+  const int gch = ri.in_base + ch;  // the string's index among all strings
+  const int vt = g.in_type[gch];
+  const int e0 = g.qs_ptr[gch], n = g.qs_ptr[gch + 1] - e0;
+  const double qin = sq[gch];
   if (vt != 1 && vt != 5) return;
   if (qin < DBL_MIN) {
     if (t == 0) s_skip[ch] = 1;
@@ -979,7 +996,7 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
       int ok = 0;
       if (act) {
         double hl = L.qh[l], ue, ve;
-        if (null_area) hl = g.bch_max;
+        if (null_area) hl = ri.bch_max;
         if (hl < h_dry) { hl = dmax(hl, ZERO); ue = ZERO; ve = ZERO; }
         else { ue = L.qu[l]; ve = L.qv[l]; }
         const double nx = g.qs_nx[e], ny = g.qs_ny[e], len = g.qs_len[e];
@@ -1033,23 +1050,25 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   __shared__ double s_at[MAX_STRINGS];
   __shared__ int s_null[MAX_STRINGS], s_skip[MAX_STRINGS];
   __shared__ double s_dlast;
-  const Ctrl* c = S.ctrl;
-  if (c->done) return;
   const int4 hd0 = tl.hdr[2 * blockIdx.x], hd1 = tl.hdr[2 * blockIdx.x + 1];
   const int c0 = hd0.x, T = hd0.y, hb = hd0.z, H = hd0.w, b = hd1.z;
+  const Ctrl* c = &S.ctrl[hd1.w];
+  if (c->done) return;
+  const RInfo ri = g.reach[hd1.w];
   const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
   const int t = threadIdx.x;
   const int N = g.N;
   const Lds L = lds_layout(lds_dyn, CAP);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (t == 0 && o.steady && st.last) {
-    s_dlast = last_cell_dt(g, o, S.din);
-    if (blockIdx.x == 0) S.ctrl->delt_last = s_dlast;  // k_control advances ttime by it
+    s_dlast = last_cell_dt(g, o, S.din, ri.last_cell);
+    // k_control advances the reach's ttime by it
+    if (b == ri.tile0) S.ctrl[hd1.w].delt_last = s_dlast;
   }
 
   // 2. inflow discharge strings (their buffers overlay the cell arrays)
   for (uint64_t qm = qmask; qm; qm &= qm - 1)
-    bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, c, __builtin_ctzll(qm), L, s_at, s_null,
+    bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, ri, __builtin_ctzll(qm), L, s_at, s_null,
                     s_skip);
 
   // 3. face states of the tile and halo cells
@@ -1127,8 +1146,8 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
         const int va = g.cvtx[j * N + i], vb = g.cvtx[(j == 2 ? 0 : j + 1) * N + i];
         const double ha = S.hvtx[va], hb2 = S.hvtx[vb];
         const bool rev = (bits >> CMS_REV) & 1;
-        boundary_flux(g, o, c, s_at, s_null, s_skip, bits, sd, fho, fuo, fvo, hi, dry_i,
-                      rev ? hb2 : ha, rev ? ha : hb2, d0, d1, d2);
+        boundary_flux(o, ri, S.sq, S.sh, s_at, s_null, s_skip, bits, sd, fho, fuo, fvo, hi,
+                      dry_i, rev ? hb2 : ha, rev ? ha : hb2, d0, d1, d2);
       }
     };
     // the sides in the reference's accumulation order (invisc_fluxes:
@@ -1385,6 +1404,7 @@ struct storm_fvt {
   double *d_qts_t = nullptr, *d_qts_v = nullptr, *d_hts_t = nullptr, *d_hts_v = nullptr;
   Tiles tl{};
   int cap = 0;
+  std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
   int iter_host = 0;
   bool started = false;
   bool finalized = false;
@@ -1497,7 +1517,7 @@ void setup_device(storm_fvt* C) {
   g.qs_nx = dupload(own, P.qs_nx); g.qs_ny = dupload(own, P.qs_ny);
   g.qs_len = dupload(own, P.qs_len);
   g.in_type = dupload(own, P.in_type);
-  g.n_in = P.n_in; g.n_out = P.n_out; g.last_cell = P.last_cell; g.bch_max = P.bch_max;
+  g.n_in = P.n_in; g.n_out = P.n_out; g.nr = P.nr;
   upload_tables(C);
   Opts& o = C->o;
   const storm_fvt_opts_t& q = P.o;
@@ -1521,6 +1541,20 @@ void setup_device(storm_fvt* C) {
   tl.hmeta = dupload(own, P.t_hmeta);
   tl.cln = dupload(own, P.t_cln);
   tl.nt = P.nt;
+  {
+    std::vector<RInfo> ri(P.nr);
+    for (int r = 0; r < P.nr; ++r) {
+      RInfo& q = ri[r];
+      q.in_base = P.r_in[r]; q.n_in = P.r_in[r + 1] - P.r_in[r];
+      q.out_base = P.r_out[r]; q.n_out = P.r_out[r + 1] - P.r_out[r];
+      q.tile0 = P.r_tile[r]; q.tile1 = P.r_tile[r + 1];
+      q.last_cell = P.r_last[r];
+      q.pad = 0;
+      q.bch_max = P.r_bch[r];
+      q.pad2 = 0.0;
+    }
+    C->g.reach = dupload(own, ri);
+  }
   set_lds_limit(C->cap);
   State& S = C->S;
   S.gwork = (q.gradient == STORM_GRAD_GAUSS) ? dalloc<double>(own, 6 * N) : nullptr;
@@ -1533,7 +1567,9 @@ void setup_device(storm_fvt* C) {
   S.part = dalloc<double>(own, 4 * (size_t)P.nt);
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
-  S.ctrl = dalloc<Ctrl>(own, 1);
+  S.ctrl = dalloc<Ctrl>(own, P.nr);
+  S.sq = dalloc<double>(own, P.n_in);
+  S.sh = dalloc<double>(own, P.n_out);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
 }
 
@@ -1552,10 +1588,17 @@ void launch_stage(storm_fvt* C, const Stage& st) {
 void start_run(storm_fvt* C) {
   // solverFVT.f90:53-127
   Prep& P = C->prep;
-  Ctrl hc;
-  std::memset(&hc, 0, sizeof(hc));
-  host_time_bcs(P, 0.0, hc.qin, hc.hbc);
-  HIPCHK(hipMemcpyAsync(C->S.ctrl, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, C->stream));
+  C->hctrl.assign(P.nr, Ctrl{});
+  std::vector<double> qin(std::max(P.n_in, 1), 0.0), hbc(std::max(P.n_out, 1), 0.0);
+  host_time_bcs(P, 0.0, qin.data(), hbc.data());
+  HIPCHK(hipMemcpyAsync(C->S.ctrl, C->hctrl.data(), sizeof(Ctrl) * P.nr, hipMemcpyHostToDevice,
+                        C->stream));
+  if (P.n_in)
+    HIPCHK(hipMemcpyAsync(C->S.sq, qin.data(), sizeof(double) * P.n_in, hipMemcpyHostToDevice,
+                          C->stream));
+  if (P.n_out)
+    HIPCHK(hipMemcpyAsync(C->S.sh, hbc.data(), sizeof(double) * P.n_out, hipMemcpyHostToDevice,
+                          C->stream));
   C->r1_flip = 0;
   switch (C->cap) {
     case 256: launch_init<256>(C); break;
@@ -1586,9 +1629,9 @@ void rk_table(int R, double a[3][3], double b[3]) {
 }
 
 void launch_control(storm_fvt* C, int begin) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_THREADS), 0, C->stream, C->g, C->o, C->S.ctrl,
-                     C->S.part, C->tl.nt, begin, C->d_out_type, C->d_qts_ptr, C->d_qts_t,
-                     C->d_qts_v, C->d_hts_ptr, C->d_hts_t, C->d_hts_v);
+  hipLaunchKernelGGL(k_control, dim3(C->prep.nr), dim3(CTRL_THREADS), 0, C->stream, C->g, C->o,
+                     C->S.ctrl, C->S.part, C->S.sq, C->S.sh, begin, C->d_out_type, C->d_qts_ptr,
+                     C->d_qts_t, C->d_qts_v, C->d_hts_ptr, C->d_hts_t, C->d_hts_v);
 }
 
 void launch_step(storm_fvt* C) {
@@ -1607,10 +1650,10 @@ void launch_step(storm_fvt* C) {
     st.s = s;
     st.last = (s == R);
     hipLaunchKernelGGL(k_vertex_stage, dim3((P.P + 255) / 256), dim3(256), 0, C->stream, C->g,
-                       C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl);
+                       C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
     if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
       hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
-                         C->S.din, C->S.gwork, C->S.ctrl);
+                         C->S.din, C->S.gwork);
     switch (C->cap) {
       case 256: launch_stage<256>(C, st); break;
       case 384: launch_stage<384>(C, st); break;
@@ -1644,13 +1687,14 @@ void storm_fvt_default_opts(storm_fvt_opts_t* o) {
   o->reorder = 1; o->block = 0; o->graph = 0;
 }
 
-int storm_fvt_create(const storm_mesh_t* mesh, const storm_fvt_opts_t* opts, storm_fvt_t** ctx) {
+static int create_ctx(const storm_mesh_t* mesh, const storm_fvt_opts_t* opts,
+                      const ReachSplit* split, storm_fvt_t** ctx) {
   if (!ctx) return STORM_E_ARG;
   *ctx = nullptr;
   std::unique_ptr<storm_fvt> C(new (std::nothrow) storm_fvt());
   if (!C) return STORM_E_NOMEM;
   try {
-    int rc = build_prep(mesh, opts, C->prep);
+    int rc = build_prep(mesh, opts, C->prep, split);
     if (rc != STORM_OK) {
       std::fprintf(stderr, "storm_fvt_create: %s\n", C->prep.err.c_str());
       return rc;
@@ -1663,6 +1707,123 @@ int storm_fvt_create(const storm_mesh_t* mesh, const storm_fvt_opts_t* opts, sto
     return STORM_E_NOMEM;
   }
   *ctx = C.release();
+  return STORM_OK;
+}
+
+int storm_fvt_create(const storm_mesh_t* mesh, const storm_fvt_opts_t* opts, storm_fvt_t** ctx) {
+  return create_ctx(mesh, opts, nullptr, ctx);
+}
+
+int storm_fvt_create_batch(int32_t n, const storm_mesh_t* meshes, const storm_fvt_opts_t* opts,
+                           storm_fvt_t** ctx) {
+  if (!ctx) return STORM_E_ARG;
+  *ctx = nullptr;
+  if (n < 1 || !meshes || !opts) return STORM_E_ARG;
+  // the reaches laid end to end as one mesh: ids offset, strings and tables
+  // appended in reach order
+  const storm_mesh_t& m0 = meshes[0];
+  int64_t np = 0, ne = 0, ned = 0, nin = 0, nout = 0, nin_nodes = 0, nout_nodes = 0, nqt = 0, nht = 0;
+  bool edges = true;
+  for (int r = 0; r < n; ++r) {
+    const storm_mesh_t& m = meshes[r];
+    if (m.n_pts < 3 || m.n_elems < 1 || m.n_in < 0 || m.n_out < 0 || !m.x || !m.y || !m.tri ||
+        !m.z || !m.h || !m.u || !m.v || !m.cd || m.z_at_vtx != m0.z_at_vtx ||
+        m.h_at_vtx != m0.h_at_vtx || m.u_at_vtx != m0.u_at_vtx || m.v_at_vtx != m0.v_at_vtx ||
+        m.cd_at_vtx != m0.cd_at_vtx)
+      return STORM_E_ARG;
+    np += m.n_pts; ne += m.n_elems;
+    edges = edges && m.n_edges > 0 && m.edge_p && m.edge_e;
+    ned += m.n_edges;
+    nin += m.n_in; nout += m.n_out;
+    if (m.n_in) { nin_nodes += m.in_ptr[m.n_in]; nqt += m.qts_ptr[m.n_in]; }
+    if (m.n_out) { nout_nodes += m.out_ptr[m.n_out]; nht += m.hts_ptr[m.n_out]; }
+  }
+  if (np > INT32_MAX / 2 || ne > INT32_MAX / 4) return STORM_E_ARG;
+  std::vector<double> x, y, z, h, u, v, cd, qt, qv, ht, hv;
+  std::vector<int32_t> tri, ep, ee, in_type, in_ptr{0}, in_nodes, qts_ptr{0}, out_type, out_ptr{0},
+      out_nodes, hts_ptr{0};
+  x.reserve(np); y.reserve(np); tri.reserve(3 * ne);
+  ReachSplit sp;
+  sp.cell_ptr = {0}; sp.in_ptr = {0}; sp.out_ptr = {0};
+  int32_t po = 0, co = 0;
+  auto app = [](std::vector<double>& d, const double* s, int k) { d.insert(d.end(), s, s + k); };
+  for (int r = 0; r < n; ++r) {
+    const storm_mesh_t& m = meshes[r];
+    app(x, m.x, m.n_pts); app(y, m.y, m.n_pts);
+    for (int t = 0; t < 3 * m.n_elems; ++t) tri.push_back(m.tri[t] + po);
+    if (edges)
+      for (int t = 0; t < 2 * m.n_edges; ++t) {
+        ep.push_back(m.edge_p[t] + po);
+        ee.push_back(m.edge_e[t] < 0 ? -1 : m.edge_e[t] + co);
+      }
+    app(z, m.z, m.z_at_vtx ? m.n_pts : m.n_elems);
+    app(h, m.h, m.h_at_vtx ? m.n_pts : m.n_elems);
+    app(u, m.u, m.u_at_vtx ? m.n_pts : m.n_elems);
+    app(v, m.v, m.v_at_vtx ? m.n_pts : m.n_elems);
+    app(cd, m.cd, m.cd_at_vtx ? m.n_pts : m.n_elems);
+    for (int k = 0; k < m.n_in; ++k) {
+      in_type.push_back(m.in_type[k]);
+      for (int t = m.in_ptr[k]; t < m.in_ptr[k + 1]; ++t) in_nodes.push_back(m.in_nodes[t] + po);
+      in_ptr.push_back((int32_t)in_nodes.size());
+      for (int t = m.qts_ptr[k]; t < m.qts_ptr[k + 1]; ++t) {
+        qt.push_back(m.qts_t[t]);
+        qv.push_back(m.qts_v[t]);
+      }
+      qts_ptr.push_back((int32_t)qt.size());
+    }
+    for (int k = 0; k < m.n_out; ++k) {
+      out_type.push_back(m.out_type[k]);
+      for (int t = m.out_ptr[k]; t < m.out_ptr[k + 1]; ++t) out_nodes.push_back(m.out_nodes[t] + po);
+      out_ptr.push_back((int32_t)out_nodes.size());
+      for (int t = m.hts_ptr[k]; t < m.hts_ptr[k + 1]; ++t) {
+        ht.push_back(m.hts_t[t]);
+        hv.push_back(m.hts_v[t]);
+      }
+      hts_ptr.push_back((int32_t)ht.size());
+    }
+    po += m.n_pts; co += m.n_elems;
+    sp.cell_ptr.push_back(co);
+    sp.in_ptr.push_back((int)in_type.size());
+    sp.out_ptr.push_back((int)out_type.size());
+  }
+  (void)nin; (void)nout; (void)nin_nodes; (void)nout_nodes; (void)nqt; (void)nht; (void)ned;
+  storm_mesh_t M{};
+  M.n_pts = po; M.n_elems = co;
+  M.x = x.data(); M.y = y.data(); M.tri = tri.data();
+  M.n_edges = edges ? (int32_t)(ep.size() / 2) : 0;
+  M.edge_p = edges ? ep.data() : nullptr;
+  M.edge_e = edges ? ee.data() : nullptr;
+  M.z_at_vtx = m0.z_at_vtx; M.h_at_vtx = m0.h_at_vtx; M.u_at_vtx = m0.u_at_vtx;
+  M.v_at_vtx = m0.v_at_vtx; M.cd_at_vtx = m0.cd_at_vtx;
+  M.z = z.data(); M.h = h.data(); M.u = u.data(); M.v = v.data(); M.cd = cd.data();
+  M.n_in = (int32_t)in_type.size();
+  M.in_type = in_type.data(); M.in_ptr = in_ptr.data(); M.in_nodes = in_nodes.data();
+  M.qts_ptr = qts_ptr.data(); M.qts_t = qt.data(); M.qts_v = qv.data();
+  M.n_out = (int32_t)out_type.size();
+  M.out_type = out_type.data(); M.out_ptr = out_ptr.data(); M.out_nodes = out_nodes.data();
+  M.hts_ptr = hts_ptr.data(); M.hts_t = ht.data(); M.hts_v = hv.data();
+  return create_ctx(&M, opts, &sp, ctx);
+}
+
+int storm_fvt_reach_status(storm_fvt_t* C, int32_t* iter, double* ttime, double* rmax,
+                           uint8_t* done) {
+  if (!C) return STORM_E_ARG;
+  try {
+    if (!C->started) start_run(C);
+    const int nr = C->prep.nr;
+    HIPCHK(hipMemcpyAsync(C->hctrl.data(), C->S.ctrl, sizeof(Ctrl) * nr, hipMemcpyDeviceToHost,
+                          C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (int r = 0; r < nr; ++r) {
+      const Ctrl& c = C->hctrl[r];
+      if (iter) iter[r] = c.iter;
+      if (ttime) ttime[r] = c.ttime;
+      if (rmax) rmax[r] = c.rmax;
+      if (done) done[r] = (uint8_t)(c.done != 0);
+    }
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  }
   return STORM_OK;
 }
 
@@ -1721,18 +1882,26 @@ int storm_fvt_run(storm_fvt_t* C, int32_t nsteps, int32_t* iter, double* ttime, 
     if (n > 0) launch_control(C, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(C->ev1, C->stream));
-    C->iter_host += n;
-    Ctrl hc;
-    HIPCHK(hipMemcpyAsync(&hc, C->S.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, C->stream));
+    const int nr = C->prep.nr;
+    HIPCHK(hipMemcpyAsync(C->hctrl.data(), C->S.ctrl, sizeof(Ctrl) * nr, hipMemcpyDeviceToHost,
+                          C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, C->ev0, C->ev1));
     C->last_run_s = ms * 1e-3;
-    C->iter_host = hc.iter;
-    if (hc.done) C->iter_host = C->prep.o.istop;  // converged: no further steps
-    if (iter) *iter = hc.iter;
-    if (ttime) *ttime = hc.ttime;
-    if (rmax) *rmax = hc.rmax;
+    // steps taken so far: the most any reach has taken; once every reach
+    // has converged no further step is launched
+    bool all_done = true;
+    C->iter_host = 0;
+    for (const Ctrl& hc : C->hctrl) {
+      C->iter_host = std::max(C->iter_host, hc.iter);
+      all_done = all_done && hc.done;
+    }
+    if (all_done) C->iter_host = C->prep.o.istop;
+    const Ctrl& h0 = C->hctrl[0];
+    if (iter) *iter = h0.iter;
+    if (ttime) *ttime = h0.ttime;
+    if (rmax) *rmax = h0.rmax;
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
@@ -1751,7 +1920,7 @@ int storm_fvt_finalize(storm_fvt_t* C) {
     const bool stepped = C->iter_host > 0;
     hipLaunchKernelGGL(k_vertex, dim3((P.P + bs - 1) / bs), dim3(bs), 0, C->stream, C->g, C->o,
                        C->last_din, stepped ? C->S.dry_post : (const uint8_t*)nullptr,
-                       C->S.hvtx, C->S.vwet, C->S.zetavtx, C->S.ctrl);
+                       C->S.hvtx, C->S.vwet, C->S.zetavtx, C->S.sh);
     hipLaunchKernelGGL(k_final_cells, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g,
                        stepped ? C->S.dry_post : (const uint8_t*)nullptr, C->S.vwet,
                        const_cast<CellD*>(C->S.din));

@@ -194,7 +194,7 @@ int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int
   return molecule(m->n_elems, (int)(ep.size() / 2), ee, ptr, cells, err);
 }
 
-int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
+int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const ReachSplit* split) {
   if (!m || !o) { S.err = "null mesh or options"; return STORM_E_ARG; }
   if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
   if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 1 || o->limiter < 0 ||
@@ -207,10 +207,37 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
       !m->v || !m->cd) {
     S.err = "incomplete mesh description"; return STORM_E_ARG;
   }
-  if (m->n_in < 0 || m->n_out < 0 || m->n_in > MAX_STRINGS || m->n_out > MAX_STRINGS) {
-    S.err = "too many boundary strings (max 64 each)"; return STORM_E_ARG;
-  }
+  if (m->n_in < 0 || m->n_out < 0) { S.err = "negative string count"; return STORM_E_ARG; }
   const int N = m->n_elems, P = m->n_pts;
+  // reaches: one unless a batch split is given
+  ReachSplit one;
+  if (!split) {
+    one.cell_ptr = {0, N};
+    one.in_ptr = {0, m->n_in};
+    one.out_ptr = {0, m->n_out};
+    split = &one;
+  }
+  const int NR = (int)split->cell_ptr.size() - 1;
+  if (NR < 1 || split->cell_ptr[0] != 0 || split->cell_ptr[NR] != N ||
+      (int)split->in_ptr.size() != NR + 1 || (int)split->out_ptr.size() != NR + 1 ||
+      split->in_ptr[NR] != m->n_in || split->out_ptr[NR] != m->n_out) {
+    S.err = "inconsistent reach split"; return STORM_E_ARG;
+  }
+  for (int r = 0; r < NR; ++r)
+    if (split->cell_ptr[r + 1] <= split->cell_ptr[r] ||
+        split->in_ptr[r + 1] - split->in_ptr[r] > MAX_STRINGS ||
+        split->out_ptr[r + 1] - split->out_ptr[r] > MAX_STRINGS ||
+        split->in_ptr[r + 1] < split->in_ptr[r] || split->out_ptr[r + 1] < split->out_ptr[r]) {
+      S.err = "empty reach or too many boundary strings (max 64 of each kind per reach)";
+      return STORM_E_ARG;
+    }
+  S.nr = NR;
+  S.r_cell = split->cell_ptr;
+  S.r_in = split->in_ptr;
+  S.r_out = split->out_ptr;
+  std::vector<int> cell_reach(N);
+  for (int r = 0; r < NR; ++r)
+    for (int i = split->cell_ptr[r]; i < split->cell_ptr[r + 1]; ++i) cell_reach[i] = r;
   for (int t = 0; t < 3 * N; ++t)
     if (m->tri[t] < 0 || m->tri[t] >= P) { S.err = "triangle vertex id out of range"; return STORM_E_MESH; }
   S.o = *o;
@@ -485,12 +512,13 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     gwt[3 * i + 1] = ((xp[2] - xc[i]) * (yp[0] - yp[2]) - (xp[0] - xp[2]) * (yp[2] - yc[i])) / den;
     gwt[3 * i + 2] = ((xp[0] - xc[i]) * (yp[1] - yp[0]) - (xp[1] - xp[0]) * (yp[0] - yc[i])) / den;
   }
-  // dataprepFVT.f90:278-286
-  S.bch_max = ZERO;
+  // dataprepFVT.f90:278-286, per reach
+  S.r_bch.assign(NR, ZERO);
   for (int k = 0; k < E; ++k)
     if (qbar_e[k] && cnt[k] == 1) {
-      S.bch_max = std::max(S.bch_max, zvtx[ep[2 * k]]);
-      S.bch_max = std::max(S.bch_max, zvtx[ep[2 * k + 1]]);
+      double& bm = S.r_bch[cell_reach[ee[2 * k]]];
+      bm = std::max(bm, zvtx[ep[2 * k]]);
+      bm = std::max(bm, zvtx[ep[2 * k + 1]]);
     }
   // dataprepFVT.f90:293-306: highest vertex; bed_slope_VTX.f90:22-38
   std::vector<double> zhigh(N), dzx(N), dzy(N);
@@ -520,8 +548,15 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
 
   // ---- cell ordering --------------------------------------------------
   std::vector<int> order;
-  if (o->reorder) order = hilbert_order(N, xc, yc);
-  else { order.resize(N); std::iota(order.begin(), order.end(), 0); }
+  order.resize(N);
+  std::iota(order.begin(), order.end(), 0);
+  if (o->reorder)
+    for (int r = 0; r < NR; ++r) {
+      const int a = split->cell_ptr[r], n = split->cell_ptr[r + 1] - a;
+      std::vector<double> rx(xc.begin() + a, xc.begin() + a + n), ry(yc.begin() + a, yc.begin() + a + n);
+      const std::vector<int> ro = hilbert_order(n, rx, ry);
+      for (int k = 0; k < n; ++k) order[a + k] = a + ro[k];
+    }
   S.perm = order;
   S.iperm.assign(N, 0);
   for (int i = 0; i < N; ++i) S.iperm[order[i]] = i;
@@ -580,7 +615,10 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
         // inflowH, outflowbyH, crit_outflow, free_flow
         uint64_t rank = (kind == SK_IN_Q1) ? 0 : (kind == SK_IN_Q5) ? 1 : (kind == SK_IN_H) ? 2
                       : (kind == SK_OUT_H) ? 3 : (kind == SK_OUT_CRIT) ? 4 : (kind == SK_OUT_FREE) ? 5 : 6;
-        slot |= kind | (1ull << CMS_LEFT) | ((uint64_t)e_str[k] << CMS_CHAIN);
+        const int rr = cell_reach[i];
+        const int base = (kind == SK_IN_Q1 || kind == SK_IN_Q5 || kind == SK_IN_H || kind == SK_NONE)
+                             ? split->in_ptr[rr] : split->out_ptr[rr];
+        slot |= kind | (1ull << CMS_LEFT) | ((uint64_t)(e_str[k] - base) << CMS_CHAIN);
         keys[j] = ((uint64_t)2 << 60) | (rank << 52) | ((uint64_t)e_str[k] << 32) | (uint64_t)e_pos[k];
       }
       if (hbar_e[k] && cnt[k] == 1) slot |= 1ull << CMS_HBAR;
@@ -614,7 +652,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
   }
   // the vertex pass's tables: row r of vertex p at r*P + p (n2t order), the
   // inverse-distance weights of c2v_setup.f90:106-113, and per vertex
-  // {zvtx, string index | degree << 32}
+  // {zvtx, string index | degree << 32 | reach << 40}
   S.vW = 0;
   for (int p = 0; p < P; ++p) S.vW = std::max(S.vW, n2t_ptr[p + 1] - n2t_ptr[p]);
   S.vell.assign((size_t)S.vW * P, -1);
@@ -626,7 +664,9 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
       S.vell[(size_t)t * P + pp] = ip[n2t[n2t_ptr[p] + t]];
       S.vwt[(size_t)t * P + pp] = c2vwid[n2t_ptr[p] + t];
     }
-    const uint64_t bits = (uint64_t)(uint32_t)S.vbc[pp] | ((uint64_t)(uint32_t)deg << 32);
+    if (deg > 255) { S.err = "a vertex has more than 255 triangles"; return STORM_E_MESH; }
+    const int vr = deg > 0 ? cell_reach[n2t[n2t_ptr[p]]] : 0;
+    const uint64_t bits = (uint64_t)(uint32_t)S.vbc[pp] | ((uint64_t)deg << 32) | ((uint64_t)vr << 40);
     S.vzb[2 * (size_t)pp] = zvtx[p];
     std::memcpy(&S.vzb[2 * (size_t)pp + 1], &bits, sizeof(bits));
   }
@@ -654,7 +694,8 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S) {
     }
     S.qs_ptr[ch + 1] = (int)S.qs_cell.size();
   }
-  S.last_cell = ip[N - 1];
+  S.r_last.resize(NR);
+  for (int r = 0; r < NR; ++r) S.r_last[r] = ip[split->cell_ptr[r + 1] - 1];
   return STORM_OK;
 }
 
@@ -666,10 +707,16 @@ int build_tiles(Prep& S, int cap) {
   // edge neighbours fit the capacity
   std::vector<int> hstamp(N, -1);
   S.t_ptr.assign(1, 0);
-  int start = 0, tile = 0;
+  S.r_tile.assign(1, 0);
+  int start = 0, tile = 0, reach = 0;
   while (start < N) {
+    while (S.r_cell[reach + 1] <= start) {
+      S.r_tile.push_back(tile);
+      ++reach;
+    }
+    const int end = S.r_cell[reach + 1];
     int T = 0, H = 0;
-    while (start + T < N) {
+    while (start + T < end) {
       const int c = start + T;
       int newh[3], nnh = 0;
       const bool was_halo = hstamp[c] == tile;
@@ -691,9 +738,13 @@ int build_tiles(Prep& S, int cap) {
     S.t_ptr.push_back(start);
     ++tile;
   }
+  while ((int)S.r_tile.size() < S.nr + 1) S.r_tile.push_back(tile);
   const int nt = tile;
   S.nt = nt;
   S.cap = cap;
+  std::vector<int> tile_reach(nt);
+  for (int r = 0; r < S.nr; ++r)
+    for (int b = S.r_tile[r]; b < S.r_tile[r + 1]; ++b) tile_reach[b] = r;
   // per tile: halo list (ascending ids) and 16-bit local neighbour ids
   S.t_hptr.assign(1, 0); S.t_hcell.clear();
   S.t_cln.assign(N, 0);
@@ -749,6 +800,7 @@ int build_tiles(Prep& S, int cap) {
     h[2] = S.t_hptr[b]; h[3] = S.t_hptr[b + 1] - S.t_hptr[b];
     h[4] = (int)(uint32_t)(S.t_qmask[b] & 0xFFFFFFFFu); h[5] = (int)(uint32_t)(S.t_qmask[b] >> 32);
     h[6] = b;
+    h[7] = tile_reach[b];
   }
   const size_t nh = S.t_hcell.size();
   S.t_hrec.assign(8 * nh, 0);

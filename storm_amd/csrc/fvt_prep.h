@@ -38,7 +38,15 @@ constexpr int CM_SLOT0 = 6, CM_SLOT_BITS = 18;
 constexpr uint64_t CMS_KIND = 0xF;
 constexpr int CMS_NL = 4, CMS_LEFT = 6, CMS_HBAR = 7, CMS_REV = 8, CMS_CHAIN = 9;
 
-constexpr int MAX_STRINGS = 64;
+constexpr int MAX_STRINGS = 64;  // boundary strings of each kind per reach
+
+// A batch of independent reaches laid end to end in one mesh: reach r owns
+// input cells [cell_ptr[r], cell_ptr[r+1]), inflow strings [in_ptr[r],
+// in_ptr[r+1]) and outflow strings [out_ptr[r], out_ptr[r+1]).  No edge joins
+// two reaches.
+struct ReachSplit {
+  std::vector<int> cell_ptr, in_ptr, out_ptr;
+};
 
 struct Prep {
   int N = 0, P = 0, E = 0, R = 2;
@@ -59,7 +67,7 @@ struct Prep {
   int vW = 0;                         // largest vertex degree
   std::vector<int> vell;              // [vW][P] incident cells in n2t order, -1 past the degree
   std::vector<double> vwt;            // [vW][P] their inverse-distance weights
-  std::vector<double> vzb;            // [2][P] interleaved: zvtx, bits (vbc | degree << 32)
+  std::vector<double> vzb;            // [2][P] interleaved: zvtx, bits (vbc | degree << 32 | reach << 40)
   // boundary strings
   int n_in = 0, n_out = 0;
   std::vector<int> in_type, out_type;
@@ -69,8 +77,12 @@ struct Prep {
   // local edge and geometry
   std::vector<int> qs_ptr, qs_cell, qs_slot;
   std::vector<double> qs_rcx, qs_rcy, qs_nx, qs_ny, qs_len;
-  double bch_max = 0.0;
-  int last_cell = 0;                  // internal id of input cell N-1
+  // reaches (one unless a batch): internal cells [r_cell[r], r_cell[r+1]),
+  // first string of each kind, the internal id of the reach's last input
+  // cell (its local dt drives steady friction), bch_max of dataprepFVT
+  int nr = 1;
+  std::vector<int> r_cell, r_in, r_out, r_last, r_tile;
+  std::vector<double> r_bch;
   std::vector<double> h0, u0, v0;     // initial state (internal order)
   // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
   // (edge neighbours outside) is listed per tile; local cell ids are 16
@@ -91,10 +103,11 @@ struct Prep {
 
 // Builds everything from the C-ABI mesh; returns STORM_OK or a STORM_E_* code
 // with P.err set.
-int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& P);
+int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& P,
+               const ReachSplit* split = nullptr);
 
-// Tiles of at most `cap` cells + halo cells (needs the cell tables of
-// build_prep).
+// Tiles of at most `cap` cells + halo cells, none spanning two reaches
+// (needs the cell tables of build_prep).
 int build_tiles(Prep& P, int cap);
 
 // Least-squares molecule only (elemnt_db.f90:42-128), input numbering.

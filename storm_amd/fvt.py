@@ -59,7 +59,8 @@ class Opts(C.Structure):
                 ("reserved", C.c_int32)]
 
 
-EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_set_state",
+EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch",
+           "storm_fvt_reach_status", "storm_fvt_set_state",
            "storm_fvt_set_bc_values", "storm_fvt_run", "storm_fvt_finalize",
            "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
            "storm_fvt_device_info", "storm_fvt_last_run_seconds", "storm_fvt_last_error",
@@ -79,6 +80,10 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_default_opts.argtypes = [C.POINTER(Opts)]
     L.storm_fvt_create.restype = C.c_int
     L.storm_fvt_create.argtypes = [C.POINTER(Mesh), C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.storm_fvt_create_batch.restype = C.c_int
+    L.storm_fvt_create_batch.argtypes = [C.c_int32, C.POINTER(Mesh), C.POINTER(Opts),
+                                         C.POINTER(C.c_void_p)]
+    L.storm_fvt_reach_status.argtypes = [C.c_void_p, _ip, _dp, _dp, C.POINTER(C.c_uint8)]
     L.storm_fvt_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.storm_fvt_set_bc_values.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_run.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
@@ -272,6 +277,56 @@ class FVTSolver:
             self.close()
         except Exception:
             pass
+
+
+class BatchSolver(FVTSolver):
+    """An ensemble of independent reaches advanced together in one context
+    (storm_fvt_create_batch).  Every reach gets the same options; state()
+    returns the reaches' arrays laid end to end, split() cuts them apart."""
+
+    def __init__(self, cases_, opts: Optional[Dict[str, object]] = None):
+        L = load()
+        o = dict(cases_[0].opts)
+        if opts:
+            o.update(opts)
+        self.opts = o
+        self._keep = []
+        meshes = (Mesh * len(cases_))()
+        for r, c in enumerate(cases_):
+            k = {}
+            meshes[r] = mesh_struct(c, k)
+            self._keep.append(k)
+        self._meshes = meshes
+        self._o = opts_struct(o)
+        h = C.c_void_p()
+        rc = L.storm_fvt_create_batch(len(cases_), meshes, C.byref(self._o), C.byref(h))
+        if rc != 0 or not h.value:
+            raise StormError(f"storm_fvt_create_batch failed ({rc})")
+        self.h = h
+        self.nr = len(cases_)
+        self.cell_ptr = np.cumsum([0] + [c.n_elems for c in cases_])
+        self.pts_ptr = np.cumsum([0] + [c.n_pts for c in cases_])
+        self.N, self.P = int(self.cell_ptr[-1]), int(self.pts_ptr[-1])
+        self.n_in = sum(len(c.inflows) for c in cases_)
+        self.n_out = sum(len(c.outflows) for c in cases_)
+
+    def status(self):
+        """Per reach: (iter, ttime, rmax, done) arrays."""
+        it = np.zeros(self.nr, dtype=np.int32)
+        tt, rm = np.zeros(self.nr), np.zeros(self.nr)
+        dn = np.zeros(self.nr, dtype=np.uint8)
+        self._chk(load().storm_fvt_reach_status(self.h, _p(it, C.c_int32), _p(tt, C.c_double),
+                                                _p(rm, C.c_double), _p(dn, C.c_uint8)))
+        return it, tt, rm, dn.astype(bool)
+
+    def split(self, st: Dict[str, np.ndarray]):
+        """Cut a state() dictionary into one dictionary per reach."""
+        out = []
+        for r in range(self.nr):
+            a, b = self.cell_ptr[r], self.cell_ptr[r + 1]
+            pa, pb = self.pts_ptr[r], self.pts_ptr[r + 1]
+            out.append({k: (v[pa:pb] if k in ("hvtx", "zetavtx") else v[a:b]) for k, v in st.items()})
+        return out
 
 
 def inspect_molecule(case):

@@ -1,0 +1,61 @@
+"""Batched ensembles: reaches of uneven size advanced together in one
+context give, reach by reach, exactly the results of separate runs; a
+converged steady reach stops while the others go on."""
+import numpy as np
+import pytest
+
+from storm_amd import cases, fvt
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("h", "qh", "qu", "qv", "u", "v", "hvtx", "zetavtx", "phi1", "phi2", "phi3", "dry")
+
+
+def _reaches(ks=range(5), steady=False):
+    # uneven sizes, different slopes, roughness, discharge and stage
+    return [cases.ensemble_reach(k, n_min=300, n_max=3000, steady=steady) for k in ks]
+
+
+def _solo(c, opts, n):
+    s = fvt.FVTSolver(c, opts)
+    s.run(n)
+    s.finalize()
+    st = s.state()
+    s.close()
+    return st
+
+
+def test_batch_equals_separate_runs():
+    rs = _reaches()
+    opts = dict(istop=40, dt_op=1, cfl=0.5, steady=False)
+    print(f"[gpu batch] {len(rs)} reaches, cells {[c.n_elems for c in rs]}", flush=True)
+    b = fvt.BatchSolver(rs, opts)
+    b.run(40)
+    b.finalize()
+    parts = b.split(b.state())
+    it, tt, rm, dn = b.status()
+    assert (it == 40).all()
+    for c, p in zip(rs, parts):
+        s = _solo(c, opts, 40)
+        for k in KEYS:
+            assert np.array_equal(p[k], s[k]), k
+    b.close()
+
+
+def test_batch_steady_reaches_retire_independently():
+    # reach 3 of this generator blows up in steady mode at CFL 0.3 (NaN
+    # residual, in the oracle too), so it is left out here
+    rs = _reaches(ks=(0, 1, 2, 4), steady=True)
+    opts = dict(istop=3000, steady=True, stop_cr=1, rstop=1e-4, dt_op=0, cfl=0.3)
+    print(f"[gpu batch steady] {len(rs)} reaches", flush=True)
+    b = fvt.BatchSolver(rs, opts)
+    b.run(3000)
+    it, tt, rm, dn = b.status()
+    assert dn.all()
+    for r, c in enumerate(rs):
+        s = fvt.FVTSolver(c, opts)
+        i1, t1, r1 = s.run(3000)
+        assert i1 == it[r] and t1 == tt[r] and r1 == rm[r]
+        s.close()
+    assert len(set(it.tolist())) > 1  # they converged at different steps
+    b.close()
