@@ -1415,8 +1415,14 @@ struct storm_fvt {
   // the water surface formed at the start of that stage (solverFVT.f90:169,483)
   const CellD* last_din = nullptr;
 
+  // two-step graph (opts.graph) and the buffer phase it was captured at
+  hipGraphExec_t gexec = nullptr;
+  const CellD* g_din = nullptr;
+  int g_flip = 0;
+
   ~storm_fvt() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (void* p : owned) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1634,13 +1640,16 @@ void launch_control(storm_fvt* C, int begin) {
                      C->d_qts_t, C->d_qts_v, C->d_hts_ptr, C->d_hts_t, C->d_hts_v);
 }
 
-void launch_step(storm_fvt* C) {
+// One time step: k_control, then per stage the vertex pass and the stage
+// kernel.  With launch == false only the host's view of the buffers advances
+// (after a graph replay of the same step).
+void launch_step(storm_fvt* C, bool launch = true) {
   Prep& P = C->prep;
   const int R = P.R;
   const int N = P.N;
   double a[3][3], b[3];
   rk_table(R, a, b);
-  launch_control(C, 1);
+  if (launch) launch_control(C, 1);
   for (int s = 1; s <= R; ++s) {
     Stage st{};
     for (int j = 0; j < s; ++j) { st.slot[j] = C->slot_ptr(j); st.alpha[j] = a[s - 1][j]; }
@@ -1649,15 +1658,17 @@ void launch_step(storm_fvt* C) {
     st.beta = b[s - 1];
     st.s = s;
     st.last = (s == R);
-    hipLaunchKernelGGL(k_vertex_stage, dim3((P.P + 255) / 256), dim3(256), 0, C->stream, C->g,
-                       C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
-    if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
-      hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
-                         C->S.din, C->S.gwork);
-    switch (C->cap) {
-      case 256: launch_stage<256>(C, st); break;
-      case 384: launch_stage<384>(C, st); break;
-      default: launch_stage<512>(C, st); break;
+    if (launch) {
+      hipLaunchKernelGGL(k_vertex_stage, dim3((P.P + 255) / 256), dim3(256), 0, C->stream, C->g,
+                         C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
+      if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
+        hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
+                           C->S.din, C->S.gwork);
+      switch (C->cap) {
+        case 256: launch_stage<256>(C, st); break;
+        case 384: launch_stage<384>(C, st); break;
+        default: launch_stage<512>(C, st); break;
+      }
     }
     C->last_din = C->S.din;
     CellD* next = C->S.dout;
@@ -1665,6 +1676,31 @@ void launch_step(storm_fvt* C) {
     C->S.din = next;
   }
   if (R == 1) C->r1_flip ^= 1;
+}
+
+// Two time steps captured as one HIP graph.  Two steps bring the buffer
+// roles (stage state double buffer, RK1 slot flip) back to where they were,
+// so the graph replays from any even step after the captured one.  No
+// memset node is recorded: the step launches kernels only.
+void ensure_graph(storm_fvt* C) {
+  if (C->gexec) return;
+  const CellD* din = C->S.din;
+  CellD* dout = C->S.dout;
+  const CellD* last = C->last_din;
+  const int flip = C->r1_flip;
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
+  launch_step(C);
+  launch_step(C);
+  const hipError_t e = hipStreamEndCapture(C->stream, &graph);
+  // the capture ran nothing: restore the host's view of the buffers
+  C->S.din = din; C->S.dout = dout; C->last_din = last; C->r1_flip = flip;
+  HIPCHK(e);
+  const hipError_t ei = hipGraphInstantiate(&C->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(ei);
+  C->g_din = din;
+  C->g_flip = flip;
 }
 
 int set_err(storm_fvt* C, int code, const std::string& msg) {
@@ -1863,6 +1899,11 @@ int storm_fvt_set_bc_values(storm_fvt_t* C, const double* qin, const double* hbc
       P.hts_v.assign(hbc, hbc + P.n_out);
     }
     upload_tables(C);
+    if (C->gexec) {  // the graph holds the old tables' addresses
+      HIPCHK(hipStreamSynchronize(C->stream));
+      (void)hipGraphExecDestroy(C->gexec);
+      C->gexec = nullptr;
+    }
     C->started = false;
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
@@ -1878,7 +1919,20 @@ int storm_fvt_run(storm_fvt_t* C, int32_t nsteps, int32_t* iter, double* ttime, 
     const int room = std::max(0, C->prep.o.istop - C->iter_host);
     const int n = std::min(nsteps, room);
     HIPCHK(hipEventRecord(C->ev0, C->stream));
-    for (int k = 0; k < n; ++k) launch_step(C);
+    int k = 0;
+    if (C->prep.o.graph && n >= 2) {
+      ensure_graph(C);
+      if (C->S.din != C->g_din || C->r1_flip != C->g_flip) {  // one step to the graph's phase
+        launch_step(C);
+        ++k;
+      }
+      for (; k + 2 <= n; k += 2) {
+        HIPCHK(hipGraphLaunch(C->gexec, C->stream));
+        launch_step(C, false);
+        launch_step(C, false);
+      }
+    }
+    for (; k < n; ++k) launch_step(C);
     if (n > 0) launch_control(C, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(C->ev1, C->stream));
