@@ -11,6 +11,12 @@ wet/dry fronts.  A step is one full SSP-RK3 time step of the whole reach
 resident in HBM when timing starts.
 
 Prints one JSON line.  `value` is cell-updates/s (triangles x steps / s).
+
+`--workload ensemble` instead runs the batched ensemble (storm_amd/cases.py
+`ensemble_reach`: reaches of log-uniform size in [5k, 200k] triangles, own
+slope, roughness, discharge and stage) from a cold start to steady state
+(RMS dh <= 1e-6 in every reach) in one context, with the autotuned launch
+configuration, and reports the wall time to steady and the cell-updates/s.
 `roofline` prices the step with the algorithmic bytes of one cell-update of
 SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
 residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
@@ -22,6 +28,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -67,6 +75,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
+    ap.add_argument("--workload", choices=["meander", "ensemble"], default="meander")
+    ap.add_argument("--reaches", type=int, default=256, help="ensemble: number of reaches")
+    ap.add_argument("--max-steps", type=int, default=60000, help="ensemble: step limit")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
@@ -78,6 +89,9 @@ def main():
         sys.exit(2)
 
     from storm_amd import cases, fvt
+    if args.workload == "ensemble":
+        ensemble(args)
+        return
     if args.calib_copy > 0:
         import torch
         nbytes = args.calib_copy << 20
@@ -130,6 +144,49 @@ def main():
                                "kind": "port",
                                "sample": "%d SSP-RK3 steps of the same 2M-triangle reach "
                                          "(%.1f s), serial C oracle" % (args.cpu_steps, cpu_t)}
+    print(json.dumps(out))
+
+
+def ensemble(args):
+    """Cold-start time to steady state of a batched ensemble."""
+    from storm_amd import cases, fvt, tune
+    t0 = time.perf_counter()
+    reaches = [cases.ensemble_reach(k, steady=True) for k in range(args.reaches)]
+    gen_s = time.perf_counter() - t0
+    # first-order HLL: the Barth-Jespersen runs of this generator stall at an
+    # RMS dh of about 1e-5, above the 1e-6 target
+    opts = dict(flux="hll", limiter=0, rkorder=2, cfl=0.9, steady=True, stop_cr=1, rstop=1e-6,
+                dt_op=0, istop=args.max_steps)
+    best, times = tune.tune(reaches, opts, steps=4)
+    opts.update(best)
+    t0 = time.perf_counter()
+    b = fvt.BatchSolver(reaches, opts)
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gpu_s = 0.0
+    while True:
+        b.run(1000)
+        gpu_s += b.last_run_seconds()
+        it, tt, rm, dn = b.status()
+        print("ensemble: %d/%d reaches converged, max step %d" % (dn.sum(), len(dn), it.max()),
+              file=sys.stderr, flush=True)
+        if dn.all() or it.max() >= args.max_steps:
+            break
+    wall = time.perf_counter() - t0
+    n = np.array([c.n_elems for c in reaches])
+    updates = float((n * it).sum())
+    out = {
+        "metric": "time-to-steady (cold start), batched ensemble",
+        "value": wall, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "cell_updates_per_s": updates / wall, "gpu_seconds": gpu_s,
+        "reaches": len(reaches), "triangles": int(n.sum()), "converged": int(dn.sum()),
+        "steps_max": int(it.max()), "steps_median": float(np.median(it)),
+        "dtype": "f64", "data": "synthetic (storm_amd/cases.py ensemble_reach, seeds 4004+k)",
+        "config": {"workload": "ensemble-%d" % len(reaches), "scheme": "HLL first order, SSP-RK2, "
+                   "steady local dt CFL 0.9, stop RMS dh <= 1e-6", "launch": best,
+                   "tune_s_per_step": times, "generate_s": round(gen_s, 1),
+                   "setup_s": round(setup_s, 1)},
+    }
     print(json.dumps(out))
 
 
