@@ -1,0 +1,55 @@
+"""The BF16 MFMA network on the GPU against its FP64 restatement: forward
+pass, every parameter gradient, determinism, and a loss that falls under
+AdamW."""
+import numpy as np
+import pytest
+
+from mpnn_ref import Ref, toy_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_and_gradients_match_fp64_reference():
+    from storm_amd import mpnn
+    print("[gpu mpnn] gradients vs FP64", flush=True)
+    src, dst, X, EF, T = toy_graph(n=300, seed=2)
+    net = mpnn.Net(layers=3, seed=7)
+    ref = Ref(layers=3)
+    p = net.params()
+    net.set_graph(src, dst, X, EF)
+    y = net.forward()
+    yr, _ = ref.forward(p, src, dst, X, EF)
+    assert _rel(y, yr) < 2e-2
+    loss = net.backward(T)
+    assert abs(loss - ref.loss(p, src, dst, X, EF, T)) <= 2e-2 * abs(loss)
+    g = net.grads()
+    gr = ref.grads(p, src, dst, X, EF, T)
+    for name, (o, n) in ref.off.items():
+        err = _rel(g[o:o + n], gr[o:o + n])
+        assert err < 5e-2, (name, err)
+    # bit-repeatable
+    net.forward()
+    net.backward(T)
+    assert np.array_equal(net.grads(), g)
+    net.close()
+
+
+def test_adamw_lowers_the_loss():
+    from storm_amd import mpnn
+    print("[gpu mpnn] training", flush=True)
+    src, dst, X, EF, T = toy_graph(n=200, seed=4)
+    T = np.tanh(X[:, :16] * 0.5).astype(np.float32)  # a learnable target
+    net = mpnn.Net(layers=2, lr=3e-3, seed=1)
+    net.set_graph(src, dst, X, EF)
+    first = None
+    for it in range(150):
+        net.forward()
+        loss = net.backward(T)
+        first = loss if first is None else first
+        net.step()
+    assert loss < 0.2 * first, (first, loss)
+    net.close()
