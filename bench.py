@@ -75,9 +75,13 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
-    ap.add_argument("--workload", choices=["meander", "ensemble"], default="meander")
-    ap.add_argument("--reaches", type=int, default=256, help="ensemble: number of reaches")
-    ap.add_argument("--max-steps", type=int, default=60000, help="ensemble: step limit")
+    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart"], default="meander")
+    ap.add_argument("--reaches", type=int, default=256,
+                    help="ensemble: number of reaches; warmstart: held-out reaches (default 64)")
+    ap.add_argument("--max-steps", type=int, default=60000, help="ensemble/warmstart: step limit")
+    ap.add_argument("--size", type=int, default=200000, help="warmstart: triangles per reach")
+    ap.add_argument("--rstop", type=float, default=1e-6, help="warmstart: RMS dh tolerance")
+    ap.add_argument("--checkpoint", default="", help="warmstart: trained network (.npz)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
@@ -91,6 +95,9 @@ def main():
     from storm_amd import cases, fvt
     if args.workload == "ensemble":
         ensemble(args)
+        return
+    if args.workload == "warmstart":
+        warmstart(args)
         return
     if args.calib_copy > 0:
         import torch
@@ -186,6 +193,73 @@ def ensemble(args):
                    "steady local dt CFL 0.9, stop RMS dh <= 1e-6", "launch": best,
                    "tune_s_per_step": times, "generate_s": round(gen_s, 1),
                    "setup_s": round(setup_s, 1)},
+    }
+    print(json.dumps(out))
+
+
+def warmstart(args):
+    """Wall time and steps to steady state (RMS dh <= rstop) of held-out
+    reaches from three starts, each run as one batched context: cold (the
+    generator's flat water surface at the outlet stage), the plane guess
+    (warmstart.baseline_state, no network), and the learned warm start
+    (features + network inference + upload inside the timed region)."""
+    from storm_amd import warmstart as W
+    n_r = args.reaches if args.reaches != 256 else 64
+    ckpt = args.checkpoint or W.DEFAULT_CHECKPOINT
+    t0 = time.perf_counter()
+    reaches = [W.heldout_reach(k, args.size) for k in range(n_r)]
+    gen_s = time.perf_counter() - t0
+    opts = dict(rstop=args.rstop, nonfinite_stop=True)
+    res, finals = {}, {}
+    for mode in ("cold", "plane", "learned"):
+        t0 = time.perf_counter()
+        init, pred_s = None, 0.0
+        if mode == "plane":
+            init = [W.baseline_state(c) for c in reaches]
+        elif mode == "learned":
+            ws = W.WarmStart.load(ckpt)
+            init = ws.predict(reaches)
+            ws.close()
+        pred_s = time.perf_counter() - t0
+        out, _ = W.steady_states(reaches, opts, args.max_steps, chunk=1000, init=init,
+                                 log=lambda m: print("%s: %s" % (mode, m), file=sys.stderr,
+                                                     flush=True))
+        wall = time.perf_counter() - t0
+        steps = np.array([it for _, it in out])
+        ok = np.array([st is not None for st, _ in out])
+        finals[mode] = out
+        res[mode] = dict(wall_s=wall, init_s=pred_s, converged=int(ok.sum()),
+                         steps_median=float(np.median(steps)), steps_max=int(steps.max()),
+                         steps_sum=int(steps.sum()),
+                         cell_updates=float(sum(c.n_elems * it for c, it in zip(reaches, steps))))
+        if mode == "learned":
+            # prediction error against the state the solver converged to
+            eh, eq = [], []
+            for (st, _), (h, u, v) in zip(out, init):
+                if st is None:
+                    continue
+                eh.append(np.sqrt(np.mean((h - st["h"]) ** 2)) / np.mean(st["h"]))
+                eq.append(np.sqrt(np.mean((h * u - st["qu"]) ** 2)) / np.mean(np.abs(st["qu"])))
+            res[mode]["pred_err_h_rms_rel"] = float(np.median(eh)) if eh else None
+            res[mode]["pred_err_hu_rms_rel"] = float(np.median(eq)) if eq else None
+        print("%s: %s" % (mode, json.dumps(res[mode])), file=sys.stderr, flush=True)
+    # how far apart the converged states of the warm and cold starts are
+    dd = []
+    for (a, _), (b, _) in zip(finals["learned"], finals["cold"]):
+        if a is not None and b is not None:
+            dd.append(np.sqrt(np.mean((a["h"] - b["h"]) ** 2)))
+    ratio = res["cold"]["wall_s"] / res["learned"]["wall_s"]
+    out = {
+        "metric": "time-to-steady, cold / learned warm start (wall-time ratio)",
+        "value": ratio, "unit": "x", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64 solver, bf16 network (fp32 accumulate)",
+        "data": "synthetic (storm_amd/warmstart.py heldout_reach, seeds 9009+k)",
+        "config": {"workload": "warmstart-%dx%d" % (n_r, args.size), "rstop": args.rstop,
+                   "scheme": "HLL first order, SSP-RK2, steady local dt CFL 0.9, stop RMS dh",
+                   "checkpoint": os.path.relpath(ckpt, ROOT), "generate_s": round(gen_s, 1)},
+        "cold": res["cold"], "plane": res["plane"], "learned": res["learned"],
+        "plane_over_cold": res["cold"]["wall_s"] / res["plane"]["wall_s"],
+        "h_rms_learned_vs_cold_final_m": float(np.median(dd)) if dd else None,
     }
     print(json.dumps(out))
 

@@ -79,7 +79,11 @@ typedef struct storm_fvt_opts {
   int32_t dt_op;     /* DELT TYPE: 0 fixed dt = cfl; 1 CFL reduction (the
                         corrected deltatFVT: cell area and perimeter) */
   int32_t steady;    /* TIME STEPPING 0: steady, local time steps */
-  int32_t stop_cr;   /* STOPPING CRIT: 0 max|phi|, 1 RMS change of h */
+  int32_t stop_cr;   /* STOPPING CRIT: 0 max|phi|, 1 RMS change of h (solverFVT.f90:407-431);
+                        2 the larger of the RMS change of h and that of the
+                        discharge (hu, hv), each relative as in 1 (no reference
+                        counterpart: 1 alone stops runs whose discharge is
+                        still far from steady) */
   int32_t istop;     /* MAX ITERATION */
   int32_t btype;     /* WALLS: 0 no-slip (wenslp3), 1 slip (weslp2) */
   int32_t hvalve;    /* HVALVE */
@@ -93,7 +97,9 @@ typedef struct storm_fvt_opts {
   int32_t reorder;   /* 1 = Hilbert-curve cell order (default), 0 = input order */
   int32_t block;     /* stage-kernel tile capacity (threads per block): 256, 384, 512; 0 = default */
   int32_t graph;     /* 1 = replay captured HIP graphs of whole steps */
-  int32_t reserved;
+  int32_t nonfinite_stop; /* 1 = a steady run (or reach of a batch) whose residual is
+                             not finite stops there, with rmax NaN; 0 = it runs on to
+                             MAX ITERATION, as the reference does */
 } storm_fvt_opts_t;
 
 typedef struct storm_fvt storm_fvt_t;

@@ -17,6 +17,9 @@
  * Loss: mean over nodes and outputs of w_i (y - t)^2.  Training: AdamW.
  * Every sum is formed in a fixed order: results are bit-repeatable.
  *
+ * Initial weights: He-uniform (seeded), zero biases, and a zero decoder,
+ * so an untrained network outputs zero.
+ *
  * Parameters are FP32 master weights, laid out in this order: W_enc
  * [f_node][H], b_enc[H]; per layer W_e [2H + f_edge][H], b_e[H], W_n [2H][H],
  * b_n[H]; W_dec [H][f_out], b_dec[f_out] (row-major [in][out]).  f_node,
@@ -57,6 +60,8 @@ int storm_mpnn_backward(storm_mpnn_t *net, const float *target, const float *w, 
 int storm_mpnn_get_grads(storm_mpnn_t *net, float *g);
 /* One AdamW update with the gradients of the last backward pass. */
 int storm_mpnn_adamw_step(storm_mpnn_t *net);
+/* Learning rate of the following AdamW updates (a schedule's current value). */
+int storm_mpnn_set_lr(storm_mpnn_t *net, float lr);
 const char *storm_mpnn_last_error(storm_mpnn_t *net);
 
 #ifdef __cplusplus

@@ -1487,11 +1487,21 @@ static int step(orc_t *s) {
   }
   find_dry_cells(s, 0);
   if (s->o.friction == 0) fric_terms_impl(s, s->delt);
-  double sumtop = ZERO, sumbot = ZERO;
+  double sumtop = ZERO, sumbot = ZERO, sumtopq = ZERO, sumbotq = ZERO;
   for (int i = 0; i < N; ++i) {
     double d = s->h[i] - RK(s, 0, 0, i);
     sumtop = sumtop + d * d;
     sumbot = sumbot + RK(s, 0, 0, i) * RK(s, 0, 0, i);
+  }
+  if (s->o.stop_cr == 2) {
+    /* stopping test 2 (no reference counterpart): the relative RMS change
+       of the discharge (hu, hv) after friction, beside that of h */
+    for (int i = 0; i < N; ++i) {
+      double qu0 = RK(s, 1, 0, i), qv0 = RK(s, 2, 0, i);
+      double du = RK(s, 1, R, i) - qu0, dv = RK(s, 2, R, i) - qv0;
+      sumtopq = sumtopq + (du * du + dv * dv);
+      sumbotq = sumbotq + (qu0 * qu0 + qv0 * qv0);
+    }
   }
   if (s->o.hvalve) {  /* solverFVT.f90:357-370 */
     for (int j = 0; j < s->nbc; ++j) {
@@ -1516,6 +1526,10 @@ static int step(orc_t *s) {
     } else {
       if (orc_equals(sumbot, ZERO)) sumbot = ONE;
       s->rmax = sqrt(sumtop / sumbot);
+      if (s->o.stop_cr == 2) {
+        if (orc_equals(sumbotq, ZERO)) sumbotq = ONE;
+        s->rmax = fmax2(s->rmax, sqrt(sumtopq / sumbotq));
+      }
     }
     if (s->rmax < s->o.rstop) stop = 1;
     s->stime = (double)s->iter;

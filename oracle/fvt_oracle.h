@@ -45,7 +45,7 @@ typedef struct orc_opts {
   int32_t rkorder;     /* 1, 2, 3 (RK ORDER) */
   int32_t dt_op;       /* 0 = fixed dt = cfl, 1 = CFL reduction (corrected deltatFVT) */
   int32_t steady;      /* 1 = steady local time stepping */
-  int32_t stop_cr;     /* 0 = max|phi|, 1 = RMS dh */
+  int32_t stop_cr;     /* 0 = max|phi|, 1 = RMS dh, 2 = max(RMS dh, RMS dq) */
   int32_t istop;       /* MAX ITERATION */
   int32_t btype;       /* 0 = no-slip walls (wenslp3), 1 = slip (weslp2) */
   int32_t hvalve;

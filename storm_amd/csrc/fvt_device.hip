@@ -51,6 +51,11 @@ __host__ __device__ constexpr int pow2_floor(int n) {
   return p;
 }
 
+// Per-tile partials of the last stage, PART doubles per tile: sum (h - h^n)^2,
+// sum (h^n)^2, max|phi|, the next step's CFL candidate, and (stopping test 2
+// only) sum |q - q^n|^2 and sum |q^n|^2 of the discharge q = (hu, hv).
+constexpr int PART = 6;
+
 // Step control of one reach, kept on the device (k_control writes it).
 struct Ctrl {
   double dt;          // dt of the current step
@@ -93,6 +98,7 @@ struct Geo {
 
 struct Opts {
   int rkorder, dt_op, steady, stop_cr, btype, hvalve, limiter, flux, gradient, friction, vclip;
+  int nonfinite_stop;
   double cfl, rstop, h_dry, h_wet, vceiling;
 };
 
@@ -262,10 +268,12 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
     __syncthreads();
   }
   if (t == 0) {
-    part[4 * b] = ZERO;
-    part[4 * b + 1] = ZERO;
-    part[4 * b + 2] = ZERO;
-    part[4 * b + 3] = s_c[0];
+    part[PART * b] = ZERO;
+    part[PART * b + 1] = ZERO;
+    part[PART * b + 2] = ZERO;
+    part[PART * b + 3] = s_c[0];
+    part[PART * b + 4] = ZERO;
+    part[PART * b + 5] = ZERO;
   }
 }
 
@@ -318,19 +326,22 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
   const RInfo ri = g.reach[blockIdx.x];
   Ctrl* c = &ctrl[blockIdx.x];
   __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS],
-      s_dt[CTRL_THREADS];
+      s_dt[CTRL_THREADS], s_topq[CTRL_THREADS], s_botq[CTRL_THREADS];
   const int t = threadIdx.x;
   if (c->done) return;
   // fixed-order reduction of the per-block partials of the last stage
   // (residual sums, max|phi|) and of the next step's CFL candidates
-  double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX;
+  double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX, topq = ZERO, botq = ZERO;
   for (int b = ri.tile0 + t; b < ri.tile1; b += CTRL_THREADS) {
-    top = top + part[4 * b];
-    bot = bot + part[4 * b + 1];
-    mx = dmax(mx, part[4 * b + 2]);
-    dtm = dmin(dtm, part[4 * b + 3]);
+    top = top + part[PART * b];
+    bot = bot + part[PART * b + 1];
+    mx = dmax(mx, part[PART * b + 2]);
+    dtm = dmin(dtm, part[PART * b + 3]);
+    topq = topq + part[PART * b + 4];
+    botq = botq + part[PART * b + 5];
   }
   s_top[t] = top; s_bot[t] = bot; s_max[t] = mx; s_dt[t] = dtm;
+  s_topq[t] = topq; s_botq[t] = botq;
   __syncthreads();
   for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
     if (t < w) {
@@ -338,6 +349,8 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
       s_bot[t] = s_bot[t] + s_bot[t + w];
       s_max[t] = dmax(s_max[t], s_max[t + w]);
       s_dt[t] = dmin(s_dt[t], s_dt[t + w]);
+      s_topq[t] = s_topq[t] + s_topq[t + w];
+      s_botq[t] = s_botq[t] + s_botq[t + w];
     }
     __syncthreads();
   }
@@ -351,10 +364,22 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
       else {
         if (equals(sumbot, ZERO)) sumbot = ONE;
         c->rmax = sqrt(sumtop / sumbot);
+        if (o.stop_cr == 2) {
+          // the same relative RMS change of the discharge (no reference
+          // counterpart: stopping test 1 looks at h alone)
+          double sumbotq = s_botq[0];
+          if (equals(sumbotq, ZERO)) sumbotq = ONE;
+          c->rmax = dmax(c->rmax, sqrt(s_topq[0] / sumbotq));
+        }
       }
       c->stime = (double)c->iter;
       c->pending = 0;
       if (c->rmax < o.rstop) { c->done = 1; return; }
+      if (o.nonfinite_stop && !(c->rmax <= DBL_MAX)) {  // NaN or infinite: cannot converge
+        c->rmax = __builtin_nan("");
+        c->done = 1;
+        return;
+      }
     } else {
       c->stime = c->stime + c->dt;
       c->rmax = s_max[0];
@@ -1118,7 +1143,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   __syncthreads();
 
   // 4. update of the tile cells
-  double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX;
+  double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX, topq = ZERO, botq = ZERO;
   if (t < T) {
     const int i = c0 + t;
     const uint64_t meta = g.cmeta[i];
@@ -1264,6 +1289,12 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
       const double dh = hn - h0n;
       top = dh * dh;
       bot = h0n * h0n;
+      if (o.stop_cr == 2) {  // discharge terms of stopping test 2
+        const double qu0 = st.slot[0][N + i], qv0 = st.slot[0][2 * N + i];
+        const double du = qu - qu0, dv = qv - qv0;
+        topq = du * du + dv * dv;
+        botq = qu0 * qu0 + qv0 * qv0;
+      }
       // hvalve (solverFVT.f90:357-370): outward normal of the cell's side
       if (o.hvalve) {
         if ((slot_bits(meta, 0) >> CMS_HBAR) & 1) {
@@ -1301,7 +1332,11 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
   double* r0 = L.fh;
   double* r1 = L.fh + CAP;
   double* r2 = L.fh + 2 * CAP;
+  double* r3 = L.fh + 3 * CAP;
+  double* r4 = L.fh + 4 * CAP;
+  const bool q2 = o.stop_cr == 2;
   r0[t] = top; r1[t] = bot; r2[t] = mxphi;
+  if (q2) { r3[t] = topq; r4[t] = botq; }
   __syncthreads();
   constexpr int P2 = pow2_floor(CAP);
   if (P2 < CAP) {
@@ -1309,6 +1344,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
       r0[t] = r0[t] + r0[t + P2];
       r1[t] = r1[t] + r1[t + P2];
       r2[t] = dmax(r2[t], r2[t + P2]);
+      if (q2) { r3[t] = r3[t] + r3[t + P2]; r4[t] = r4[t] + r4[t + P2]; }
     }
     __syncthreads();
   }
@@ -1317,13 +1353,16 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
       r0[t] = r0[t] + r0[t + w];
       r1[t] = r1[t] + r1[t + w];
       r2[t] = dmax(r2[t], r2[t + w]);
+      if (q2) { r3[t] = r3[t] + r3[t + w]; r4[t] = r4[t] + r4[t + w]; }
     }
     __syncthreads();
   }
   if (t == 0) {
-    S.part[4 * b] = r0[0];
-    S.part[4 * b + 1] = r1[0];
-    S.part[4 * b + 2] = r2[0];
+    S.part[PART * b] = r0[0];
+    S.part[PART * b + 1] = r1[0];
+    S.part[PART * b + 2] = r2[0];
+    S.part[PART * b + 4] = q2 ? r3[0] : ZERO;
+    S.part[PART * b + 5] = q2 ? r4[0] : ZERO;
   }
   __syncthreads();
   r0[t] = cand;
@@ -1336,7 +1375,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
     if (t < w) r0[t] = dmin(r0[t], r0[t + w]);
     __syncthreads();
   }
-  if (t == 0) S.part[4 * b + 3] = r0[0];
+  if (t == 0) S.part[PART * b + 3] = r0[0];
 }
 
 // to_vertices' cell pass alone (solverFVT.f90:483 -> to_vertices.f90:44-55),
@@ -1530,6 +1569,7 @@ void setup_device(storm_fvt* C) {
   o.rkorder = q.rkorder; o.dt_op = q.dt_op; o.steady = q.steady; o.stop_cr = q.stop_cr;
   o.btype = q.btype; o.hvalve = q.hvalve; o.limiter = q.limiter; o.flux = q.flux;
   o.gradient = q.gradient; o.friction = q.friction; o.vclip = q.vclip;
+  o.nonfinite_stop = q.nonfinite_stop;
   o.cfl = q.cfl; o.rstop = q.rstop; o.h_dry = q.h_dry; o.h_wet = q.h_wet; o.vceiling = q.vceiling;
   const size_t N = P.N;
   const int R = P.R;
@@ -1570,7 +1610,7 @@ void setup_device(storm_fvt* C) {
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
   S.zeta = dalloc<double>(own, N);
-  S.part = dalloc<double>(own, 4 * (size_t)P.nt);
+  S.part = dalloc<double>(own, PART * (size_t)P.nt);
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
   S.ctrl = dalloc<Ctrl>(own, P.nr);

@@ -199,7 +199,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
   if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
   if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 1 || o->limiter < 0 ||
       o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
-      o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 1) {
+      o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2) {
     S.err = "invalid option value"; return STORM_E_ARG;
   }
   if (!(o->h_dry > 0) || !(o->h_wet >= 0) || !(o->cfl > 0)) { S.err = "invalid threshold or cfl"; return STORM_E_ARG; }

@@ -317,7 +317,7 @@ struct storm_mpnn {
   DBuf<int> src, dst, in_ptr, in_idx, out_ptr, out_idx;
   DBuf<float> xin, ef;
   std::vector<DBuf<float>> H, Xe, Mg, Xn, U;
-  DBuf<float> Y, dY, dH, dH2, dXn, dM, dXe, dZ0, part;
+  DBuf<float> Y, dY, dH, dH2, dXn, dM, dXe, dZ0, part, tgt, wgt;
   DBuf<double> lossp;
   bool have_fwd = false;
 
@@ -446,7 +446,7 @@ int storm_mpnn_create(const storm_mpnn_cfg_t* cfg, storm_mpnn_t** out) {
     S->o_dec_w = o; o += (int64_t)HID * cfg->f_out;
     S->o_dec_b = o; o += cfg->f_out;
     S->np = o;
-    // He-uniform weights, zero biases
+    // He-uniform weights, zero biases, zero decoder
     std::vector<float> h(o, 0.f);
     uint64_t st = cfg->seed;
     auto fill = [&](int64_t off, int fan_in, int fan_out) {
@@ -461,7 +461,7 @@ int storm_mpnn_create(const storm_mpnn_cfg_t* cfg, storm_mpnn_t** out) {
       fill(S->o_ew[l], Ke, HID);
       fill(S->o_nw[l], 2 * HID, HID);
     }
-    fill(S->o_dec_w, HID, cfg->f_out);
+    // the decoder starts at zero: an untrained network outputs zero
     S->p.need(o); S->g.need(o); S->m.need(o); S->v.need(o);
     HIPCHK(hipMemcpy(S->p.p, h.data(), sizeof(float) * o, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(S->g.p, 0, sizeof(float) * o));
@@ -572,7 +572,8 @@ int storm_mpnn_backward(storm_mpnn_t* S, const float* target, const float* w, do
   if (!S || !target || !S->have_fwd) return fail(S, "backward needs a forward pass and a target");
   try {
     const int N = S->N, F = S->cfg.f_out;
-    DBuf<float> T, Wt;
+    DBuf<float>& T = S->tgt;
+    DBuf<float>& Wt = S->wgt;
     T.need((size_t)N * F);
     HIPCHK(hipMemcpyAsync(T.p, target, sizeof(float) * N * F, hipMemcpyHostToDevice, S->stream));
     if (w) {
@@ -622,6 +623,12 @@ int storm_mpnn_adamw_step(storm_mpnn_t* S) {
   } catch (const Err& e) {
     return fail(S, e.msg);
   }
+  return 0;
+}
+
+int storm_mpnn_set_lr(storm_mpnn_t* S, float lr) {
+  if (!S || !(lr >= 0.f)) return -1;
+  S->cfg.lr = lr;
   return 0;
 }
 

@@ -56,7 +56,7 @@ class Opts(C.Structure):
                 ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
                 ("h_wet", C.c_double), ("vceiling", C.c_double),
                 ("reorder", C.c_int32), ("block", C.c_int32), ("graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("nonfinite_stop", C.c_int32)]
 
 
 EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch",
@@ -179,6 +179,7 @@ def opts_struct(o: Dict[str, object]) -> Opts:
     s.reorder = int(o.get("reorder", 1))
     s.block = int(o.get("block", 0))
     s.graph = int(o.get("graph", 0))
+    s.nonfinite_stop = int(bool(o.get("nonfinite_stop", False)))
     return s
 
 

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+from typing import Optional
 
 import numpy as np
 
@@ -14,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libstorm_mpnn.so")
 EXPORTS = ["storm_mpnn_create", "storm_mpnn_destroy", "storm_mpnn_num_params",
            "storm_mpnn_get_params", "storm_mpnn_set_params", "storm_mpnn_set_graph",
            "storm_mpnn_forward", "storm_mpnn_backward", "storm_mpnn_get_grads",
-           "storm_mpnn_adamw_step", "storm_mpnn_last_error"]
+           "storm_mpnn_adamw_step", "storm_mpnn_set_lr", "storm_mpnn_last_error"]
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -52,6 +53,7 @@ def load(path: str = LIB_PATH):
     L.storm_mpnn_backward.argtypes = [C.c_void_p, _fp, _fp, C.POINTER(C.c_double)]
     L.storm_mpnn_get_grads.argtypes = [C.c_void_p, _fp]
     L.storm_mpnn_adamw_step.argtypes = [C.c_void_p]
+    L.storm_mpnn_set_lr.argtypes = [C.c_void_p, C.c_float]
     L.storm_mpnn_last_error.restype = C.c_char_p
     L.storm_mpnn_last_error.argtypes = [C.c_void_p]
     _lib = L
@@ -91,7 +93,12 @@ class Net:
                                               _p(d, C.c_int32), _p(nf, C.c_float),
                                               _p(ef, C.c_float)))
 
-    def forward(self) -> np.ndarray:
+    def forward(self, copy: bool = True) -> Optional[np.ndarray]:
+        """Forward pass; returns y [n][f_out] (None with copy=False, which
+        leaves the outputs on the device for backward)."""
+        if not copy:
+            self._chk(load().storm_mpnn_forward(self.h, None))
+            return None
         y = np.zeros((self.n, self.cfg.f_out), dtype=np.float32)
         self._chk(load().storm_mpnn_forward(self.h, _p(y, C.c_float)))
         return y
@@ -107,6 +114,9 @@ class Net:
 
     def step(self):
         self._chk(load().storm_mpnn_adamw_step(self.h))
+
+    def set_lr(self, lr: float):
+        self._chk(load().storm_mpnn_set_lr(self.h, float(lr)))
 
     def params(self) -> np.ndarray:
         p = np.zeros(self.n_params, dtype=np.float32)

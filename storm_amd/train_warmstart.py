@@ -1,0 +1,106 @@
+"""Train the warm-start network on steady states the batched FP64 solver
+generates.
+
+    python -m storm_amd.train_warmstart --reaches 512 --out <path.npz>
+
+1. Training reaches k = 0..R-1 come from the ensemble generator at 20k-100k
+   triangles (seeds 5005+k).  The batched solver runs them together, from
+   the plane guess of storm_amd.warmstart.baseline_state, to RMS dh <= rstop
+   (default 1e-8: at 1e-6 the RMS-change test still leaves centimetre errors
+   in h).  A reach that diverges or does not converge is left out.
+2. Features and targets (storm_amd.warmstart.frame / targets); a held-out
+   validation share.
+3. AdamW on mini-batches of whole reaches, all on the GPU in the network's
+   own forward, backward and AdamW kernels.
+4. The checkpoint (FP32 parameters + a JSON record of the data and the loss
+   history) is written with WarmStart.save.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import warmstart as W
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int):
+    """Steady states of training reaches 0..n-1 (None where not converged)."""
+    states, steps, walls = [], [], []
+    for s in range(0, n_reaches, gen_batch):
+        ks = list(range(s, min(n_reaches, s + gen_batch)))
+        rs = [W.training_reach(k) for k in ks]
+        init = [W.baseline_state(c) for c in rs]
+        out, wall = W.steady_states(rs, dict(rstop=rstop, nonfinite_stop=True), max_steps,
+                                    chunk=5000, init=init, log=log)
+        walls.append(wall)
+        for (st, it) in out:
+            states.append(st)
+            steps.append(it)
+        log("generated reaches %d-%d: %d converged, %.1f s" %
+            (ks[0], ks[-1], sum(o[0] is not None for o in out), wall))
+    return states, steps, walls
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reaches", type=int, default=512)
+    ap.add_argument("--gen-batch", type=int, default=128)
+    ap.add_argument("--rstop", type=float, default=1e-8)
+    ap.add_argument("--max-steps", type=int, default=150000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=4, help="reaches per mini-batch")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-5, help="end of the cosine decay")
+    ap.add_argument("--val", type=float, default=0.05, help="validation share")
+    ap.add_argument("--time-limit", type=float, default=0.0, help="training seconds (0: none)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+
+    t0 = time.perf_counter()
+    states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps)
+    gen_s = time.perf_counter() - t0
+    ks = [k for k, st in enumerate(states) if st is not None]
+    frames, ys = [], []
+    for k in ks:
+        c = W.training_reach(k)
+        fr = W.frame(c)
+        st = states[k]
+        frames.append(fr)
+        ys.append(W.targets(fr, st["qh"], st["qu"], st["qv"]))
+    states = None
+    n_val = max(1, int(round(a.val * len(ks))))
+    tr_f, tr_y = frames[:-n_val], ys[:-n_val]
+    va_f, va_y = frames[-n_val:], ys[-n_val:]
+    log("training on %d reaches, validating on %d" % (len(tr_f), len(va_f)))
+    ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed)
+    val0 = ws.loss(va_f, va_y, a.batch)
+    t1 = time.perf_counter()
+    ws.train(tr_f, tr_y, a.epochs, batch=a.batch, seed=a.seed, log=log,
+             time_limit=a.time_limit or None, lr=a.lr, lr_final=a.lr_final)
+    train_s = time.perf_counter() - t1
+    val1 = ws.loss(va_f, va_y, a.batch)
+    zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
+    meta = dict(
+        reaches=a.reaches, converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
+        train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
+        steps_median=float(np.median([steps[k] for k in ks])),
+        generate_s=gen_s, train_s=train_s, epochs_run=len(ws.history), batch=a.batch,
+        lr=a.lr, lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1, val_loss_of_zero_output=zero,
+        generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
+    ws.save(a.out, meta)
+    log("saved %s: val loss %.5f (untrained %.5f, zero output %.5f)" % (a.out, val1, val0, zero))
+    print(json.dumps({k: v for k, v in meta.items() if k not in ("val_keys",)}))
+    ws.close()
+
+
+if __name__ == "__main__":
+    main()

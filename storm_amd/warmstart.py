@@ -1,0 +1,375 @@
+"""Learned warm start of the steady FVT solver.
+
+An edge message-passing network (storm_amd/mpnn.py over libstorm_mpnn.so:
+BF16 MFMA GEMMs with FP32 accumulation, hand-written forward, backward and
+AdamW kernels) predicts an initial cell state (h, hu, hv) of a reach from its
+bed elevation, Manning n, and boundary discharge and stage.  The FP64 solver
+then starts from the prediction and runs to the same residual as a cold
+start, so it is both the fallback and the judge of the prediction: a poor
+prediction costs steps, never accuracy.  No counterpart exists in the
+reference, which starts from the user's initial conditions
+(Source_xsy-64bit/dataprepFVT.f90:90-130).
+
+Graph: one node per cell, one directed edge each way across every interior
+side.
+
+Reach frame.  The inputs are put in a frame of the reach: the axis ``e``
+runs from the midpoint of the inflow string to the midpoint of the outflow
+string, ``dist`` is a cell's distance to the outlet along it, ``H`` is the
+outflow stage, ``q`` the inflow discharge per metre of inflow string, and
+``zfit = a + S*dist`` the least-squares line of the cell bed elevations
+against ``dist`` (the fitted bed slope S).  The frame assumes a reach whose
+axis is straight (the flumes of the ensemble generator); a curved reach
+would need a curvilinear coordinate in its place.
+
+Node features (16): bed relative to the outlet stage, bed relative to the
+fitted line, Manning n, q, depth of the outlet stage over the fitted bed,
+fitted slope, distance to the outlet and to the inlet, inflow / outflow /
+wall flags, cell size, the discharge and stage carried by the flagged
+boundary cells, two zeros.  Edge features (16): the centroid offset in the
+reach frame and its length, the bed step and the step of the bed relative to
+the fitted line, a constant, ten zeros.
+
+Outputs, each scaled to order one:
+  y0 = (zeta - H - S*dist) / 0.1 m   water surface relative to the plane
+                                     through the outlet stage parallel to
+                                     the fitted bed
+  y1 = (hu_e / q - 1) / 0.01         discharge along the axis
+  y2 = (hv_e / q) / 0.01             discharge across the axis
+The other 13 outputs are trained to zero.  ``decode`` turns them back into
+h = max(zeta - zc, 0), hu, hv and the u, v that storm_fvt_set_state takes.
+With all outputs zero the decode is the plane itself with uniform discharge
+(``baseline_state``), the non-learned guess the network improves on.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import cases as _cases
+
+F_NODE = 16
+F_EDGE = 16
+F_OUT = 16
+D_SCALE = 5.0       # m, depth/elevation scale of the features
+Y0_SCALE = 0.1      # m, water-surface output
+Q_SCALE = 0.01      # relative discharge outputs
+
+
+@dataclasses.dataclass
+class Frame:
+    """The reach frame and the graph of one reach."""
+    src: np.ndarray
+    dst: np.ndarray
+    node: np.ndarray        # (N, 16) float32
+    edge: np.ndarray        # (E, 16) float32
+    zc: np.ndarray          # cell bed elevation
+    dist: np.ndarray        # distance to the outlet along the axis
+    e: np.ndarray           # unit axis (2,)
+    H: float
+    q: float
+    S: float
+    a: float
+
+
+def _string_geometry(case, b):
+    xs, ys = case.x[b.nodes], case.y[b.nodes]
+    length = float(np.hypot(np.diff(xs), np.diff(ys)).sum())
+    mid = np.array([xs.mean(), ys.mean()])
+    return length, mid
+
+
+def _string_cells(case, nodes) -> np.ndarray:
+    """Cells with a side on the string (consecutive nodes of the chain)."""
+    key = {}
+    for k in np.nonzero(case.edge_e[:, 1] < 0)[0]:  # string sides are boundary sides
+        p0, p1 = case.edge_p[k]
+        key[(min(p0, p1), max(p0, p1))] = k
+    out = []
+    for a, b in zip(nodes[:-1], nodes[1:]):
+        k = key.get((min(a, b), max(a, b)))
+        if k is not None:
+            out.append(case.edge_e[k, 0])
+    return np.unique(np.array(out, dtype=np.int64))
+
+
+def frame(case) -> Frame:
+    """Graph, features and reach frame of a case (one inflow string with a
+    discharge and one outflow string with a stage)."""
+    if len(case.inflows) != 1 or len(case.outflows) != 1:
+        raise ValueError("the warm start needs exactly one inflow and one outflow string")
+    bi, bo = case.inflows[0], case.outflows[0]
+    if bi.kind not in (1, 5) or bo.kind != 1 or len(bo.v) == 0:
+        raise ValueError("the warm start needs a discharge inflow (vtype 1/5) and a stage outflow")
+    w_in, m_in = _string_geometry(case, bi)
+    _, m_out = _string_geometry(case, bo)
+    ax = m_out - m_in
+    e = ax / np.hypot(*ax)
+    tri = case.tri
+    xc = case.x[tri].sum(axis=1) / 3.0
+    yc = case.y[tri].sum(axis=1) / 3.0
+    zc = case.zvtx[tri].sum(axis=1) / 3.0
+    dist = (m_out[0] - xc) * e[0] + (m_out[1] - yc) * e[1]
+    span = float(np.hypot(*ax))
+    H = float(bo.v[0])
+    q = float(bi.v[0]) / w_in
+    A = np.vstack([np.ones_like(dist), dist]).T
+    a, S = np.linalg.lstsq(A, zc, rcond=None)[0]
+    zfit = a + S * dist
+    # cell areas (t_areas.f90:40-41 form)
+    x0, x1, x2 = (case.x[tri[:, j]] for j in range(3))
+    y0, y1, y2 = (case.y[tri[:, j]] for j in range(3))
+    area = 0.5 * np.abs((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0))
+    N = case.n_elems
+    is_in = np.zeros(N)
+    is_in[_string_cells(case, bi.nodes)] = 1.0
+    is_out = np.zeros(N)
+    is_out[_string_cells(case, bo.nodes)] = 1.0
+    ee = case.edge_e
+    bnd = ee[:, 1] < 0
+    is_wall = np.zeros(N)
+    is_wall[ee[bnd, 0]] = 1.0
+    is_wall = np.clip(is_wall - is_in - is_out, 0.0, 1.0)
+    X = np.zeros((N, F_NODE), dtype=np.float32)
+    X[:, 0] = (zc - H) / D_SCALE
+    X[:, 1] = (zc - zfit) / Y0_SCALE
+    X[:, 2] = case.cd / 0.04 - 1.0
+    X[:, 3] = q / D_SCALE
+    X[:, 4] = (H - zfit) / D_SCALE
+    X[:, 5] = S * 1000.0
+    X[:, 6] = dist / 1000.0
+    X[:, 7] = (span - dist) / 1000.0
+    X[:, 8] = is_in
+    X[:, 9] = is_out
+    X[:, 10] = is_wall
+    X[:, 11] = np.sqrt(area) / D_SCALE
+    X[:, 12] = is_in * q / D_SCALE
+    X[:, 13] = is_out * (H - zc) / D_SCALE
+    inter = ee[:, 1] >= 0
+    i, j = ee[inter, 0], ee[inter, 1]
+    src = np.concatenate([i, j]).astype(np.int32)
+    dst = np.concatenate([j, i]).astype(np.int32)
+    dx = xc[dst] - xc[src]
+    dy = yc[dst] - yc[src]
+    E = src.shape[0]
+    EF = np.zeros((E, F_EDGE), dtype=np.float32)
+    EF[:, 0] = (dx * e[0] + dy * e[1]) / D_SCALE
+    EF[:, 1] = (-dx * e[1] + dy * e[0]) / D_SCALE
+    EF[:, 2] = np.hypot(dx, dy) / D_SCALE
+    za = zc - zfit
+    EF[:, 3] = (zc[dst] - zc[src]) / Y0_SCALE
+    EF[:, 4] = (za[dst] - za[src]) / Y0_SCALE
+    EF[:, 5] = 1.0
+    return Frame(src, dst, X, EF, zc, dist, e, H, q,
+                 float(S), float(a))
+
+
+def targets(fr: Frame, h, qu, qv) -> np.ndarray:
+    """Network outputs that decode to the cell state (h, hu, hv)."""
+    N = h.shape[0]
+    T = np.zeros((N, F_OUT), dtype=np.float64)
+    zeta = h + fr.zc
+    T[:, 0] = (zeta - fr.H - fr.S * fr.dist) / Y0_SCALE
+    T[:, 1] = ((qu * fr.e[0] + qv * fr.e[1]) / fr.q - 1.0) / Q_SCALE
+    T[:, 2] = ((qv * fr.e[0] - qu * fr.e[1]) / fr.q) / Q_SCALE
+    return T.astype(np.float32)
+
+
+def decode(fr: Frame, y: np.ndarray, h_dry: float = 1e-3):
+    """(h, u, v) of network outputs y (N, >=3)."""
+    y = np.asarray(y, dtype=np.float64)
+    zeta = fr.H + fr.S * fr.dist + Y0_SCALE * y[:, 0]
+    h = np.maximum(zeta - fr.zc, 0.0)
+    qa = fr.q * (1.0 + Q_SCALE * y[:, 1])
+    qc = fr.q * Q_SCALE * y[:, 2]
+    hu = qa * fr.e[0] - qc * fr.e[1]
+    hv = qa * fr.e[1] + qc * fr.e[0]
+    wet = h > h_dry
+    u = np.where(wet, hu / np.where(wet, h, 1.0), 0.0)
+    v = np.where(wet, hv / np.where(wet, h, 1.0), 0.0)
+    return h, u, v
+
+
+def baseline_state(case, h_dry: float = 1e-3):
+    """The decode of zero outputs: the plane through the outlet stage
+    parallel to the fitted bed, with uniform discharge along the axis."""
+    fr = frame(case)
+    return decode(fr, np.zeros((case.n_elems, 3)), h_dry)
+
+
+# --------------------------------------------------------------------------
+# Training data: steady states from the batched FP64 solver
+# --------------------------------------------------------------------------
+
+STEADY_OPTS = dict(flux="hll", limiter=0, rkorder=2, cfl=0.9, steady=True, stop_cr=1,
+                   rstop=1e-6, dt_op=0, graph=1)
+
+
+def steady_states(reaches: Sequence, opts: Optional[Dict] = None, max_steps: int = 60000,
+                  chunk: int = 1000, init: Optional[List] = None, log=None):
+    """Run reaches together (one batched context) until each converges.
+    Returns per reach: (state dict or None if it did not converge, steps),
+    and the wall time.  init: optional per-reach (h, u, v) starting states."""
+    from . import fvt
+    o = dict(STEADY_OPTS)
+    if opts:
+        o.update(opts)
+    o["istop"] = max_steps
+    t0 = time.perf_counter()
+    b = fvt.BatchSolver(list(reaches), o)
+    if init is not None:
+        b.set_state(np.concatenate([s[0] for s in init]), np.concatenate([s[1] for s in init]),
+                    np.concatenate([s[2] for s in init]))
+    while True:
+        b.run(chunk)
+        it, tt, rm, dn = b.status()
+        if log:
+            log("steady: %d/%d converged, step %d" % (dn.sum(), len(dn), it.max()))
+        if dn.all() or it.max() >= max_steps:
+            break
+    wall = time.perf_counter() - t0
+    parts = b.split(b.state())
+    b.close()
+    out = []
+    for r, p in enumerate(parts):
+        ok = bool(dn[r]) and np.isfinite(rm[r])
+        out.append((p if ok else None, int(it[r])))
+    return out, wall
+
+
+# --------------------------------------------------------------------------
+# The network
+# --------------------------------------------------------------------------
+
+def _union(frames: Sequence[Frame], ys: Optional[Sequence[np.ndarray]] = None):
+    """Disjoint union of reach graphs (one batch)."""
+    off = 0
+    src, dst, X, EF, T = [], [], [], [], []
+    for k, fr in enumerate(frames):
+        src.append(fr.src + off)
+        dst.append(fr.dst + off)
+        X.append(fr.node)
+        EF.append(fr.edge)
+        if ys is not None:
+            T.append(ys[k])
+        off += fr.node.shape[0]
+    cat = np.concatenate
+    return (cat(src).astype(np.int32), cat(dst).astype(np.int32), cat(X), cat(EF),
+            cat(T) if ys is not None else None)
+
+
+class WarmStart:
+    """The trained network and what it needs to predict a reach's state."""
+
+    def __init__(self, layers: int = 6, lr: float = 1e-3, weight_decay: float = 1e-4,
+                 seed: int = 0, params: Optional[np.ndarray] = None):
+        from . import mpnn
+        self.layers = layers
+        self.net = mpnn.Net(layers=layers, f_node=F_NODE, f_edge=F_EDGE, f_out=F_OUT, lr=lr,
+                            weight_decay=weight_decay, seed=seed)
+        if params is not None:
+            self.net.set_params(params)
+        self.history: List[Dict] = []
+
+    # -- training -------------------------------------------------------
+    def train(self, frames: Sequence[Frame], ys: Sequence[np.ndarray], epochs: int,
+              batch: int = 4, seed: int = 0, log=None, time_limit: Optional[float] = None,
+              lr: float = 1e-3, lr_final: float = 1e-5):
+        """AdamW over mini-batches of whole reaches (disjoint unions), the
+        learning rate decaying from lr to lr_final on a cosine over the
+        epochs."""
+        rng = np.random.default_rng(seed)
+        n = len(frames)
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            c = 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs - 1, 1)))
+            self.net.set_lr(lr_final + (lr - lr_final) * c)
+            order = rng.permutation(n)
+            tot, cnt = 0.0, 0
+            for s in range(0, n, batch):
+                idx = order[s:s + batch]
+                src, dst, X, EF, T = _union([frames[i] for i in idx], [ys[i] for i in idx])
+                self.net.set_graph(src, dst, X, EF)
+                self.net.forward(copy=False)
+                loss = self.net.backward(T)
+                self.net.step()
+                tot += loss * len(idx)
+                cnt += len(idx)
+            rec = dict(epoch=ep, loss=tot / cnt, seconds=time.perf_counter() - t0)
+            self.history.append(rec)
+            if log:
+                log("epoch %d loss %.5f (%.1f s)" % (ep, rec["loss"], rec["seconds"]))
+            if time_limit and time.perf_counter() - t0 > time_limit:
+                break
+        return self.history
+
+    def loss(self, frames: Sequence[Frame], ys: Sequence[np.ndarray], batch: int = 4) -> float:
+        """Mean loss over reaches without updating."""
+        tot = 0.0
+        for s in range(0, len(frames), batch):
+            src, dst, X, EF, T = _union(frames[s:s + batch], ys[s:s + batch])
+            self.net.set_graph(src, dst, X, EF)
+            y = self.net.forward()
+            tot += float(((y - T) ** 2).mean()) * len(frames[s:s + batch])
+        return tot / len(frames)
+
+    # -- inference ------------------------------------------------------
+    def predict_outputs(self, frames: Sequence[Frame], batch: int = 4) -> List[np.ndarray]:
+        out = []
+        for s in range(0, len(frames), batch):
+            fs = frames[s:s + batch]
+            src, dst, X, EF, _ = _union(fs)
+            self.net.set_graph(src, dst, X, EF)
+            y = self.net.forward()
+            o = 0
+            for fr in fs:
+                n = fr.node.shape[0]
+                out.append(y[o:o + n])
+                o += n
+        return out
+
+    def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3):
+        """Per case: (h, u, v) for storm_fvt_set_state."""
+        frs = [frame(c) for c in cases_]
+        ys = self.predict_outputs(frs, batch)
+        return [decode(fr, y, h_dry) for fr, y in zip(frs, ys)]
+
+    # -- checkpoint ---------------------------------------------------------
+    def save(self, path: str, meta: Optional[Dict] = None):
+        m = dict(layers=self.layers, f_node=F_NODE, f_edge=F_EDGE, f_out=F_OUT,
+                 history=self.history)
+        if meta:
+            m.update(meta)
+        np.savez_compressed(path, params=self.net.params(), meta=json.dumps(m))
+
+    @classmethod
+    def load(cls, path: str) -> "WarmStart":
+        d = np.load(path, allow_pickle=False)
+        meta = json.loads(str(d["meta"]))
+        ws = cls(layers=int(meta["layers"]), params=d["params"])
+        ws.meta = meta
+        return ws
+
+    def close(self):
+        self.net.close()
+
+
+DEFAULT_CHECKPOINT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                  "warmstart_mpnn.npz")
+
+
+def training_reach(k: int, **kw):
+    """Training reach k: the ensemble generator at 20k-100k triangles,
+    seeds 5005+k."""
+    return _cases.ensemble_reach(k, n_min=20000, n_max=100000, base_seed=5005, steady=True, **kw)
+
+
+def heldout_reach(k: int, n: int = 200000, **kw):
+    """Held-out reach k: the ensemble generator at n triangles, seeds
+    9009+k (disjoint from the training seeds)."""
+    return _cases.ensemble_reach(k, n_min=n, n_max=n, base_seed=9009, steady=True, **kw)

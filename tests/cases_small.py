@@ -41,7 +41,8 @@ def option_cases():
     """Cases for options whose reference routine is defective or not on the
     reference's active path (so only the oracle, pinned by closed forms,
     checks them): the corrected HLL and BGNVC fluxes, Rusanov, Green-Gauss
-    gradients, explicit Manning friction and the corrected CFL time step."""
+    gradients, explicit Manning friction, the corrected CFL time step, and
+    stopping test 2."""
     return {
         "flume_hll_cfl": (cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=31,
                                       dt_op=1, cfl=0.9, istop=40, q_in=15.0, flux="hll"), 40),
@@ -56,4 +57,10 @@ def option_cases():
         "flume_explicit_friction": (cases.flume(nx=20, ny=6, length=100.0, width=30.0,
                                                 seed=35, dt_op=1, cfl=0.5, istop=30, q_in=15.0,
                                                 friction=1), 30),
+        # stopping test 2 (discharge and depth), converging near step 6,500
+        "flume_stage_steady_stop2": (cases.flume(nx=16, ny=6, length=100.0, width=30.0,
+                                                 seed=21, dt_op=0, cfl=0.3, istop=20000,
+                                                 in_kind=4, out_kind=1, q_in=2.3, h_out=2.0,
+                                                 h0=2.0, steady=True, stop_cr=2, rstop=1e-5,
+                                                 limiter=0, flux="hll"), 20000),
     }

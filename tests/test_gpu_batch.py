@@ -59,3 +59,29 @@ def test_batch_steady_reaches_retire_independently():
         s.close()
     assert len(set(it.tolist())) > 1  # they converged at different steps
     b.close()
+
+
+def test_nonfinite_stop_retires_a_diverging_run():
+    """A steady run at CFL 4 diverges (NaN residual from step 10 in the oracle
+    too).  By default it runs on to MAX ITERATION as the reference does; with
+    nonfinite_stop it stops at the first non-finite residual."""
+    import oracle as O
+    c = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41, dt_op=0, cfl=4.0,
+                    istop=200, q_in=15.0, steady=True, stop_cr=1, rstop=1e-12, limiter=0,
+                    flux="hll")
+    print("[gpu nonfinite] CFL 4 steady flume", flush=True)
+    o = O.Oracle(c)
+    first = None
+    for _ in range(200):
+        r = o.run(1)
+        if first is None and not np.isfinite(r["rmax"]):
+            first = r["iter"]
+    assert first is not None
+    s = fvt.FVTSolver(c)
+    it, _, rm = s.run(200)
+    assert it == 200 and np.isnan(rm)
+    s.close()
+    s = fvt.FVTSolver(c, dict(nonfinite_stop=True))
+    it, _, rm = s.run(200)
+    assert it == first and np.isnan(rm)
+    s.close()

@@ -20,6 +20,12 @@ def test_forward_and_gradients_match_fp64_reference():
     net = mpnn.Net(layers=3, seed=7)
     ref = Ref(layers=3)
     p = net.params()
+    # the decoder starts at zero (an untrained network outputs zero, and every
+    # other gradient vanishes): give it weights so every gradient is checked
+    o, n = ref.off["dec_w"]
+    assert not p[o:o + n].any()
+    p[o:o + n] = np.random.default_rng(5).uniform(-0.3, 0.3, n)
+    net.set_params(p)
     net.set_graph(src, dst, X, EF)
     y = net.forward()
     yr, _ = ref.forward(p, src, dst, X, EF)

@@ -1,7 +1,7 @@
 """HIP path vs the CPU oracle for the options whose reference routine is
 defective or off the reference's active path (corrected HLL and BGNVC,
 Rusanov, Green-Gauss gradients, explicit friction, the corrected CFL time
-step).  The oracle's forms of these are pinned by the closed-form tests in
+step, stopping test 2).  The oracle's forms of these are pinned by the closed-form tests in
 test_oracle_closed_forms.py."""
 import numpy as np
 import pytest
@@ -34,6 +34,7 @@ def test_option_parity(name):
     sg = s.state()
     assert it == r_o["iter"]
     assert tt == pytest.approx(r_o["ttime"], rel=1e-12)
+    assert rm == pytest.approx(r_o["rmax"], rel=1e-8)
     for k in ("h", "qh", "qu", "qv", "u", "v"):
         assert _cmp(so[k], sg[k]) <= RTOL, (k, _cmp(so[k], sg[k]))
     assert np.array_equal(so["dry"], sg["dry"])

@@ -220,3 +220,38 @@ def test_flat_lake_at_rest_stays_at_rest():
     s = o.state()
     assert np.abs(s["u"]).max() < 1e-12 and np.abs(s["v"]).max() < 1e-12
     np.testing.assert_allclose(s["h"], 2.0, atol=1e-12)
+
+
+def test_stop_cr2_is_the_larger_relative_rms_change_of_h_and_discharge():
+    """Stopping test 2 (no reference counterpart) against its definition,
+    formed from two consecutive states in NumPy: the larger of
+    sqrt(sum dh^2 / sum h^2) and sqrt(sum |dq|^2 / sum |q|^2), q = (hu, hv),
+    with the start-of-step values in the denominators."""
+    import oracle as O
+    for stop_cr in (1, 2):
+        c = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41, dt_op=0, cfl=0.3,
+                        istop=3000, q_in=15.0, steady=True, stop_cr=stop_cr, rstop=1e-30,
+                        limiter=0, flux="hll")
+        o = O.Oracle(c)
+        o.run(50)
+        s0 = o.state()
+        r = o.run(1)
+        s1 = o.state()
+        dh = math.sqrt(((s1["qh"] - s0["qh"]) ** 2).sum() / (s0["qh"] ** 2).sum())
+        dq = math.sqrt(((s1["qu"] - s0["qu"]) ** 2 + (s1["qv"] - s0["qv"]) ** 2).sum() /
+                       (s0["qu"] ** 2 + s0["qv"] ** 2).sum())
+        want = dh if stop_cr == 1 else max(dh, dq)
+        assert r["rmax"] == pytest.approx(want, rel=1e-12)
+        assert dq > 10 * dh  # the discharge term decides test 2 here
+
+
+def test_stop_cr2_converges_geometrically_with_a_stage_inflow():
+    """With a stage inflow (no fixed-point tolerance on the inflow discharge)
+    the discharge change of stopping test 2 decays geometrically."""
+    import oracle as O
+    c = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=21, dt_op=0, cfl=0.3,
+                    istop=30000, in_kind=4, out_kind=1, q_in=2.3, h_out=2.0, h0=2.0,
+                    steady=True, stop_cr=2, rstop=1e-30, limiter=0, flux="hll")
+    o = O.Oracle(c)
+    r = [o.run(2000)["rmax"] for _ in range(5)]
+    assert all(b < 0.5 * a for a, b in zip(r[1:], r[2:]))

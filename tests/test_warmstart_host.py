@@ -1,0 +1,74 @@
+"""Host side of the learned warm start: the reach frame, the graph, and the
+target/decode pair (no device needed)."""
+import numpy as np
+
+from storm_amd import cases, warmstart as W
+
+
+def _reach():
+    return cases.ensemble_reach(3, n_min=2000, n_max=2000, base_seed=77)
+
+
+def test_graph_is_the_cell_adjacency_both_ways():
+    c = _reach()
+    fr = W.frame(c)
+    inter = c.edge_e[:, 1] >= 0
+    assert fr.src.shape[0] == 2 * inter.sum()
+    pairs = set(zip(fr.src.tolist(), fr.dst.tolist()))
+    assert all((b, a) in pairs for a, b in pairs)
+    assert fr.node.shape == (c.n_elems, W.F_NODE) and fr.edge.shape == (fr.src.shape[0], W.F_EDGE)
+    assert np.isfinite(fr.node).all() and np.isfinite(fr.edge).all()
+    # inflow / outflow flags sit at x = 0 and x = L
+    xc = c.x[c.tri].mean(axis=1)
+    assert xc[fr.node[:, 8] > 0].max() < 30.0
+    assert xc[fr.node[:, 9] > 0].min() > 970.0
+
+
+def test_frame_of_a_flume():
+    c = _reach()
+    fr = W.frame(c)
+    assert np.allclose(fr.e, [1.0, 0.0], atol=1e-3)
+    assert fr.H == c.outflows[0].v[0]
+    # q: discharge over the inflow string's length (100 m)
+    assert abs(fr.q - c.inflows[0].v[0] / 100.0) < 1e-9 * fr.q
+    # the fitted bed slope recovers the generator's slope (noise sd 0.02 m)
+    assert abs(fr.S - c.meta["slope"]) < 0.05 * c.meta["slope"] + 2e-5
+
+
+def test_decode_inverts_targets():
+    c = _reach()
+    fr = W.frame(c)
+    rng = np.random.default_rng(0)
+    h = np.maximum(fr.H + fr.S * fr.dist - fr.zc + rng.normal(0, 0.05, c.n_elems), 0.0)
+    h[:5] = 0.0  # some dry cells
+    qu = fr.q * (1 + rng.normal(0, 0.01, c.n_elems))
+    qv = fr.q * rng.normal(0, 0.01, c.n_elems)
+    T = W.targets(fr, h, qu, qv).astype(np.float64)
+    hd, ud, vd = W.decode(fr, T)
+    wet = h > 1e-3
+    # float32 targets: 0.1 m * 2^-24 relative on O(1) values
+    assert np.abs(hd - h).max() < 1e-6
+    assert np.allclose(ud[wet] * hd[wet], qu[wet], rtol=1e-5, atol=1e-6 * fr.q)
+    assert np.allclose(vd[wet] * hd[wet], qv[wet], rtol=1e-5, atol=1e-6 * fr.q)
+    assert (ud[~wet] == 0).all() and (vd[~wet] == 0).all()
+
+
+def test_baseline_is_the_plane_with_uniform_discharge():
+    c = _reach()
+    fr = W.frame(c)
+    h, u, v = W.baseline_state(c)
+    assert np.allclose(h, np.maximum(fr.H + fr.S * fr.dist - fr.zc, 0.0))
+    wet = h > 1e-3
+    assert np.allclose(u[wet] * h[wet], fr.q * fr.e[0])
+    assert np.allclose(v[wet] * h[wet], fr.q * fr.e[1], atol=1e-12 + 1e-3 * fr.q)
+
+
+def test_union_offsets_the_graphs():
+    frs = [W.frame(cases.ensemble_reach(k, n_min=500, n_max=900, base_seed=5)) for k in range(3)]
+    src, dst, X, EF, T = W._union(frs, [np.zeros((f.node.shape[0], 16), np.float32) for f in frs])
+    n = [f.node.shape[0] for f in frs]
+    assert X.shape[0] == sum(n) and T.shape[0] == sum(n)
+    assert src.max() < sum(n) and dst.max() < sum(n)
+    o = n[0]
+    e0 = frs[0].src.shape[0]
+    assert (src[e0:e0 + frs[1].src.shape[0]] >= o).all()
