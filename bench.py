@@ -230,7 +230,7 @@ def warmstart(args):
         finals[mode] = out
         res[mode] = dict(wall_s=wall, init_s=pred_s, converged=int(ok.sum()),
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
-                         steps_sum=int(steps.sum()),
+                         steps_sum=int(steps.sum()), steps=steps.tolist(),
                          cell_updates=float(sum(c.n_elems * it for c, it in zip(reaches, steps))))
         if mode == "learned":
             # prediction error against the state the solver converged to

@@ -929,35 +929,40 @@ constexpr int QMAX = 512;  // edges of one inflowQbyH string
 struct Lds {
   double *fh, *fu, *fv;          // [3][CAP] face states, side-major
   double *z, *h;                 // [CAP] bed and stage-input depth
-  double *vx, *vy, *vz;          // [3][CAP] vertex coordinates and bed of the tile cells
+  double *vx, *vy, *vz;          // [3][TMAX] vertex coordinates and bed of the owned cells
   uint8_t* dry;                  // [CAP] dry flags
   double *qh, *qu, *qv, *qa, *qq;  // [QMAX] string edges (overlay)
   uint8_t* qok;                    // [QMAX] (overlay)
 };
 
+// Face states, bed and depth for every cell of the tile and halo; vertex
+// coordinates and bed for the owned cells only.  At capacity 256 this is
+// 38.3 KB, so four blocks fit the 160 KB of a CU.
 __host__ __device__ constexpr size_t lds_bytes(int cap) {
-  return sizeof(double) * 20 * (size_t)cap + (size_t)cap;
+  return sizeof(double) * (11 * (size_t)cap + 9 * (size_t)tile_tmax(cap)) + (size_t)cap;
 }
 
 __device__ __forceinline__ Lds lds_layout(double* base, int cap) {
   Lds L;
   L.fh = base; L.fu = L.fh + 3 * cap; L.fv = L.fu + 3 * cap;
   L.z = L.fv + 3 * cap; L.h = L.z + cap;
-  L.vx = L.h + cap; L.vy = L.vx + 3 * cap; L.vz = L.vy + 3 * cap;
-  L.dry = reinterpret_cast<uint8_t*>(L.vz + 3 * cap);
+  const int tm = tile_tmax(cap);
+  L.vx = L.h + cap; L.vy = L.vx + 3 * tm; L.vz = L.vy + 3 * tm;
+  L.dry = reinterpret_cast<uint8_t*>(L.vz + 3 * tm);
   // overlay of the 20*cap cell doubles: 5*QMAX doubles and QMAX flags
   L.qh = base; L.qu = L.qh + QMAX; L.qv = L.qu + QMAX; L.qa = L.qv + QMAX; L.qq = L.qa + QMAX;
   L.qok = reinterpret_cast<uint8_t*>(L.qq + QMAX);
   return L;
 }
-static_assert(5 * QMAX * 8 + QMAX <= 20 * 256 * 8, "string buffers must fit the cell arrays");
+static_assert(5 * QMAX * 8 + QMAX <= 11 * 256 * 8, "string buffers must fit the cell arrays");
 
 __device__ __forceinline__ int lid(uint64_t w, int j) { return (int)(w >> (16 * j)) & 0xFFFF; }
 
 // side j (known only at run time) of tile cell t
 template <int CAP>
 __device__ __forceinline__ Side side_dyn(const Lds& L, int t, int j) {
-  const int a = j * CAP + t, b = (j == 2 ? 0 : j + 1) * CAP + t;
+  constexpr int TM = tile_tmax(CAP);
+  const int a = j * TM + t, b = (j == 2 ? 0 : j + 1) * TM + t;
   return side_of(L.vx[a], L.vy[a], L.vz[a], L.vx[b], L.vy[b], L.vz[b]);
 }
 
@@ -1069,8 +1074,11 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 //      last stage also implicit friction, hvalve, the residual partial sums
 //      and the next step's CFL candidate (per-tile partials, reduced in a
 //      fixed order by k_control).
+// At most 128 VGPRs (four waves per SIMD, which the shared memory also
+// allows at capacity 256); the compiler spills a few values to scratch.
 template <int CAP>
-__global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
+__global__ void __launch_bounds__(CAP) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   extern __shared__ double lds_dyn[];
   __shared__ double s_at[MAX_STRINGS];
   __shared__ int s_null[MAX_STRINGS], s_skip[MAX_STRINGS];
@@ -1125,7 +1133,8 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
     if (own) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        L.vx[k * CAP + t] = tr.x[k]; L.vy[k * CAP + t] = tr.y[k]; L.vz[k * CAP + t] = tr.zv[k];
+        constexpr int TM = tile_tmax(CAP);
+        L.vx[k * TM + t] = tr.x[k]; L.vy[k * TM + t] = tr.y[k]; L.vz[k * TM + t] = tr.zv[k];
       }
     }
     if (own && F.fix) {
@@ -1152,7 +1161,8 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
     Tri tr;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      tr.x[k] = L.vx[k * CAP + t]; tr.y[k] = L.vy[k * CAP + t]; tr.zv[k] = L.vz[k * CAP + t];
+      constexpr int TM = tile_tmax(CAP);
+      tr.x[k] = L.vx[k * TM + t]; tr.y[k] = L.vy[k * TM + t]; tr.zv[k] = L.vz[k * TM + t];
     }
     const double dhx = F.dhx, dhy = F.dhy, hi = F.h, umag_in = F.umag;
     const bool dry_i = F.dry;
@@ -1179,7 +1189,7 @@ __global__ void __launch_bounds__(CAP) k_stage(Geo g, Opts o, Tiles tl, Stage st
     // interior edges by number, walls, boundary routines), each added to phi
     // as it is formed
     double p0 = ZERO, p1 = ZERO, p2 = ZERO;
-#pragma unroll
+#pragma unroll 1  // one side at a time: fewer live registers
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
       double d0, d1, d2;

@@ -728,7 +728,7 @@ int build_tiles(Prep& S, int cap) {
         if (!dup) newh[nnh++] = nb;
       }
       const int T2 = T + 1, H2 = H + nnh - (was_halo ? 1 : 0);
-      if (T2 + H2 > cap) break;
+      if (T2 + H2 > cap || T2 > tile_tmax(cap)) break;
       for (int k = 0; k < nnh; ++k) hstamp[newh[k]] = tile;
       if (was_halo) hstamp[c] = -1;
       T = T2; H = H2;
