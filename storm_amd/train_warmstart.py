@@ -10,8 +10,10 @@ generates.
    in h).  A reach that diverges or does not converge is left out.
 2. Features and targets (storm_amd.warmstart.frame / targets); a held-out
    validation share.
-3. AdamW on mini-batches of whole reaches, all on the GPU in the network's
-   own forward, backward and AdamW kernels.
+3. AdamW on batches that join windows of a few thousand cells cut from many
+   different reaches (the prediction depends mostly on each reach's slope,
+   roughness, discharge and stage, so a batch must span many reaches), all
+   on the GPU in the network's own forward, backward and AdamW kernels.
 4. The checkpoint (FP32 parameters + a JSON record of the data and the loss
    history) is written with WarmStart.save.
 """
@@ -55,12 +57,12 @@ def main(argv=None):
     ap.add_argument("--gen-batch", type=int, default=128)
     ap.add_argument("--rstop", type=float, default=1e-8)
     ap.add_argument("--max-steps", type=int, default=150000)
-    ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--batch", type=int, default=4, help="reaches per mini-batch")
+    ap.add_argument("--steps", type=int, default=4000, help="AdamW updates")
+    ap.add_argument("--reaches-per-step", type=int, default=32)
+    ap.add_argument("--window", type=int, default=4096, help="cells cut from each reach")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5, help="end of the cosine decay")
     ap.add_argument("--val", type=float, default=0.05, help="validation share")
-    ap.add_argument("--time-limit", type=float, default=0.0, help="training seconds (0: none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
@@ -82,18 +84,19 @@ def main(argv=None):
     va_f, va_y = frames[-n_val:], ys[-n_val:]
     log("training on %d reaches, validating on %d" % (len(tr_f), len(va_f)))
     ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed)
-    val0 = ws.loss(va_f, va_y, a.batch)
+    val0 = ws.loss(va_f, va_y)
     t1 = time.perf_counter()
-    ws.train(tr_f, tr_y, a.epochs, batch=a.batch, seed=a.seed, log=log,
-             time_limit=a.time_limit or None, lr=a.lr, lr_final=a.lr_final)
+    ws.train_windows(tr_f, tr_y, a.steps, reaches_per_step=a.reaches_per_step, size=a.window,
+                     seed=a.seed, log=log, lr=a.lr, lr_final=a.lr_final)
     train_s = time.perf_counter() - t1
-    val1 = ws.loss(va_f, va_y, a.batch)
+    val1 = ws.loss(va_f, va_y)
     zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
     meta = dict(
         reaches=a.reaches, converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
         train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
         steps_median=float(np.median([steps[k] for k in ks])),
-        generate_s=gen_s, train_s=train_s, epochs_run=len(ws.history), batch=a.batch,
+        generate_s=gen_s, train_s=train_s, steps=a.steps, reaches_per_step=a.reaches_per_step,
+        window=a.window,
         lr=a.lr, lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1, val_loss_of_zero_output=zero,
         generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
     ws.save(a.out, meta)

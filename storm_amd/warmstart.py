@@ -78,6 +78,8 @@ class Frame:
     q: float
     S: float
     a: float
+    order: Optional[np.ndarray] = None   # cells along the axis (set by window)
+    rank: Optional[np.ndarray] = None
 
 
 def _string_geometry(case, b):
@@ -249,6 +251,32 @@ def steady_states(reaches: Sequence, opts: Optional[Dict] = None, max_steps: int
 # The network
 # --------------------------------------------------------------------------
 
+def window(fr: Frame, y: np.ndarray, start: int, size: int, hops: int):
+    """A training sample cut from a reach: the `size` cells ranked start..
+    start+size-1 along the axis (downstream order), the edges between them,
+    their features and targets, and loss weights that are zero within `hops`
+    edges of the cut (where the network's receptive field of `hops` layers
+    is truncated) and one elsewhere."""
+    if fr.rank is None:
+        order = np.argsort(fr.dist, kind="stable")[::-1]  # upstream first
+        fr.order = order.astype(np.int64)
+        fr.rank = np.empty_like(fr.order)
+        fr.rank[fr.order] = np.arange(fr.order.shape[0])
+    a, b = start, start + size
+    rs, rd = fr.rank[fr.src], fr.rank[fr.dst]
+    ins, ind = (rs >= a) & (rs < b), (rd >= a) & (rd < b)
+    keep = ins & ind
+    src = (rs[keep] - a).astype(np.int32)
+    dst = (rd[keep] - a).astype(np.int32)
+    near = np.zeros(size, dtype=bool)
+    near[rs[ins & ~ind] - a] = True  # cells with a neighbour outside the window
+    for _ in range(hops - 1):
+        near[dst[near[src]]] = True
+    cells = fr.order[a:b]
+    return (src, dst, fr.node[cells], fr.edge[keep], y[cells],
+            np.where(near, 0.0, 1.0).astype(np.float32))
+
+
 def _union(frames: Sequence[Frame], ys: Optional[Sequence[np.ndarray]] = None):
     """Disjoint union of reach graphs (one batch)."""
     off = 0
@@ -311,14 +339,61 @@ class WarmStart:
                 break
         return self.history
 
-    def loss(self, frames: Sequence[Frame], ys: Sequence[np.ndarray], batch: int = 4) -> float:
-        """Mean loss over reaches without updating."""
+    def train_windows(self, frames: Sequence[Frame], ys: Sequence[np.ndarray], steps: int,
+                      reaches_per_step: int = 32, size: int = 4096, seed: int = 0, log=None,
+                      lr: float = 1e-3, lr_final: float = 1e-5, log_every: int = 100):
+        """AdamW over batches that each join windows of `size` cells cut from
+        `reaches_per_step` different reaches (sampled without replacement),
+        so every update sees many reaches' parameters at once.  The next
+        batch is assembled on a host thread while the device runs the
+        current one.  The learning rate decays from lr to lr_final on a
+        cosine over the steps."""
+        from concurrent.futures import ThreadPoolExecutor
+        rng = np.random.default_rng(seed)
+        n = len(frames)
+        r = min(reaches_per_step, n)
+
+        def batch():
+            parts, off = [], 0
+            for i in rng.choice(n, r, replace=False):
+                fr, y = frames[i], ys[i]
+                k = min(size, fr.node.shape[0])
+                a = int(rng.integers(0, fr.node.shape[0] - k + 1))
+                src, dst, X, EF, T, w = window(fr, y, a, k, self.layers)
+                parts.append((src + off, dst + off, X, EF, T, w))
+                off += k
+            return tuple(np.concatenate([p[j] for p in parts]) for j in range(6))
+
+        t0 = time.perf_counter()
+        tot, cnt = 0.0, 0
+        with ThreadPoolExecutor(1) as ex:
+            nxt = ex.submit(batch)
+            for step in range(steps):
+                src, dst, X, EF, T, w = nxt.result()
+                if step + 1 < steps:
+                    nxt = ex.submit(batch)
+                c = 0.5 * (1.0 + np.cos(np.pi * step / max(steps - 1, 1)))
+                self.net.set_lr(lr_final + (lr - lr_final) * c)
+                self.net.set_graph(src, dst, X, EF)
+                self.net.forward(copy=False)
+                tot += self.net.backward(T, w)
+                cnt += 1
+                self.net.step()
+                if (step + 1) % log_every == 0 or step + 1 == steps:
+                    rec = dict(step=step + 1, loss=tot / cnt, seconds=time.perf_counter() - t0)
+                    self.history.append(rec)
+                    if log:
+                        log("step %d loss %.5f (%.1f s)" % (rec["step"], rec["loss"], rec["seconds"]))
+                    tot, cnt = 0.0, 0
+        return self.history
+
+    def loss(self, frames: Sequence[Frame], ys: Sequence[np.ndarray]) -> float:
+        """Mean over reaches of each reach's mean loss, without updating."""
         tot = 0.0
-        for s in range(0, len(frames), batch):
-            src, dst, X, EF, T = _union(frames[s:s + batch], ys[s:s + batch])
-            self.net.set_graph(src, dst, X, EF)
-            y = self.net.forward()
-            tot += float(((y - T) ** 2).mean()) * len(frames[s:s + batch])
+        for fr, y in zip(frames, ys):
+            self.net.set_graph(fr.src, fr.dst, fr.node, fr.edge)
+            out = self.net.forward()
+            tot += float(((out - y) ** 2).mean())
         return tot / len(frames)
 
     # -- inference ------------------------------------------------------

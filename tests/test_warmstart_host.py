@@ -72,3 +72,39 @@ def test_union_offsets_the_graphs():
     o = n[0]
     e0 = frs[0].src.shape[0]
     assert (src[e0:e0 + frs[1].src.shape[0]] >= o).all()
+
+
+def test_window_cut_and_loss_weights():
+    c = cases.ensemble_reach(1, n_min=3000, n_max=3000, base_seed=9)
+    fr = W.frame(c)
+    y = np.arange(c.n_elems * 16, dtype=np.float32).reshape(-1, 16)
+    size, a, hops = 800, 1000, 3
+    src, dst, X, EF, T, w = W.window(fr, y, a, size, hops)
+    cells = fr.order[a:a + size]
+    # the window is a run along the axis (upstream first) with its own edges
+    assert np.all(np.diff(fr.dist[fr.order]) <= 0)
+    assert np.array_equal(X, fr.node[cells]) and np.array_equal(T, y[cells])
+    assert src.min() >= 0 and max(src.max(), dst.max()) < size
+    pairs = set(zip(cells[src].tolist(), cells[dst].tolist()))
+    full = set(zip(fr.src.tolist(), fr.dst.tolist()))
+    inside = set(cells.tolist())
+    assert pairs == {(i, j) for i, j in full if i in inside and j in inside}
+    assert EF.shape[0] == src.shape[0]
+    # weight zero exactly within `hops` edges of a cell that has an outside
+    # neighbour (brute-force breadth-first distance)
+    nb = {}
+    for i, j in full:
+        nb.setdefault(i, []).append(j)
+    frontier = {i for i in inside if any(j not in inside for j in nb.get(i, []))}
+    dist = {i: 1 for i in frontier}
+    cur = frontier
+    for d in range(2, hops + 1):
+        cur = {j for i in cur for j in nb.get(i, []) if j in inside and j not in dist}
+        for j in cur:
+            dist[j] = d
+    want = np.array([0.0 if int(cid) in dist else 1.0 for cid in cells], dtype=np.float32)
+    assert np.array_equal(w, want)
+    assert 0 < w.sum() < size
+    # a window that takes the whole reach has no cut
+    *_, w_all = W.window(fr, y, 0, c.n_elems, hops)
+    assert w_all.all()
