@@ -64,6 +64,9 @@ def main(argv=None):
     ap.add_argument("--lr-final", type=float, default=1e-5, help="end of the cosine decay")
     ap.add_argument("--val", type=float, default=0.05, help="validation share")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--features", default=str(W.FEATURES),
+                    help="node feature set(s), comma-separated: one network per set, "
+                         "trained on the same data (out gets a _f<set> suffix when several)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -71,38 +74,41 @@ def main(argv=None):
     states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps)
     gen_s = time.perf_counter() - t0
     ks = [k for k, st in enumerate(states) if st is not None]
-    frames, ys = [], []
-    for k in ks:
-        c = W.training_reach(k)
-        fr = W.frame(c)
-        st = states[k]
-        frames.append(fr)
-        ys.append(W.targets(fr, st["qh"], st["qu"], st["qv"]))
-    states = None
-    n_val = max(1, int(round(a.val * len(ks))))
-    tr_f, tr_y = frames[:-n_val], ys[:-n_val]
-    va_f, va_y = frames[-n_val:], ys[-n_val:]
-    log("training on %d reaches, validating on %d" % (len(tr_f), len(va_f)))
-    ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed)
-    val0 = ws.loss(va_f, va_y)
-    t1 = time.perf_counter()
-    ws.train_windows(tr_f, tr_y, a.steps, reaches_per_step=a.reaches_per_step, size=a.window,
-                     seed=a.seed, log=log, lr=a.lr, lr_final=a.lr_final)
-    train_s = time.perf_counter() - t1
-    val1 = ws.loss(va_f, va_y)
-    zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
-    meta = dict(
-        reaches=a.reaches, converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
-        train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
-        steps_median=float(np.median([steps[k] for k in ks])),
-        generate_s=gen_s, train_s=train_s, steps=a.steps, reaches_per_step=a.reaches_per_step,
-        window=a.window,
-        lr=a.lr, lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1, val_loss_of_zero_output=zero,
-        generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
-    ws.save(a.out, meta)
-    log("saved %s: val loss %.5f (untrained %.5f, zero output %.5f)" % (a.out, val1, val0, zero))
-    print(json.dumps({k: v for k, v in meta.items() if k not in ("val_keys",)}))
-    ws.close()
+    sets = [int(f) for f in a.features.split(",")]
+    for fs in sets:
+        frames, ys = [], []
+        for k in ks:
+            c = W.training_reach(k)
+            fr = W.frame(c, fs)
+            st = states[k]
+            frames.append(fr)
+            ys.append(W.targets(fr, st["qh"], st["qu"], st["qv"]))
+        n_val = max(1, int(round(a.val * len(ks))))
+        tr_f, tr_y = frames[:-n_val], ys[:-n_val]
+        va_f, va_y = frames[-n_val:], ys[-n_val:]
+        log("feature set %d: training on %d reaches, validating on %d" % (fs, len(tr_f), len(va_f)))
+        ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed, features=fs)
+        val0 = ws.loss(va_f, va_y)
+        t1 = time.perf_counter()
+        ws.train_windows(tr_f, tr_y, a.steps, reaches_per_step=a.reaches_per_step,
+                         size=a.window, seed=a.seed, log=log, lr=a.lr, lr_final=a.lr_final)
+        train_s = time.perf_counter() - t1
+        val1 = ws.loss(va_f, va_y)
+        zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
+        meta = dict(
+            reaches=a.reaches, converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
+            train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
+            steps_median=float(np.median([steps[k] for k in ks])),
+            generate_s=gen_s, train_s=train_s, steps=a.steps,
+            reaches_per_step=a.reaches_per_step, window=a.window, lr=a.lr,
+            lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1,
+            val_loss_of_zero_output=zero,
+            generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
+        out = a.out if len(sets) == 1 else a.out.replace(".npz", "_f%d.npz" % fs)
+        ws.save(out, meta)
+        log("saved %s: val loss %.5f (untrained %.5f, zero output %.5f)" % (out, val1, val0, zero))
+        print(json.dumps({k: v for k, v in meta.items() if k not in ("val_keys",)}))
+        ws.close()
 
 
 if __name__ == "__main__":

@@ -22,13 +22,19 @@ against ``dist`` (the fitted bed slope S).  The frame assumes a reach whose
 axis is straight (the flumes of the ensemble generator); a curved reach
 would need a curvilinear coordinate in its place.
 
-Node features (16): bed relative to the outlet stage, bed relative to the
-fitted line, Manning n, q, depth of the outlet stage over the fitted bed,
-fitted slope, distance to the outlet and to the inlet, inflow / outflow /
-wall flags, cell size, the discharge and stage carried by the flagged
-boundary cells, two zeros.  Edge features (16): the centroid offset in the
-reach frame and its length, the bed step and the step of the bed relative to
-the fitted line, a constant, ten zeros.
+Node features (16), set 2 (the default): bed relative to the outlet stage,
+bed relative to the fitted line, logs of Manning n, q, the outlet depth h_out
+(outlet stage over the fitted bed) and the fitted slope, distance to the
+outlet and to the inlet, inflow / outflow / wall flags, cell size, the
+dimensionless distance S*dist/h_out, the log of the outlet Froude number
+squared q^2/(g h_out^3), and the depth of the outlet stage over the fitted
+bed at the cell.  Set 1 (checkpoints trained before set 2) has n, q, h_out
+and S on linear scales, the discharge and stage of the flagged boundary cells
+in place of the two dimensionless groups, and a zero last.  The logs and the
+dimensionless groups let the layers form the power laws a steady profile
+depends on as sums.  Edge features (16): the centroid offset in the reach
+frame and its length, the bed step and the step of the bed relative to the
+fitted line, a constant, ten zeros.
 
 Outputs, each scaled to order one:
   y0 = (zeta - H - S*dist) / 0.05 m  water surface relative to the plane
@@ -103,7 +109,10 @@ def _string_cells(case, nodes) -> np.ndarray:
     return np.unique(np.array(out, dtype=np.int64))
 
 
-def frame(case) -> Frame:
+FEATURES = 2  # node feature set of new networks
+
+
+def frame(case, features: int = FEATURES) -> Frame:
     """Graph, features and reach frame of a case (one inflow string with a
     discharge and one outflow string with a stage)."""
     if len(case.inflows) != 1 or len(case.outflows) != 1:
@@ -143,18 +152,32 @@ def frame(case) -> Frame:
     X = np.zeros((N, F_NODE), dtype=np.float32)
     X[:, 0] = (zc - H) / D_SCALE
     X[:, 1] = (zc - zfit) / Y0_SCALE
-    X[:, 2] = case.cd / 0.04 - 1.0
-    X[:, 3] = q / D_SCALE
-    X[:, 4] = (H - zfit) / D_SCALE
-    X[:, 5] = S * 1000.0
     X[:, 6] = dist / 1000.0
     X[:, 7] = (span - dist) / 1000.0
     X[:, 8] = is_in
     X[:, 9] = is_out
     X[:, 10] = is_wall
     X[:, 11] = np.sqrt(area) / D_SCALE
-    X[:, 12] = is_in * q / D_SCALE
-    X[:, 13] = is_out * (H - zc) / D_SCALE
+    if features == 1:
+        X[:, 2] = case.cd / 0.04 - 1.0
+        X[:, 3] = q / D_SCALE
+        X[:, 4] = (H - zfit) / D_SCALE
+        X[:, 5] = S * 1000.0
+        X[:, 12] = is_in * q / D_SCALE
+        X[:, 13] = is_out * (H - zc) / D_SCALE
+    elif features == 2:
+        h_out = H - a  # outlet stage over the fitted bed at the outlet (dist 0)
+        if not (h_out > 0 and S > 0):
+            raise ValueError("the warm start needs a positive outlet depth and bed slope")
+        X[:, 2] = np.log(case.cd / 0.04)
+        X[:, 3] = np.log(q / D_SCALE)
+        X[:, 4] = np.log(h_out / D_SCALE)
+        X[:, 5] = np.log(S * 1000.0)
+        X[:, 12] = S * dist / h_out
+        X[:, 13] = np.log(q * q / (9.81 * h_out ** 3))
+        X[:, 14] = (H - zfit) / D_SCALE
+    else:
+        raise ValueError("unknown feature set %r" % features)
     inter = ee[:, 1] >= 0
     i, j = ee[inter, 0], ee[inter, 1]
     src = np.concatenate([i, j]).astype(np.int32)
@@ -298,9 +321,10 @@ class WarmStart:
     """The trained network and what it needs to predict a reach's state."""
 
     def __init__(self, layers: int = 6, lr: float = 1e-3, weight_decay: float = 1e-4,
-                 seed: int = 0, params: Optional[np.ndarray] = None):
+                 seed: int = 0, params: Optional[np.ndarray] = None, features: int = FEATURES):
         from . import mpnn
         self.layers = layers
+        self.features = features
         self.net = mpnn.Net(layers=layers, f_node=F_NODE, f_edge=F_EDGE, f_out=F_OUT, lr=lr,
                             weight_decay=weight_decay, seed=seed)
         if params is not None:
@@ -413,13 +437,14 @@ class WarmStart:
 
     def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3):
         """Per case: (h, u, v) for storm_fvt_set_state."""
-        frs = [frame(c) for c in cases_]
+        frs = [frame(c, self.features) for c in cases_]
         ys = self.predict_outputs(frs, batch)
         return [decode(fr, y, h_dry) for fr, y in zip(frs, ys)]
 
     # -- checkpoint ---------------------------------------------------------
     def save(self, path: str, meta: Optional[Dict] = None):
         m = dict(layers=self.layers, f_node=F_NODE, f_edge=F_EDGE, f_out=F_OUT,
+                 features=self.features, y0_scale=Y0_SCALE, q_scale=Q_SCALE,
                  history=self.history)
         if meta:
             m.update(meta)
@@ -429,7 +454,10 @@ class WarmStart:
     def load(cls, path: str) -> "WarmStart":
         d = np.load(path, allow_pickle=False)
         meta = json.loads(str(d["meta"]))
-        ws = cls(layers=int(meta["layers"]), params=d["params"])
+        if meta.get("y0_scale", Y0_SCALE) != Y0_SCALE or meta.get("q_scale", Q_SCALE) != Q_SCALE:
+            raise ValueError("checkpoint %s was trained with other output scales" % path)
+        ws = cls(layers=int(meta["layers"]), params=d["params"],
+                 features=int(meta.get("features", 1)))
         ws.meta = meta
         return ws
 
