@@ -15,8 +15,14 @@ Prints one JSON line.  `value` is cell-updates/s (triangles x steps / s).
 `--workload ensemble` instead runs the batched ensemble (storm_amd/cases.py
 `ensemble_reach`: reaches of log-uniform size in [5k, 200k] triangles, own
 slope, roughness, discharge and stage) from a cold start to steady state
-(RMS dh <= 1e-6 in every reach) in one context, with the autotuned launch
-configuration, and reports the wall time to steady and the cell-updates/s.
+(RMS dh <= 1e-6 in every reach; a reach that diverges stops and is counted
+apart) in one context, with the autotuned launch configuration, and reports
+the wall time to steady and the cell-updates/s.
+
+`--workload warmstart` times held-out reaches (200k triangles each) to steady
+state from a cold start, from the plane guess, and from the learned warm
+start (storm_amd/warmstart.py), and reports the cold / learned wall-time
+ratio.
 `roofline` prices the step with the algorithmic bytes of one cell-update of
 SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
 residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
@@ -165,7 +171,7 @@ def ensemble(args):
     # first-order HLL: the Barth-Jespersen runs of this generator stall at an
     # RMS dh of about 1e-5, above the 1e-6 target
     opts = dict(flux="hll", limiter=0, rkorder=2, cfl=0.9, steady=True, stop_cr=1, rstop=1e-6,
-                dt_op=0, istop=args.max_steps)
+                dt_op=0, istop=args.max_steps, nonfinite_stop=True)
     best, times = tune.tune(reaches, opts, steps=4)
     opts.update(best)
     t0 = time.perf_counter()
@@ -177,7 +183,7 @@ def ensemble(args):
         b.run(1000)
         gpu_s += b.last_run_seconds()
         it, tt, rm, dn = b.status()
-        print("ensemble: %d/%d reaches converged, max step %d" % (dn.sum(), len(dn), it.max()),
+        print("ensemble: %d/%d reaches stopped, max step %d" % (dn.sum(), len(dn), it.max()),
               file=sys.stderr, flush=True)
         if dn.all() or it.max() >= args.max_steps:
             break
@@ -188,7 +194,8 @@ def ensemble(args):
         "metric": "time-to-steady (cold start), batched ensemble",
         "value": wall, "unit": "s", "n_gpus": 1, "higher_is_better": False,
         "cell_updates_per_s": updates / wall, "gpu_seconds": gpu_s,
-        "reaches": len(reaches), "triangles": int(n.sum()), "converged": int(dn.sum()),
+        "reaches": len(reaches), "triangles": int(n.sum()),
+        "converged": int((dn & np.isfinite(rm)).sum()), "diverged": int((~np.isfinite(rm)).sum()),
         "steps_max": int(it.max()), "steps_median": float(np.median(it)),
         "dtype": "f64", "data": "synthetic (storm_amd/cases.py ensemble_reach, seeds 4004+k)",
         "config": {"workload": "ensemble-%d" % len(reaches), "scheme": "HLL first order, SSP-RK2, "

@@ -33,6 +33,27 @@
 #define ONE 1.0
 #define ZERO 0.0
 static const double ONE_THIRD = 1.0 / 3.0;
+
+/* Sensitivity probe (test use only, off by default): with probability p per
+ * call, the friction's h**(1/3) is moved by one ulp, as another libm's pow
+ * may round it.  Drawn from a xorshift64 stream seeded by the caller. */
+static double sens_p = 0.0;
+static uint64_t sens_state = 1;
+void orc_set_pow_perturbation(double p, uint64_t seed) {
+  sens_p = p;
+  sens_state = seed | 1;
+}
+static double fric_pow(double h) {
+  double r = pow(h, ONE_THIRD);
+  if (sens_p > 0.0) {
+    sens_state ^= sens_state << 13;
+    sens_state ^= sens_state >> 7;
+    sens_state ^= sens_state << 17;
+    if ((double)(sens_state >> 11) * (1.0 / 9007199254740992.0) < sens_p)
+      r = nextafter(r, (sens_state & 1) ? INFINITY : -INFINITY);
+  }
+  return r;
+}
 /* REAL(mp), PARAMETER :: small = 1.0E-9 -- a default-real literal
  * (solverFVT.f90:44, deltatFVT.f90:30, flux_ac.f90:46). */
 static const double SMALL = (double)1.0e-9f;
@@ -1283,7 +1304,7 @@ static void free_flow(orc_t *s) {
 /* fric_terms.f90:24-98 (explicit Manning friction; a stage source) */
 static void fric_terms(orc_t *s) {
   for (int i = 0; i < s->N; ++i) {
-    double r = s->dry[i] ? ZERO : G * s->cd[i] * s->cd[i] / pow(s->h[i], ONE_THIRD);
+    double r = s->dry[i] ? ZERO : G * s->cd[i] * s->cd[i] / fric_pow(s->h[i]);
     r = -r * s->u_mag[i];
     s->phi[s->N + i] = s->phi[s->N + i] + r * s->u[i] * s->area[i];
     s->phi[2 * s->N + i] = s->phi[2 * s->N + i] + r * s->v[i] * s->area[i];
@@ -1373,7 +1394,7 @@ static void fric_terms_impl(orc_t *s, double delta_t) {
   const int R = s->o.rkorder;
   for (int i = 0; i < s->N; ++i) s->rough[i] = ZERO;
   for (int i = 0; i < s->N; ++i)
-    if (!s->dry[i]) s->rough[i] = G * s->cd[i] * s->cd[i] / pow(s->h[i], ONE_THIRD);
+    if (!s->dry[i]) s->rough[i] = G * s->cd[i] * s->cd[i] / fric_pow(s->h[i]);
   for (int i = 0; i < s->N; ++i)
     if (!s->dry[i]) s->rough[i] = s->h[i] / (s->h[i] + delta_t * s->rough[i] * s->u_mag[i]);
   for (int i = 0; i < s->N; ++i) { s->u[i] = s->u[i] * s->rough[i]; s->v[i] = s->v[i] * s->rough[i]; }

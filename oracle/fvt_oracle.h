@@ -61,6 +61,9 @@ typedef struct orc orc_t;
 
 orc_t *orc_create(const orc_case_t *c, const orc_opts_t *o, char *err, int errlen);
 void orc_destroy(orc_t *s);
+/* Test probe: move the friction's h**(1/3) by one ulp with probability p
+ * per call (p = 0, the default, leaves the reference arithmetic). */
+void orc_set_pow_perturbation(double p, uint64_t seed);
 /* Advance up to nsteps time steps (solverFVT.f90:129-478); stops early when a
  * steady run converges or iter reaches istop.  Returns steps taken. */
 int orc_run(orc_t *s, int nsteps, int32_t *iter, double *ttime, double *rmax);

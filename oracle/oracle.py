@@ -76,6 +76,7 @@ def lib():
         L.orc_dt_cfl.argtypes = [C.c_void_p]
         L.orc_topology.restype = C.c_int64
         L.orc_topology.argtypes = [C.c_void_p, _ip, _ip, _ip]
+        L.orc_set_pow_perturbation.argtypes = [C.c_double, C.c_uint64]
         _lib = L
     return _lib
 
@@ -202,6 +203,12 @@ class Oracle:
         t3 = np.zeros(3 * self.N, dtype=np.int32)
         lib().orc_topology(self.h, _p(ptr, C.c_int32), _p(lst, C.c_int32), _p(t3, C.c_int32))
         return ptr, lst[:n], t3.reshape(self.N, 3)
+
+
+def set_pow_perturbation(p: float, seed: int = 1):
+    """Sensitivity probe: move the friction's h**(1/3) by one ulp with
+    probability p per call, as another libm's pow may round it (0 = off)."""
+    lib().orc_set_pow_perturbation(float(p), int(seed))
 
 
 def flux(kind: str, nx, ny, hl, ul, vl, hr, ur, vr, cl=0.0, cr=0.0, h0=1e-3):

@@ -936,8 +936,7 @@ struct Lds {
 };
 
 // Face states, bed and depth for every cell of the tile and halo; vertex
-// coordinates and bed for the owned cells only.  At capacity 256 this is
-// 38.3 KB, so four blocks fit the 160 KB of a CU.
+// coordinates and bed for the owned cells only: 38.3 KB at capacity 256.
 __host__ __device__ constexpr size_t lds_bytes(int cap) {
   return sizeof(double) * (11 * (size_t)cap + 9 * (size_t)tile_tmax(cap)) + (size_t)cap;
 }
@@ -1074,10 +1073,10 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 //      last stage also implicit friction, hvalve, the residual partial sums
 //      and the next step's CFL candidate (per-tile partials, reduced in a
 //      fixed order by k_control).
-// At most 128 VGPRs (four waves per SIMD, which the shared memory also
-// allows at capacity 256); the compiler spills a few values to scratch.
+// Three waves per SIMD (about 150 VGPRs, no spills).  Forcing four (128
+// VGPRs) spills the prefetched phase-4 inputs and runs 30 % slower.
 template <int CAP>
-__global__ void __launch_bounds__(CAP) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(CAP)
 k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   extern __shared__ double lds_dyn[];
   __shared__ double s_at[MAX_STRINGS];
@@ -1104,6 +1103,25 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, ri, __builtin_ctzll(qm), L, s_at, s_null,
                     s_skip);
 
+  // phase-4 inputs of an owned cell, loaded now so their latency overlaps
+  // phase 3: its rk slots 0 and s-1 (the only ones with a nonzero alpha in
+  // the SSP tables), metadata, local neighbour ids and cell record
+  double pf_h0 = ZERO, pf_u0 = ZERO, pf_v0 = ZERO, pf_h1 = ZERO, pf_u1 = ZERO, pf_v1 = ZERO;
+  uint64_t pf_meta = 0, pf_ln = 0;
+  CellS pf_cs{};
+  if (t < T) {
+    const int i = c0 + t;
+    const double* s0 = st.slot[0];
+    pf_h0 = s0[i]; pf_u0 = s0[N + i]; pf_v0 = s0[2 * N + i];
+    if (st.s >= 2) {
+      const double* s1 = st.slot[st.s - 1];
+      pf_h1 = s1[i]; pf_u1 = s1[N + i]; pf_v1 = s1[2 * N + i];
+    }
+    pf_meta = g.cmeta[i];
+    pf_ln = tl.cln[i];
+    pf_cs = ld(&g.cs[i]);
+  }
+
   // 3. face states of the tile and halo cells
   Faces F;
   if (t < T + H) {
@@ -1112,7 +1130,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     uint64_t meta;
     if (own) {
       i = c0 + t;
-      meta = g.cmeta[i];
+      meta = pf_meta;
       nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
       v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
     } else {
@@ -1140,12 +1158,14 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     if (own && F.fix) {
       st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1) (hgradfix2.f90:80-81)
       st.hin[2 * N + i] = ZERO;
+      if (st.s >= 2) { pf_u1 = ZERO; pf_v1 = ZERO; }  // the same values, prefetched
+      else { pf_u0 = ZERO; pf_v0 = ZERO; }
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       L.fh[j * CAP + t] = F.fh[j]; L.fu[j * CAP + t] = F.fu[j]; L.fv[j * CAP + t] = F.fv[j];
     }
-    L.z[t] = g.cs[i].z;
+    L.z[t] = own ? pf_cs.z : g.cs[i].z;
     L.h[t] = F.h;
     L.dry[t] = F.dry;
   }
@@ -1155,9 +1175,9 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX, topq = ZERO, botq = ZERO;
   if (t < T) {
     const int i = c0 + t;
-    const uint64_t meta = g.cmeta[i];
-    const uint64_t ln = tl.cln[i];
-    const CellS si = ld(&g.cs[i]);
+    const uint64_t meta = pf_meta;
+    const uint64_t ln = pf_ln;
+    const CellS si = pf_cs;
     Tri tr;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1198,7 +1218,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     }
     if (o.friction == 1) {
       // fric_terms.f90:24-98 (explicit Manning), after the edge fluxes
-      double r = dry_i ? ZERO : G * si.cd * si.cd / pow(hi, ONE_THIRD);
+      double r = dry_i ? ZERO : G * si.cd * si.cd / pow_one_third(hi);
       r = -r * umag_in;
       p1 = p1 + r * F.up * area;
       p2 = p2 + r * F.vp * area;
@@ -1245,16 +1265,18 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       delt = o.cfl * area / (d * perim);
     }
     const double dt = delt / area;
+    // the SSP combination: slots 0 and s-1 in ascending order (the slots in
+    // between have alpha 0 in every table: rk_table)
     double qh = ZERO, qu = ZERO, qv = ZERO;
-    for (int jj = 0; jj < st.s; ++jj) {
-      const double a = st.alpha[jj];
-      if (a == ZERO) continue;
-      const double* sl = st.slot[jj];
-      qh = qh + a * sl[i];
-      qu = qu + a * sl[N + i];
-      qv = qv + a * sl[2 * N + i];
+    if (st.alpha[0] != ZERO) {
+      const double a = st.alpha[0];
+      qh = qh + a * pf_h0; qu = qu + a * pf_u0; qv = qv + a * pf_v0;
     }
-    const double h0n = st.slot[0][i];  // h at the start of the step (residual)
+    if (st.s >= 2 && st.alpha[st.s - 1] != ZERO) {
+      const double a = st.alpha[st.s - 1];
+      qh = qh + a * pf_h1; qu = qu + a * pf_u1; qv = qv + a * pf_v1;
+    }
+    const double h0n = pf_h0;  // h at the start of the step (residual)
     qh = qh + st.beta * dt * p0;
     qu = (qu + st.beta * dt * p1);
     qv = (qv + st.beta * dt * p2);
@@ -1287,7 +1309,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       double rough = ZERO;
       if (!dpost) {
         const double fdt = o.steady ? s_dlast : c->dt;
-        rough = G * si.cd * si.cd / pow(hn, ONE_THIRD);
+        rough = G * si.cd * si.cd / pow_one_third(hn);
         rough = hn / (hn + fdt * rough * um);
       }
       if (o.friction == 0) {

@@ -255,3 +255,28 @@ def test_stop_cr2_converges_geometrically_with_a_stage_inflow():
     o = O.Oracle(c)
     r = [o.run(2000)["rmax"] for _ in range(5)]
     assert all(b < 0.5 * a for a, b in zip(r[1:], r[2:]))
+
+
+def test_full_flume_run_amplifies_one_ulp_pow_differences():
+    """The 20,000-triangle flume (HLL, SSP-RK2, CFL 0.9) is sensitive to
+    rounding: moving 1 in 1,000 of the friction's h**(1/3) results by one ulp
+    (another libm's pow rounds that way now and then) leaves h untouched
+    through 400 steps and moves it by far more than 1e-10 by step 2,000.
+    This bounds what any implementation without the reference's own libm can
+    match over the full run."""
+    import oracle as O
+    c = cases.flume(flux="hll")
+    out = {}
+    for p in (0.0, 1e-3):
+        O.set_pow_perturbation(p, 12345)
+        try:
+            o = O.Oracle(c, dict(istop=2000))
+            o.run(400)
+            h400 = o.state()["h"].copy()
+            o.run(1600)
+            out[p] = (h400, o.state()["h"].copy())
+        finally:
+            O.set_pow_perturbation(0.0)
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+    assert rel(out[1e-3][0], out[0.0][0]) <= 1e-12
+    assert rel(out[1e-3][1], out[0.0][1]) > 1e-6
