@@ -33,10 +33,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int):
-    """Steady states of training reaches 0..n-1 (None where not converged)."""
+def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int,
+             gen_time: float = 0.0):
+    """Steady states of training reaches 0..n-1 (None where not converged).
+    gen_time > 0: no further batch is started once that many seconds have
+    passed, so fewer than n reaches may come back."""
     states, steps, walls = [], [], []
+    t0 = time.perf_counter()
     for s in range(0, n_reaches, gen_batch):
+        if gen_time > 0 and time.perf_counter() - t0 > gen_time:
+            log("generation time limit reached after %d reaches" % s)
+            break
         ks = list(range(s, min(n_reaches, s + gen_batch)))
         rs = [W.training_reach(k) for k in ks]
         init = [W.baseline_state(c) for c in rs]
@@ -57,6 +64,10 @@ def main(argv=None):
     ap.add_argument("--gen-batch", type=int, default=128)
     ap.add_argument("--rstop", type=float, default=1e-8)
     ap.add_argument("--max-steps", type=int, default=150000)
+    ap.add_argument("--gen-time", type=float, default=0.0,
+                    help="start no new generation batch after this many seconds (0: no limit)")
+    ap.add_argument("--train-time", type=float, default=0.0,
+                    help="fit the AdamW updates into this many seconds (0: no limit)")
     ap.add_argument("--steps", type=int, default=4000, help="AdamW updates")
     ap.add_argument("--reaches-per-step", type=int, default=32)
     ap.add_argument("--window", type=int, default=4096, help="cells cut from each reach")
@@ -71,7 +82,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
 
     t0 = time.perf_counter()
-    states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps)
+    states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps, a.gen_time)
     gen_s = time.perf_counter() - t0
     ks = [k for k, st in enumerate(states) if st is not None]
     sets = [int(f) for f in a.features.split(",")]
@@ -91,15 +102,16 @@ def main(argv=None):
         val0 = ws.loss(va_f, va_y)
         t1 = time.perf_counter()
         ws.train_windows(tr_f, tr_y, a.steps, reaches_per_step=a.reaches_per_step,
-                         size=a.window, seed=a.seed, log=log, lr=a.lr, lr_final=a.lr_final)
+                         size=a.window, seed=a.seed, log=log, lr=a.lr, lr_final=a.lr_final,
+                         time_limit=a.train_time or None)
         train_s = time.perf_counter() - t1
         val1 = ws.loss(va_f, va_y)
         zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
         meta = dict(
-            reaches=a.reaches, converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
+            reaches=len(states), converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
             train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
             steps_median=float(np.median([steps[k] for k in ks])),
-            generate_s=gen_s, train_s=train_s, steps=a.steps,
+            generate_s=gen_s, train_s=train_s, steps=ws.history[-1]["step"],
             reaches_per_step=a.reaches_per_step, window=a.window, lr=a.lr,
             lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1,
             val_loss_of_zero_output=zero,

@@ -365,13 +365,16 @@ class WarmStart:
 
     def train_windows(self, frames: Sequence[Frame], ys: Sequence[np.ndarray], steps: int,
                       reaches_per_step: int = 32, size: int = 4096, seed: int = 0, log=None,
-                      lr: float = 1e-3, lr_final: float = 1e-5, log_every: int = 100):
+                      lr: float = 1e-3, lr_final: float = 1e-5, log_every: int = 100,
+                      time_limit: Optional[float] = None):
         """AdamW over batches that each join windows of `size` cells cut from
         `reaches_per_step` different reaches (sampled without replacement),
         so every update sees many reaches' parameters at once.  The next
         batch is assembled on a host thread while the device runs the
         current one.  The learning rate decays from lr to lr_final on a
-        cosine over the steps."""
+        cosine over the steps.  time_limit (s): after the first 50 updates
+        the step count is lowered, if needed, to what fits in that time at
+        the measured rate, and the cosine runs over the lowered count."""
         from concurrent.futures import ThreadPoolExecutor
         rng = np.random.default_rng(seed)
         n = len(frames)
@@ -392,7 +395,14 @@ class WarmStart:
         tot, cnt = 0.0, 0
         with ThreadPoolExecutor(1) as ex:
             nxt = ex.submit(batch)
-            for step in range(steps):
+            step = 0
+            while step < steps:
+                if time_limit and step == 50:
+                    fit = int(50 * time_limit / max(time.perf_counter() - t0, 1e-9))
+                    if fit < steps:
+                        steps = max(fit, 51)
+                        if log:
+                            log("time limit: %d updates" % steps)
                 src, dst, X, EF, T, w = nxt.result()
                 if step + 1 < steps:
                     nxt = ex.submit(batch)
@@ -409,6 +419,7 @@ class WarmStart:
                     if log:
                         log("step %d loss %.5f (%.1f s)" % (rec["step"], rec["loss"], rec["seconds"]))
                     tot, cnt = 0.0, 0
+                step += 1
         return self.history
 
     def loss(self, frames: Sequence[Frame], ys: Sequence[np.ndarray]) -> float:
