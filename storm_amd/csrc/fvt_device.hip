@@ -1085,11 +1085,42 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   const int4 hd0 = tl.hdr[2 * blockIdx.x], hd1 = tl.hdr[2 * blockIdx.x + 1];
   const int c0 = hd0.x, T = hd0.y, hb = hd0.z, H = hd0.w, b = hd1.z;
   const Ctrl* c = &S.ctrl[hd1.w];
+  const int t = threadIdx.x;
+  const int N = g.N;
+
+  // Loads that need only the tile header are issued first, so their latency
+  // overlaps the wait for the reach's control block and the inflow strings:
+  // an owned cell's phase-4 inputs (its rk slots 0 and s-1, the only ones
+  // with a nonzero alpha in the SSP tables, metadata, local neighbour ids
+  // and cell record) and, for every tile and halo cell, the neighbour and
+  // vertex ids phase 3 gathers through.
+  double pf_h0 = ZERO, pf_u0 = ZERO, pf_v0 = ZERO, pf_h1 = ZERO, pf_u1 = ZERO, pf_v1 = ZERO;
+  uint64_t pf_meta = 0, pf_ln = 0;
+  CellS pf_cs{};
+  int ci = 0, nb0 = -1, nb1 = -1, nb2 = -1, v0 = 0, v1 = 0, v2 = 0;
+  if (t < T) {
+    const int i = c0 + t;
+    const double* s0 = st.slot[0];
+    pf_h0 = s0[i]; pf_u0 = s0[N + i]; pf_v0 = s0[2 * N + i];
+    if (st.s >= 2) {
+      const double* s1 = st.slot[st.s - 1];
+      pf_h1 = s1[i]; pf_u1 = s1[N + i]; pf_v1 = s1[2 * N + i];
+    }
+    pf_meta = g.cmeta[i];
+    pf_ln = tl.cln[i];
+    pf_cs = ld(&g.cs[i]);
+    ci = i;
+    nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
+    v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
+  } else if (t < T + H) {
+    const int4 r0 = tl.hrec[2 * (hb + t - T)], r1 = tl.hrec[2 * (hb + t - T) + 1];
+    ci = r0.x; nb0 = r0.y; nb1 = r0.z; nb2 = r0.w;
+    v0 = r1.x; v1 = r1.y; v2 = r1.z;
+    pf_meta = tl.hmeta[hb + t - T];
+  }
   if (c->done) return;
   const RInfo ri = g.reach[hd1.w];
   const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
-  const int t = threadIdx.x;
-  const int N = g.N;
   const Lds L = lds_layout(lds_dyn, CAP);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (t == 0 && o.steady && st.last) {
@@ -1103,42 +1134,12 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, ri, __builtin_ctzll(qm), L, s_at, s_null,
                     s_skip);
 
-  // phase-4 inputs of an owned cell, loaded now so their latency overlaps
-  // phase 3: its rk slots 0 and s-1 (the only ones with a nonzero alpha in
-  // the SSP tables), metadata, local neighbour ids and cell record
-  double pf_h0 = ZERO, pf_u0 = ZERO, pf_v0 = ZERO, pf_h1 = ZERO, pf_u1 = ZERO, pf_v1 = ZERO;
-  uint64_t pf_meta = 0, pf_ln = 0;
-  CellS pf_cs{};
-  if (t < T) {
-    const int i = c0 + t;
-    const double* s0 = st.slot[0];
-    pf_h0 = s0[i]; pf_u0 = s0[N + i]; pf_v0 = s0[2 * N + i];
-    if (st.s >= 2) {
-      const double* s1 = st.slot[st.s - 1];
-      pf_h1 = s1[i]; pf_u1 = s1[N + i]; pf_v1 = s1[2 * N + i];
-    }
-    pf_meta = g.cmeta[i];
-    pf_ln = tl.cln[i];
-    pf_cs = ld(&g.cs[i]);
-  }
-
   // 3. face states of the tile and halo cells
   Faces F;
   if (t < T + H) {
     const bool own = t < T;
-    int i, nb0, nb1, nb2, v0, v1, v2;
-    uint64_t meta;
-    if (own) {
-      i = c0 + t;
-      meta = pf_meta;
-      nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
-      v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
-    } else {
-      const int4 r0 = tl.hrec[2 * (hb + t - T)], r1 = tl.hrec[2 * (hb + t - T) + 1];
-      i = r0.x; nb0 = r0.y; nb1 = r0.z; nb2 = r0.w;
-      v0 = r1.x; v1 = r1.y; v2 = r1.z;
-      meta = tl.hmeta[hb + t - T];
-    }
+    const int i = ci;
+    const uint64_t meta = pf_meta;
     Tri tr;
     {
       const double4 q0 = g.vxyz[v0], q1 = g.vxyz[v1], q2 = g.vxyz[v2];
@@ -1359,55 +1360,47 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     }
   }
   if (!st.last) return;
-  // per-tile partials: fixed tree over the block
-  __syncthreads();
-  double* r0 = L.fh;
-  double* r1 = L.fh + CAP;
-  double* r2 = L.fh + 2 * CAP;
-  double* r3 = L.fh + 3 * CAP;
-  double* r4 = L.fh + 4 * CAP;
+  // per-tile partials: a fixed butterfly within each wave (every lane ends
+  // with the same bits), then the waves' results in wave order
   const bool q2 = o.stop_cr == 2;
-  r0[t] = top; r1[t] = bot; r2[t] = mxphi;
-  if (q2) { r3[t] = topq; r4[t] = botq; }
+  double r0 = top, r1 = bot, r2 = mxphi, r3 = cand, r4 = topq, r5 = botq;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    r0 = r0 + __shfl_xor(r0, off);
+    r1 = r1 + __shfl_xor(r1, off);
+    r2 = dmax(r2, __shfl_xor(r2, off));
+    r3 = dmin(r3, __shfl_xor(r3, off));
+    if (q2) {
+      r4 = r4 + __shfl_xor(r4, off);
+      r5 = r5 + __shfl_xor(r5, off);
+    }
+  }
+  constexpr int NW = CAP / 64;
+  __shared__ double s_red[NW][PART];
+  if ((t & 63) == 0) {
+    double* q = s_red[t >> 6];
+    q[0] = r0; q[1] = r1; q[2] = r2; q[3] = r3; q[4] = r4; q[5] = r5;
+  }
   __syncthreads();
-  constexpr int P2 = pow2_floor(CAP);
-  if (P2 < CAP) {
-    if (t < CAP - P2) {
-      r0[t] = r0[t] + r0[t + P2];
-      r1[t] = r1[t] + r1[t + P2];
-      r2[t] = dmax(r2[t], r2[t + P2]);
-      if (q2) { r3[t] = r3[t] + r3[t + P2]; r4[t] = r4[t] + r4[t + P2]; }
-    }
-    __syncthreads();
-  }
-  for (int w = P2 / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      r0[t] = r0[t] + r0[t + w];
-      r1[t] = r1[t] + r1[t + w];
-      r2[t] = dmax(r2[t], r2[t + w]);
-      if (q2) { r3[t] = r3[t] + r3[t + w]; r4[t] = r4[t] + r4[t + w]; }
-    }
-    __syncthreads();
-  }
   if (t == 0) {
-    S.part[PART * b] = r0[0];
-    S.part[PART * b + 1] = r1[0];
-    S.part[PART * b + 2] = r2[0];
-    S.part[PART * b + 4] = q2 ? r3[0] : ZERO;
-    S.part[PART * b + 5] = q2 ? r4[0] : ZERO;
+    double a0 = s_red[0][0], a1 = s_red[0][1], a2 = s_red[0][2], a3 = s_red[0][3];
+    double a4 = s_red[0][4], a5 = s_red[0][5];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      a0 = a0 + s_red[w][0];
+      a1 = a1 + s_red[w][1];
+      a2 = dmax(a2, s_red[w][2]);
+      a3 = dmin(a3, s_red[w][3]);
+      a4 = a4 + s_red[w][4];
+      a5 = a5 + s_red[w][5];
+    }
+    S.part[PART * b] = a0;
+    S.part[PART * b + 1] = a1;
+    S.part[PART * b + 2] = a2;
+    S.part[PART * b + 3] = a3;
+    S.part[PART * b + 4] = q2 ? a4 : ZERO;
+    S.part[PART * b + 5] = q2 ? a5 : ZERO;
   }
-  __syncthreads();
-  r0[t] = cand;
-  __syncthreads();
-  if (P2 < CAP) {
-    if (t < CAP - P2) r0[t] = dmin(r0[t], r0[t + P2]);
-    __syncthreads();
-  }
-  for (int w = P2 / 2; w > 0; w >>= 1) {
-    if (t < w) r0[t] = dmin(r0[t], r0[t + w]);
-    __syncthreads();
-  }
-  if (t == 0) S.part[PART * b + 3] = r0[0];
 }
 
 // to_vertices' cell pass alone (solverFVT.f90:483 -> to_vertices.f90:44-55),
