@@ -195,8 +195,9 @@ __device__ __forceinline__ Side side_of(double xa, double ya, double za, double 
   Side s;
   const double dx = xa - xb, dy = ya - yb;
   s.len = sqrt(dx * dx + dy * dy);
-  s.nx = (yb - ya) / s.len;
-  s.ny = (xa - xb) / s.len;
+  const Recip rl = recip(s.len, yb - ya);
+  s.nx = qdiv(rl, yb - ya, s.len);
+  s.ny = qdiv(rl, xa - xb, s.len);
   s.tx = -s.ny;  // (xb - xa) / len, the same value
   s.ty = s.nx;   // (yb - ya) / len
   s.mx = HALF * (xa + xb);  // dataprepFVT.f90:168-169
@@ -643,9 +644,10 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
         }
       }
       const double d = a * cc - b * b;
-      gzx = (cc * dz - b * ez) / d; gzy = (a * ez - dz * b) / d;
-      gux = (cc * du - b * eu) / d; guy = (a * eu - du * b) / d;
-      gvx = (cc * dv - b * ev) / d; gvy = (a * ev - dv * b) / d;
+      const Recip rd = recip(d, cc * dz - b * ez);
+      gzx = qdiv(rd, cc * dz - b * ez, d); gzy = qdiv(rd, a * ez - dz * b, d);
+      gux = qdiv(rd, cc * du - b * eu, d); guy = qdiv(rd, a * eu - du * b, d);
+      gvx = qdiv(rd, cc * dv - b * ev, d); gvy = qdiv(rd, a * ev - dv * b, d);
     }
   }
   double rcx[3], rcy[3];
@@ -668,10 +670,12 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
   }
   // bed_slope_VTX.f90:32-33
   const double area = tri_area(tr);
-  const double dzx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
-                             tr.zv[2] * (tr.y[0] - tr.y[1])) / area;
-  const double dzy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
-                             tr.zv[2] * (tr.x[1] - tr.x[0])) / area;
+  const double nzx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                             tr.zv[2] * (tr.y[0] - tr.y[1]));
+  const double nzy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                             tr.zv[2] * (tr.x[1] - tr.x[0]));
+  const Recip ra = recip(area, nzx);
+  const double dzx = qdiv(ra, nzx, area), dzy = qdiv(ra, nzy, area);
   double dhx = gzx - dzx, dhy = gzy - dzy;
   double up = ui, vp = vi;
   bool fix = false;
@@ -936,7 +940,8 @@ struct Lds {
 };
 
 // Face states, bed and depth for every cell of the tile and halo; vertex
-// coordinates and bed for the owned cells only: 38.3 KB at capacity 256.
+// coordinates and bed for the owned cells only: 40.3 KB at capacity 256
+// (three blocks per CU, as many as the registers allow).
 __host__ __device__ constexpr size_t lds_bytes(int cap) {
   return sizeof(double) * (11 * (size_t)cap + 9 * (size_t)tile_tmax(cap)) + (size_t)cap;
 }
@@ -1248,10 +1253,13 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
         gx = a * (bb0 * zp[0] + bb1 * zp[1] + bb2 * zp[2]);
         gy = a * (cc0 * zp[0] + cc1 * zp[1] + cc2 * zp[2]);
       } else {
-        gx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
-                     tr.zv[2] * (tr.y[0] - tr.y[1])) / area;
-        gy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
-                     tr.zv[2] * (tr.x[1] - tr.x[0])) / area;
+        const double ngx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                                   tr.zv[2] * (tr.y[0] - tr.y[1]));
+        const double ngy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                                   tr.zv[2] * (tr.x[1] - tr.x[0]));
+        const Recip ra = recip(area, ngx);
+        gx = qdiv(ra, ngx, area);
+        gy = qdiv(ra, ngy, area);
       }
       p1 = p1 - G * hi * gx * area;
       p2 = p2 - G * hi * gy * area;

@@ -41,6 +41,37 @@ SD double pow_one_third(double h) {
   return c + (lo + c * t);
 }
 
+// Quotients that share a denominator d, each with the bits of the plain
+// IEEE division n / d.  The compiler's gfx950 division is: d scaled by
+// v_div_scale (a no-op unless n / d is near overflow or underflow), its
+// reciprocal refined by two Newton steps, n scaled, q = n * r, one
+// correction fma, v_div_fmas and v_div_fixup.  `recip` runs the part that
+// depends on d alone once; `qdiv` runs the rest per numerator, after checking
+// that this numerator scales d the same way (otherwise it divides the plain
+// way).  Same instructions on the same operands, so the same bits.
+struct Recip {
+  double ds, r;  // scaled denominator, refined reciprocal
+};
+SD Recip recip(double d, double n) {
+  bool f;
+  const double ds = __builtin_amdgcn_div_scale(n, d, false, &f);
+  double r = __builtin_amdgcn_rcp(ds);
+  double e = fma(-ds, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-ds, r, 1.0);
+  r = fma(r, e, r);
+  return Recip{ds, r};
+}
+SD double qdiv(const Recip& R, double n, double d) {
+  bool f, scaled;
+  const double ds = __builtin_amdgcn_div_scale(n, d, false, &f);
+  if (__double_as_longlong(ds) != __double_as_longlong(R.ds)) return n / d;
+  const double ns = __builtin_amdgcn_div_scale(n, d, true, &scaled);
+  const double q = ns * R.r;
+  const double rem = fma(-R.ds, q, ns);
+  return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem, R.r, q, scaled), d, n);
+}
+
 SD double dmin(double a, double b) { return (a < b) ? a : b; }
 SD double dmax(double a, double b) { return (a > b) ? a : b; }
 
@@ -66,8 +97,10 @@ SD void flux_ac(double nx, double ny, double hl, double ul, double vl, double hr
   hr = dmax(hr, SMALL);
   const double hrs = sqrt(hr);
   const double den = hrs + hls;
-  const double ua = (ur * hrs + ul * hls) / den;
-  const double va = (vr * hrs + vl * hls) / den;
+  const double nua = ur * hrs + ul * hls, nva = vr * hrs + vl * hls;
+  const Recip rden = recip(den, nua);
+  const double ua = qdiv(rden, nua, den);
+  const double va = qdiv(rden, nva, den);
   const double ca = sqrt(HALF * G * (hl + hr));
   const double q0 = hr - hl, q1 = hr * ur - hl * ul, q2 = hr * vr - hl * vl;
   // right eigenvectors (columns) times |lambda|
@@ -78,9 +111,13 @@ SD void flux_ac(double nx, double ny, double hl, double ul, double vl, double hr
   const double t10 = ny * e0, t11 = (ua - ca * nx) * e1, t12 = (ua + ca * nx) * e2;
   const double t20 = (-nx) * e0, t21 = (va - ca * ny) * e1, t22 = (va + ca * ny) * e2;
   // left eigenvectors (rows)
+  // HALF * x / ca is (HALF * x) / ca: three quotients by ca
+  const double n1 = HALF * (ua * nx + va * ny), n2 = HALF * nx, n3 = HALF * ny;
+  const Recip rca = recip(ca, n1);
+  const double c1 = qdiv(rca, n1, ca), c2 = qdiv(rca, n2, ca), c3 = qdiv(rca, n3, ca);
   const double l00 = -(ua * ny - va * nx), l01 = ny, l02 = -nx;
-  const double l10 = HALF * (ua * nx + va * ny) / ca + HALF, l11 = -(HALF * nx / ca), l12 = -(HALF * ny / ca);
-  const double l20 = -(HALF * (ua * nx + va * ny) / ca) + HALF, l21 = HALF * nx / ca, l22 = HALF * ny / ca;
+  const double l10 = c1 + HALF, l11 = -c2, l12 = -c3;
+  const double l20 = -c1 + HALF, l21 = c2, l22 = c3;
   const double a00 = t00 * l00 + t01 * l10 + t02 * l20, a01 = t00 * l01 + t01 * l11 + t02 * l21,
                a02 = t00 * l02 + t01 * l12 + t02 * l22;
   const double a10 = t10 * l00 + t11 * l10 + t12 * l20, a11 = t10 * l01 + t11 * l11 + t12 * l21,

@@ -39,11 +39,13 @@ constexpr uint64_t CMS_KIND = 0xF;
 constexpr int CMS_NL = 4, CMS_LEFT = 6, CMS_HBAR = 7, CMS_REV = 8, CMS_CHAIN = 9;
 
 constexpr int MAX_STRINGS = 64;  // boundary strings of each kind per reach
-// Most cells a tile of capacity cap (cells + halo) owns: whole waves of 64,
-// about three quarters of the capacity (192 of 256).  The last wave of a
-// block then holds only halo cells and skips the per-cell update, and the
-// stage kernel sizes the owned cells' vertex arrays in shared memory by it.
-constexpr int tile_tmax(int cap) { return (cap * 3 / 4) / 64 * 64; }
+// Most cells a tile of capacity cap (cells + halo) owns: the whole capacity,
+// so a tile's halo cells share its last wave with owned cells.  (Capping the
+// owned cells at whole waves, 192 of 256, left the last wave idle in the
+// per-cell update and needed 4 % more tiles; at three waves per SIMD that was
+// the slower choice.)  The stage kernel sizes the owned cells' vertex arrays
+// in shared memory by it.
+constexpr int tile_tmax(int cap) { return cap; }
 
 // A batch of independent reaches laid end to end in one mesh: reach r owns
 // input cells [cell_ptr[r], cell_ptr[r+1]), inflow strings [in_ptr[r],
