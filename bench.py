@@ -26,8 +26,13 @@ ratio.
 `roofline` prices the step with the algorithmic bytes of one cell-update of
 SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
 residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
-times the CPU oracle (a serial C restatement of the reference step) on the
-same reach for a bounded number of steps.
+times the reference Fortran itself (oracle/_ref/ref_driver, built from the
+reference's sources with flang -O2, serial, as the reference runs) on a
+bounded sample of the same workload: the meander generator cut to 4 km
+(400,000 triangles, the same cross-section, cell size and bump density) for
+20 steps, timing its solverFVT call alone.  Where that build is absent it
+times the CPU oracle (a serial C restatement of the reference step) instead;
+the oracle's figure on 10 steps of the full 2M reach is reported beside it.
 """
 import argparse
 import json
@@ -43,6 +48,36 @@ sys.path.insert(0, ROOT)
 B_STEP_RK3 = 1248.0          # algorithmic bytes per cell-update, SSP-RK3
 PEAK_HBM = 8.0e12            # B/s, MI355X
 METRIC = "FP64 cell-updates/s + HBM % roofline; time-to-steady warm vs cold"
+
+
+REF_SAMPLE = dict(ns=800, length=4000.0, n_bumps=40)  # meander cut to 4 km, 400k triangles
+REF_STEPS = 20
+
+
+def cpu_reference(steps=REF_STEPS):
+    """The flang-built reference on the bounded meander sample: cell-updates
+    per second of its solverFVT call, and that call's seconds."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the baseline only
+    from storm_amd import cases
+    if not os.path.exists(O.REF_DRIVER):
+        print("bench.py: %s is not built; timing the oracle only" % O.REF_DRIVER,
+              file=sys.stderr)
+        return None
+    case = cases.meander(istop=steps, **REF_SAMPLE)
+    with tempfile.TemporaryDirectory() as d:
+        run = cases.write_run(case, d)
+        r = subprocess.run([O.REF_DRIVER, os.path.basename(run), "out.bin"], cwd=d,
+                           capture_output=True, text=True, timeout=600)
+        t = [float(l.split()[-1]) for l in r.stdout.splitlines()
+             if l.startswith("solverFVT seconds")]
+        if r.returncode != 0 or not t:
+            print("bench.py: the reference run failed (exit %d): %s" %
+                  (r.returncode, (r.stdout + r.stderr)[-2000:]), file=sys.stderr)
+            return None
+    return case.n_elems * steps / t[0], t[0]
 
 
 def cpu_baseline(case, steps):
@@ -74,8 +109,8 @@ def traffic_from_profiles(workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
@@ -155,10 +190,20 @@ def main():
     }
     if not args.no_cpu:
         cpu_rate, cpu_t = cpu_baseline(case, args.cpu_steps)
-        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": 1,
-                               "kind": "port",
-                               "sample": "%d SSP-RK3 steps of the same 2M-triangle reach "
-                                         "(%.1f s), serial C oracle" % (args.cpu_steps, cpu_t)}
+        port = {"value": cpu_rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                "sample": "%d SSP-RK3 steps of the same 2M-triangle reach (%.1f s), serial C "
+                          "oracle" % (args.cpu_steps, cpu_t)}
+        ref = cpu_reference()
+        if ref is None:
+            out["cpu_baseline"] = port
+        else:
+            out["cpu_baseline"] = {
+                "value": ref[0], "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": "reference Fortran (flang -O2, serial), solverFVT call alone: %d "
+                          "SSP-RK3 steps of the meander generator cut to 4 km (%d triangles, "
+                          "same cross-section, cell size and bump density), %.1f s"
+                          % (REF_STEPS, 400000, ref[1]),
+                "port": port}
     print(json.dumps(out))
 
 
@@ -226,9 +271,12 @@ def warmstart(args):
         init, pred_s = None, 0.0
         if mode == "plane":
             init = [W.baseline_state(c) for c in reaches]
-        elif mode == "learned":
+        elif mode in ("learned", "learned_surface"):
+            # learned_surface: the network's water surface with the plane's
+            # uniform discharge (warmstart.decode)
             ws = W.WarmStart.load(ckpt)
-            init = ws.predict(reaches)
+            init = ws.predict(reaches,
+                              discharge="uniform" if mode == "learned_surface" else "learned")
             ws.close()
         pred_s = time.perf_counter() - t0
         out, _ = W.steady_states(reaches, opts, args.max_steps, chunk=1000, init=init,
@@ -242,7 +290,7 @@ def warmstart(args):
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
                          steps_sum=int(steps.sum()), steps=steps.tolist(),
                          cell_updates=float(sum(c.n_elems * it for c, it in zip(reaches, steps))))
-        if mode == "learned":
+        if mode in ("learned", "learned_surface"):
             # prediction error against the state the solver converged to
             eh, eq = [], []
             for (st, _), (h, u, v) in zip(out, init):
@@ -270,6 +318,7 @@ def warmstart(args):
                    "scheme": "HLL first order, SSP-RK2, steady local dt CFL 0.9, stop RMS dh",
                    "checkpoint": os.path.relpath(ckpt, ROOT), "generate_s": round(gen_s, 1)},
         "cold": res.get("cold"), "plane": res.get("plane"), "learned": res.get("learned"),
+        "learned_surface": res.get("learned_surface"),
         "plane_over_cold": wall("cold") / wall("plane") if wall("cold") and wall("plane") else None,
         "h_rms_learned_vs_cold_final_m": float(np.median(dd)) if dd else None,
     }

@@ -9,6 +9,9 @@
 !
 ! Usage: ref_driver <case.run> <out.bin>
 !
+! The wall time of the solverFVT call alone is printed on standard output as
+! "solverFVT seconds <t>" (the CPU baseline of the benchmark reads it).
+!
 ! Output layout (stream access, native endianness):
 !   int32  n_elems, n_pts, iter, rkorder
 !   real64 ttime, rmax
@@ -28,6 +31,7 @@ PROGRAM ref_driver
   CHARACTER (LEN=250) :: fname, oname
   CHARACTER (LEN=160) :: title
   INTEGER :: iter, i, u_out
+  INTEGER (KIND=8) :: c0, c1, crate
   INTEGER, ALLOCATABLE :: dryi(:)
   REAL (KIND=mp) :: ttime, rmax
 
@@ -36,7 +40,10 @@ PROGRAM ref_driver
   cgns = .FALSE.
   CALL read_mfa(fname, title)
   CALL dataprepFVT
+  CALL SYSTEM_CLOCK(c0, crate)
   CALL solverFVT(iter, ttime, rmax, fname, title)
+  CALL SYSTEM_CLOCK(c1)
+  WRITE (*, '(A,ES16.8)') 'solverFVT seconds ', REAL(c1 - c0, mp) / REAL(crate, mp)
 
   ALLOCATE(dryi(n_elems))
   DO i = 1, n_elems

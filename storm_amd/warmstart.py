@@ -208,13 +208,18 @@ def targets(fr: Frame, h, qu, qv) -> np.ndarray:
     return T.astype(np.float32)
 
 
-def decode(fr: Frame, y: np.ndarray, h_dry: float = 1e-3):
-    """(h, u, v) of network outputs y (N, >=3)."""
+def decode(fr: Frame, y: np.ndarray, h_dry: float = 1e-3, discharge: str = "learned"):
+    """(h, u, v) of network outputs y (N, >=3).  discharge="uniform" keeps
+    the learned water surface but takes the plane's uniform discharge along
+    the axis in place of the learned one (outputs 1 and 2 read as zero)."""
     y = np.asarray(y, dtype=np.float64)
     zeta = fr.H + fr.S * fr.dist + Y0_SCALE * y[:, 0]
     h = np.maximum(zeta - fr.zc, 0.0)
-    qa = fr.q * (1.0 + Q_SCALE * y[:, 1])
-    qc = fr.q * Q_SCALE * y[:, 2]
+    if discharge not in ("learned", "uniform"):
+        raise ValueError("discharge must be 'learned' or 'uniform'")
+    lq = discharge == "learned"
+    qa = fr.q * (1.0 + Q_SCALE * y[:, 1]) if lq else np.full_like(h, fr.q)
+    qc = fr.q * Q_SCALE * y[:, 2] if lq else np.zeros_like(h)
     hu = qa * fr.e[0] - qc * fr.e[1]
     hv = qa * fr.e[1] + qc * fr.e[0]
     wet = h > h_dry
@@ -446,11 +451,13 @@ class WarmStart:
                 o += n
         return out
 
-    def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3):
-        """Per case: (h, u, v) for storm_fvt_set_state."""
+    def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3,
+                discharge: str = "learned"):
+        """Per case: (h, u, v) for storm_fvt_set_state (``decode``'s
+        discharge choice)."""
         frs = [frame(c, self.features) for c in cases_]
         ys = self.predict_outputs(frs, batch)
-        return [decode(fr, y, h_dry) for fr, y in zip(frs, ys)]
+        return [decode(fr, y, h_dry, discharge) for fr, y in zip(frs, ys)]
 
     # -- checkpoint ---------------------------------------------------------
     def save(self, path: str, meta: Optional[Dict] = None):
