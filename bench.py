@@ -116,7 +116,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
-    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart"], default="meander")
+    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart", "dambreak"],
+                    default="meander")
+    ap.add_argument("--t-end", type=float, default=300.0, help="dambreak: simulated seconds")
     ap.add_argument("--reaches", type=int, default=256,
                     help="ensemble: number of reaches; warmstart: held-out reaches (default 64)")
     ap.add_argument("--max-steps", type=int, default=60000, help="ensemble/warmstart: step limit")
@@ -141,6 +143,9 @@ def main():
         return
     if args.workload == "warmstart":
         warmstart(args)
+        return
+    if args.workload == "dambreak":
+        dambreak(args)
         return
     if args.calib_copy > 0:
         import torch
@@ -247,6 +252,58 @@ def ensemble(args):
                    "steady local dt CFL 0.9, stop RMS dh <= 1e-6", "launch": best,
                    "tune_s_per_step": times, "generate_s": round(gen_s, 1),
                    "setup_s": round(setup_s, 1)},
+    }
+    print(json.dumps(out))
+
+
+def cell_areas(case):
+    x, y, t = case.x, case.y, case.tri
+    return 0.5 * np.abs((x[t[:, 1]] - x[t[:, 0]]) * (y[t[:, 2]] - y[t[:, 0]]) -
+                        (x[t[:, 2]] - x[t[:, 0]]) * (y[t[:, 1]] - y[t[:, 0]]))
+
+
+def dambreak(args):
+    """The dam break in full: 1M triangles from rest until the simulated time
+    reaches t_end (default 300 s), Rusanov flux, SSP-RK2, CFL 0.5 with dt from
+    the CFL reduction, h_dry 1e-4, closed walls.  The run stops at the first
+    step that ends at or past t_end.  Reports the wall time, the steps, the
+    cell-updates/s and the change of the water volume (closed walls: only
+    the depth clip at the wet/dry front changes it)."""
+    from storm_amd import cases, fvt
+    case = cases.dambreak()
+    area = cell_areas(case)
+    v0 = float((case.h * area).sum())
+    t0 = time.perf_counter()
+    s = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph))
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gpu_s, it, tt = 0.0, 0, 0.0
+    while tt < args.t_end:
+        # steps to go at the last chunk's mean dt, less a margin; the last
+        # few are run one at a time
+        if it == 0:
+            n = 1000
+        else:
+            n = int(0.95 * (args.t_end - tt) / (tt / it))
+            n = max(1, min(n, 5000))
+        it, tt, _ = s.run(n)
+        gpu_s += s.last_run_seconds()
+        print("dambreak: step %d, t = %.3f s" % (it, tt), file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = s.state()
+    s.close()
+    v1 = float((st["qh"] * area).sum())
+    n = case.n_elems
+    out = {
+        "metric": "time to t = %g s, dam break over a dry bed" % args.t_end,
+        "value": wall, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "steps": it, "sim_time_s": tt, "gpu_seconds": gpu_s,
+        "cell_updates_per_s": n * it / wall, "volume_rel_change": (v1 - v0) / v0,
+        "wet_fraction": float((st["h"] > case.opts["h_dry"]).mean()),
+        "dtype": "f64", "data": "synthetic (storm_amd/cases.py dambreak, seed 3003)",
+        "config": {"workload": "dambreak-1M", "triangles": n,
+                   "scheme": "Rusanov, LSQ+Barth-Jespersen, SSP-RK2, CFL 0.5, h_dry 1e-4",
+                   "setup_s": round(setup_s, 3)},
     }
     print(json.dumps(out))
 

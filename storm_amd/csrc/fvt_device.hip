@@ -329,7 +329,8 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
   __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS],
       s_dt[CTRL_THREADS], s_topq[CTRL_THREADS], s_botq[CTRL_THREADS];
   const int t = threadIdx.x;
-  if (c->done) return;
+  const Ctrl cq = *c;  // one load of the whole control block
+  if (cq.done) return;
   // fixed-order reduction of the per-block partials of the last stage
   // (residual sums, max|phi|) and of the next step's CFL candidates
   double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX, topq = ZERO, botq = ZERO;
@@ -356,48 +357,51 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
     __syncthreads();
   }
   if (t != 0) return;
-  if (c->pending) {
+  Ctrl cc = cq;  // thread 0 works on a register copy, stored back once
+  if (cc.pending) {
     double sumtop = s_top[0], sumbot = s_bot[0];
-    c->sumtop = sumtop;
-    c->sumbot = sumbot;
+    cc.sumtop = sumtop;
+    cc.sumbot = sumbot;
     if (o.steady) {
-      if (o.stop_cr == 0) c->rmax = s_max[0];
+      if (o.stop_cr == 0) cc.rmax = s_max[0];
       else {
         if (equals(sumbot, ZERO)) sumbot = ONE;
-        c->rmax = sqrt(sumtop / sumbot);
+        cc.rmax = sqrt(sumtop / sumbot);
         if (o.stop_cr == 2) {
           // the same relative RMS change of the discharge (no reference
           // counterpart: stopping test 1 looks at h alone)
           double sumbotq = s_botq[0];
           if (equals(sumbotq, ZERO)) sumbotq = ONE;
-          c->rmax = dmax(c->rmax, sqrt(s_topq[0] / sumbotq));
+          cc.rmax = dmax(cc.rmax, sqrt(s_topq[0] / sumbotq));
         }
       }
-      c->stime = (double)c->iter;
-      c->pending = 0;
-      if (c->rmax < o.rstop) { c->done = 1; return; }
-      if (o.nonfinite_stop && !(c->rmax <= DBL_MAX)) {  // NaN or infinite: cannot converge
-        c->rmax = __builtin_nan("");
-        c->done = 1;
+      cc.stime = (double)cc.iter;
+      cc.pending = 0;
+      if (cc.rmax < o.rstop) { cc.done = 1; *c = cc; return; }
+      if (o.nonfinite_stop && !(cc.rmax <= DBL_MAX)) {  // NaN or infinite: cannot converge
+        cc.rmax = __builtin_nan("");
+        cc.done = 1;
+        *c = cc;
         return;
       }
     } else {
-      c->stime = c->stime + c->dt;
-      c->rmax = s_max[0];
-      get_time_bcs(ri, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, c->stime, sq, sh);
-      c->pending = 0;
+      cc.stime = cc.stime + cc.dt;
+      cc.rmax = s_max[0];
+      get_time_bcs(ri, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, cc.stime, sq, sh);
+      cc.pending = 0;
     }
   }
   if (begin) {
-    c->iter = c->iter + 1;
+    cc.iter = cc.iter + 1;
     if (!o.steady) {
-      c->dt = (o.dt_op == 0) ? o.cfl : s_dt[0];
-      c->ttime = c->ttime + c->dt;
+      cc.dt = (o.dt_op == 0) ? o.cfl : s_dt[0];
+      cc.ttime = cc.ttime + cc.dt;
     } else {
-      c->ttime = c->ttime + c->delt_last;
+      cc.ttime = cc.ttime + cc.delt_last;
     }
-    c->pending = 1;
+    cc.pending = 1;
   }
+  *c = cc;
 }
 
 // ---------------------------------------------------------------- vertex pass
