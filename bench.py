@@ -113,10 +113,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=0,
+                    help="1: replay whole steps from a captured HIP graph")
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
-    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart", "dambreak"],
+    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart", "dambreak", "flume"],
                     default="meander")
     ap.add_argument("--t-end", type=float, default=300.0, help="dambreak: simulated seconds")
     ap.add_argument("--reaches", type=int, default=256,
@@ -146,6 +147,9 @@ def main():
         return
     if args.workload == "dambreak":
         dambreak(args)
+        return
+    if args.workload == "flume":
+        flume(args)
         return
     if args.calib_copy > 0:
         import torch
@@ -253,6 +257,53 @@ def ensemble(args):
                    "tune_s_per_step": times, "generate_s": round(gen_s, 1),
                    "setup_s": round(setup_s, 1)},
     }
+    print(json.dumps(out))
+
+
+def flume(args):
+    """The straight flume in full: 20,000 triangles, 2,000 steps (HLL, LSQ +
+    Barth-Jespersen, SSP-RK2, CFL 0.9 with dt from the CFL reduction), from
+    the same state on the GPU and, for the CPU baseline, through the
+    reference Fortran on the same .run case (its solverFVT call timed)."""
+    import subprocess
+    import tempfile
+    from storm_amd import cases, fvt
+    steps = 2000
+    case = cases.flume(flux="hll", istop=steps)
+    s = fvt.FVTSolver(case, dict(istop=steps, graph=args.graph))
+    t0 = time.perf_counter()
+    it, tt, _ = s.run(steps)
+    wall = time.perf_counter() - t0
+    gpu_s = s.last_run_seconds()
+    s.close()
+    n = case.n_elems
+    out = {
+        "metric": "FP64 cell-updates/s, flume (2,000 steps)", "value": n * it / wall,
+        "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True, "steps": it,
+        "wall_s": wall, "gpu_seconds": gpu_s, "sim_time_s": tt, "dtype": "f64",
+        "data": "synthetic (storm_amd/cases.py flume, seed 1001)",
+        "config": {"workload": "flume-20k", "triangles": n,
+                   "scheme": "HLL, LSQ+Barth-Jespersen, SSP-RK2, CFL 0.9", "graph": args.graph},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the baseline only
+        if os.path.exists(O.REF_DRIVER):
+            with tempfile.TemporaryDirectory() as d:
+                run = cases.write_run(case, d)
+                r = subprocess.run([O.REF_DRIVER, os.path.basename(run), "out.bin"], cwd=d,
+                                   capture_output=True, text=True, timeout=600)
+                ts = [float(l.split()[-1]) for l in r.stdout.splitlines()
+                      if l.startswith("solverFVT seconds")]
+            if r.returncode == 0 and ts:
+                out["cpu_baseline"] = {"value": n * steps / ts[0], "unit": "cell-updates/s",
+                                       "cores": 1, "kind": "reference",
+                                       "sample": "the whole run through the reference Fortran "
+                                                 "(flang -O2, serial), solverFVT call alone, "
+                                                 "%.1f s" % ts[0]}
+            else:
+                print("bench.py: the reference run failed (exit %d)" % r.returncode,
+                      file=sys.stderr)
     print(json.dumps(out))
 
 

@@ -92,6 +92,7 @@ struct Geo {
   const int *qs_ptr, *qs_cell, *qs_slot;
   const double *qs_rcx, *qs_rcy, *qs_nx, *qs_ny, *qs_len;
   const int* in_type;
+  const int *svptr, *svtx;  // per reach, the vertices of its stage strings
   const RInfo* reach;
   int n_in, n_out, nr;  // string totals, reaches
 };
@@ -119,6 +120,10 @@ struct State {
   uint8_t *dry_post, *vwet;
   double *sq, *sh;    // current Q (or stage) of every inflow string, stage of every outflow string
   Ctrl* ctrl;         // one per reach
+  // per reach, `done` as the first stage of the latest step saw it (that
+  // stage's kernel writes it), for the next step's first vertex pass, which
+  // runs while k_control may be writing `done`
+  int* vdone;
 };
 
 // Tiles of the stage kernel: tile b owns cells [tptr[b], tptr[b+1]) (a run of
@@ -152,12 +157,12 @@ __device__ __forceinline__ void st4(CellD* p, double a, double b, double c, doub
 // tile hit the L2 that fetched the neighbouring tiles.  The map is a
 // bijection on [0, gridDim.x).
 constexpr int NXCD = 8;
-__device__ __forceinline__ int xcd_tile() {
-  const int nb = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ int xcd_map(int nb, int b) {
   const int q = nb / NXCD, r = nb % NXCD;
   const int x = b % NXCD, k = b / NXCD;
   return x * q + (x < r ? x : r) + k;
 }
+__device__ __forceinline__ int xcd_tile() { return xcd_map(gridDim.x, blockIdx.x); }
 
 // ---------------------------------------------------------------- cell geometry
 // Everything geometric is formed on the fly from the three vertices, with the
@@ -317,24 +322,34 @@ __device__ void get_time_bcs(const RInfo& ri, const int* out_type, const int* qt
   }
 }
 
+// What k_control reads and writes besides the mesh and options.
+struct CtrlArgs {
+  Ctrl* ctrl;
+  const double* part;
+  double *sq, *sh, *hvtx;
+  int begin;
+  const int *out_type, *qts_ptr, *hts_ptr;
+  const double *qts_t, *qts_v, *hts_t, *hts_v;
+};
+
 // Step bookkeeping of solverFVT.f90:132-155 (begin) and :338-472 (end of the
-// previous step: residuals, stopping test, time, boundary tables).
-__global__ void __launch_bounds__(CTRL_THREADS)
-k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh, int begin,
-          const int* out_type, const int* qts_ptr, const double* qts_t, const double* qts_v,
-          const int* hts_ptr, const double* hts_t, const double* hts_v) {
-  // one block per reach
-  const RInfo ri = g.reach[blockIdx.x];
-  Ctrl* c = &ctrl[blockIdx.x];
-  __shared__ double s_top[CTRL_THREADS], s_bot[CTRL_THREADS], s_max[CTRL_THREADS],
-      s_dt[CTRL_THREADS], s_topq[CTRL_THREADS], s_botq[CTRL_THREADS];
+// previous step: residuals, stopping test, time, boundary tables) for reach
+// r, by a block of NT threads.
+template <int NT>
+__device__ void control_block(const Geo& g, const Opts& o, const CtrlArgs& A, int r) {
+  const RInfo ri = g.reach[r];
+  Ctrl* c = &A.ctrl[r];
+  const double* part = A.part;
+  double* sh = A.sh;
+  __shared__ double s_top[NT], s_bot[NT], s_max[NT], s_dt[NT], s_topq[NT], s_botq[NT];
+  __shared__ int s_go;
   const int t = threadIdx.x;
   const Ctrl cq = *c;  // one load of the whole control block
   if (cq.done) return;
   // fixed-order reduction of the per-block partials of the last stage
   // (residual sums, max|phi|) and of the next step's CFL candidates
   double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX, topq = ZERO, botq = ZERO;
-  for (int b = ri.tile0 + t; b < ri.tile1; b += CTRL_THREADS) {
+  for (int b = ri.tile0 + t; b < ri.tile1; b += NT) {
     top = top + part[PART * b];
     bot = bot + part[PART * b + 1];
     mx = dmax(mx, part[PART * b + 2]);
@@ -345,7 +360,7 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
   s_top[t] = top; s_bot[t] = bot; s_max[t] = mx; s_dt[t] = dtm;
   s_topq[t] = topq; s_botq[t] = botq;
   __syncthreads();
-  for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
+  for (int w = NT / 2; w > 0; w >>= 1) {
     if (t < w) {
       s_top[t] = s_top[t] + s_top[t + w];
       s_bot[t] = s_bot[t] + s_bot[t + w];
@@ -356,52 +371,72 @@ k_control(Geo g, Opts o, Ctrl* ctrl, const double* part, double* sq, double* sh,
     }
     __syncthreads();
   }
-  if (t != 0) return;
-  Ctrl cc = cq;  // thread 0 works on a register copy, stored back once
-  if (cc.pending) {
-    double sumtop = s_top[0], sumbot = s_bot[0];
-    cc.sumtop = sumtop;
-    cc.sumbot = sumbot;
-    if (o.steady) {
-      if (o.stop_cr == 0) cc.rmax = s_max[0];
-      else {
-        if (equals(sumbot, ZERO)) sumbot = ONE;
-        cc.rmax = sqrt(sumtop / sumbot);
-        if (o.stop_cr == 2) {
-          // the same relative RMS change of the discharge (no reference
-          // counterpart: stopping test 1 looks at h alone)
-          double sumbotq = s_botq[0];
-          if (equals(sumbotq, ZERO)) sumbotq = ONE;
-          cc.rmax = dmax(cc.rmax, sqrt(s_topq[0] / sumbotq));
+  if (t == 0) {
+    Ctrl cc = cq;  // thread 0 works on a register copy, stored back once
+    bool go = true;
+    if (cc.pending) {
+      double sumtop = s_top[0], sumbot = s_bot[0];
+      cc.sumtop = sumtop;
+      cc.sumbot = sumbot;
+      if (o.steady) {
+        if (o.stop_cr == 0) cc.rmax = s_max[0];
+        else {
+          if (equals(sumbot, ZERO)) sumbot = ONE;
+          cc.rmax = sqrt(sumtop / sumbot);
+          if (o.stop_cr == 2) {
+            // the same relative RMS change of the discharge (no reference
+            // counterpart: stopping test 1 looks at h alone)
+            double sumbotq = s_botq[0];
+            if (equals(sumbotq, ZERO)) sumbotq = ONE;
+            cc.rmax = dmax(cc.rmax, sqrt(s_topq[0] / sumbotq));
+          }
         }
+        cc.stime = (double)cc.iter;
+        cc.pending = 0;
+        if (cc.rmax < o.rstop) { cc.done = 1; go = false; }
+        else if (o.nonfinite_stop && !(cc.rmax <= DBL_MAX)) {  // NaN or infinite: cannot converge
+          cc.rmax = __builtin_nan("");
+          cc.done = 1;
+          go = false;
+        }
+      } else {
+        cc.stime = cc.stime + cc.dt;
+        cc.rmax = s_max[0];
+        get_time_bcs(ri, A.out_type, A.qts_ptr, A.qts_t, A.qts_v, A.hts_ptr, A.hts_t, A.hts_v,
+                     cc.stime, A.sq, sh);
+        cc.pending = 0;
       }
-      cc.stime = (double)cc.iter;
-      cc.pending = 0;
-      if (cc.rmax < o.rstop) { cc.done = 1; *c = cc; return; }
-      if (o.nonfinite_stop && !(cc.rmax <= DBL_MAX)) {  // NaN or infinite: cannot converge
-        cc.rmax = __builtin_nan("");
-        cc.done = 1;
-        *c = cc;
-        return;
+    }
+    if (go && A.begin) {
+      cc.iter = cc.iter + 1;
+      if (!o.steady) {
+        cc.dt = (o.dt_op == 0) ? o.cfl : s_dt[0];
+        cc.ttime = cc.ttime + cc.dt;
+      } else {
+        cc.ttime = cc.ttime + cc.delt_last;
       }
-    } else {
-      cc.stime = cc.stime + cc.dt;
-      cc.rmax = s_max[0];
-      get_time_bcs(ri, out_type, qts_ptr, qts_t, qts_v, hts_ptr, hts_t, hts_v, cc.stime, sq, sh);
-      cc.pending = 0;
+      cc.pending = 1;
     }
+    *c = cc;
+    s_go = go;
+    // the stage of every stage string at the new time, for the pass below
+    for (int k = 0; k < ri.n_out; ++k) s_top[k] = sh[ri.out_base + k];
   }
-  if (begin) {
-    cc.iter = cc.iter + 1;
-    if (!o.steady) {
-      cc.dt = (o.dt_op == 0) ? o.cfl : s_dt[0];
-      cc.ttime = cc.ttime + cc.dt;
-    } else {
-      cc.ttime = cc.ttime + cc.delt_last;
-    }
-    cc.pending = 1;
+  __syncthreads();
+  if (!s_go) return;
+  // enforce_bcs.f90:40-56 for the nodes of the reach's stage strings, at the
+  // step's new time: the step's first vertex pass runs beside this block and
+  // leaves these vertices' depths to it
+  for (int k = g.svptr[r] + t; k < g.svptr[r + 1]; k += NT) {
+    const int p = g.svtx[k];
+    const double zv = g.vxyz[p].z;
+    A.hvtx[p] = dmax(ZERO, s_top[g.vbc[p] - ri.out_base] - zv);
   }
-  *c = cc;
+}
+
+// k_control alone, one block per reach (the end of a run)
+__global__ void __launch_bounds__(CTRL_THREADS) k_control(Geo g, Opts o, CtrlArgs A) {
+  control_block<CTRL_THREADS>(g, o, A, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- vertex pass
@@ -461,18 +496,23 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
   return v;
 }
 
-// The vertex pass of every stage: depths for the wall fluxes and wet flags
-// for the cells, from the stage input's water surface (NaN marks a dry cell).
-__global__ void __launch_bounds__(256)
-k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
-               const double* sh) {
-  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+// The vertex pass of every stage for vertex p: depths for the wall fluxes and
+// wet flags for the cells, from the stage input's water surface (NaN marks a
+// dry cell).  first: the pass of a step's first stage, which runs beside that
+// step's k_control.  It reads none of what k_control writes: a stopped reach
+// is known by `vdone`, and the depths of stage-string vertices (which depend
+// on the strings' new stages) are left to k_control.
+__device__ __forceinline__ void vertex_stage(const Geo& g, const Opts& o, const double* zeta,
+                                             double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
+                                             const int* vdone, const double* sh, int first,
+                                             int p) {
   if (p >= g.P) return;
   const double2 zb = g.vzb[p];
   const double zv = zb.x;
   const uint64_t bits = (uint64_t)__double_as_longlong(zb.y);
   const int vb = (int)(uint32_t)bits, deg = (int)((bits >> 32) & 0xFF);
-  if (ctrl[(int)(bits >> 40)].done) return;
+  const int r = (int)(bits >> 40);
+  if (first ? vdone[r] : ctrl[r].done) return;
   const size_t P = (size_t)g.P;
   double id = ZERO, a = ZERO, zr = zv;
   int counter = 0;
@@ -501,8 +541,32 @@ k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, c
   double hv = zeta_v - zv;
   if (hv < ZERO) hv = ZERO;
   vwet[p] = hv > o.h_dry;
-  if (vb >= 0) hv = dmax(ZERO, sh[vb] - zv);
+  if (vb >= 0) {
+    if (first) return;
+    hv = dmax(ZERO, sh[vb] - zv);
+  }
   hvtx[p] = hv;
+}
+
+// The vertex pass of stages 2 and up.
+__global__ void __launch_bounds__(256)
+k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
+               const double* sh) {
+  vertex_stage(g, o, zeta, hvtx, vwet, ctrl, nullptr, sh, 0, xcd_tile() * blockDim.x + threadIdx.x);
+}
+
+// The first stage's vertex pass and, as its first blocks (one per reach), the
+// step's k_control: the two need only the previous step's last stage and
+// write disjoint data, so one launch runs both.
+__global__ void __launch_bounds__(256)
+k_vertex_first(Geo g, Opts o, const double* zeta, uint8_t* vwet, const int* vdone, CtrlArgs A) {
+  const int nr = g.nr;
+  if ((int)blockIdx.x < nr) {
+    control_block<256>(g, o, A, blockIdx.x);
+    return;
+  }
+  const int p = xcd_map(gridDim.x - nr, blockIdx.x - nr) * blockDim.x + threadIdx.x;
+  vertex_stage(g, o, zeta, A.hvtx, vwet, A.ctrl, vdone, A.sh, 1, p);
 }
 
 // All vertices after the time loop (solverFVT.f90:483), with zetavtx.
@@ -1127,8 +1191,10 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     v0 = r1.x; v1 = r1.y; v2 = r1.z;
     pf_meta = tl.hmeta[hb + t - T];
   }
-  if (c->done) return;
   const RInfo ri = g.reach[hd1.w];
+  const int done = c->done;
+  if (st.s == 1 && t == 0 && b == ri.tile0) S.vdone[hd1.w] = done;
+  if (done) return;
   const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
   const Lds L = lds_layout(lds_dyn, CAP);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
@@ -1481,6 +1547,7 @@ struct storm_fvt {
   Tiles tl{};
   int cap = 0;
   std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
+  std::vector<int> hvdone;  // zeros for State::vdone at the start of a run
   int iter_host = 0;
   bool started = false;
   bool finalized = false;
@@ -1599,6 +1666,7 @@ void setup_device(storm_fvt* C) {
   g.qs_nx = dupload(own, P.qs_nx); g.qs_ny = dupload(own, P.qs_ny);
   g.qs_len = dupload(own, P.qs_len);
   g.in_type = dupload(own, P.in_type);
+  g.svptr = dupload(own, P.r_svptr); g.svtx = dupload(own, P.r_svtx);
   g.n_in = P.n_in; g.n_out = P.n_out; g.nr = P.nr;
   upload_tables(C);
   Opts& o = C->o;
@@ -1651,6 +1719,7 @@ void setup_device(storm_fvt* C) {
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
   S.ctrl = dalloc<Ctrl>(own, P.nr);
+  S.vdone = dalloc<int>(own, P.nr);
   S.sq = dalloc<double>(own, P.n_in);
   S.sh = dalloc<double>(own, P.n_out);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
@@ -1675,6 +1744,9 @@ void start_run(storm_fvt* C) {
   std::vector<double> qin(std::max(P.n_in, 1), 0.0), hbc(std::max(P.n_out, 1), 0.0);
   host_time_bcs(P, 0.0, qin.data(), hbc.data());
   HIPCHK(hipMemcpyAsync(C->S.ctrl, C->hctrl.data(), sizeof(Ctrl) * P.nr, hipMemcpyHostToDevice,
+                        C->stream));
+  C->hvdone.assign(P.nr, 0);
+  HIPCHK(hipMemcpyAsync(C->S.vdone, C->hvdone.data(), sizeof(int) * P.nr, hipMemcpyHostToDevice,
                         C->stream));
   if (P.n_in)
     HIPCHK(hipMemcpyAsync(C->S.sq, qin.data(), sizeof(double) * P.n_in, hipMemcpyHostToDevice,
@@ -1711,22 +1783,31 @@ void rk_table(int R, double a[3][3], double b[3]) {
   }
 }
 
-void launch_control(storm_fvt* C, int begin) {
-  hipLaunchKernelGGL(k_control, dim3(C->prep.nr), dim3(CTRL_THREADS), 0, C->stream, C->g, C->o,
-                     C->S.ctrl, C->S.part, C->S.sq, C->S.sh, begin, C->d_out_type, C->d_qts_ptr,
-                     C->d_qts_t, C->d_qts_v, C->d_hts_ptr, C->d_hts_t, C->d_hts_v);
+CtrlArgs ctrl_args(storm_fvt* C, int begin) {
+  CtrlArgs A;
+  A.ctrl = C->S.ctrl; A.part = C->S.part; A.sq = C->S.sq; A.sh = C->S.sh; A.hvtx = C->S.hvtx;
+  A.begin = begin;
+  A.out_type = C->d_out_type; A.qts_ptr = C->d_qts_ptr; A.hts_ptr = C->d_hts_ptr;
+  A.qts_t = C->d_qts_t; A.qts_v = C->d_qts_v; A.hts_t = C->d_hts_t; A.hts_v = C->d_hts_v;
+  return A;
 }
 
-// One time step: k_control, then per stage the vertex pass and the stage
-// kernel.  With launch == false only the host's view of the buffers advances
-// (after a graph replay of the same step).
+void launch_control(storm_fvt* C, int begin) {
+  hipLaunchKernelGGL(k_control, dim3(C->prep.nr), dim3(CTRL_THREADS), 0, C->stream, C->g, C->o,
+                     ctrl_args(C, begin));
+}
+
+// One time step: per stage the vertex pass and the stage kernel, where the
+// first stage's vertex pass launch also runs the step's k_control (both
+// need only the previous step's last stage).  With launch == false only the
+// host's view of the buffers advances (after a graph replay of the same
+// step).
 void launch_step(storm_fvt* C, bool launch = true) {
   Prep& P = C->prep;
   const int R = P.R;
   const int N = P.N;
   double a[3][3], b[3];
   rk_table(R, a, b);
-  if (launch) launch_control(C, 1);
   for (int s = 1; s <= R; ++s) {
     Stage st{};
     for (int j = 0; j < s; ++j) { st.slot[j] = C->slot_ptr(j); st.alpha[j] = a[s - 1][j]; }
@@ -1736,8 +1817,13 @@ void launch_step(storm_fvt* C, bool launch = true) {
     st.s = s;
     st.last = (s == R);
     if (launch) {
-      hipLaunchKernelGGL(k_vertex_stage, dim3((P.P + 255) / 256), dim3(256), 0, C->stream, C->g,
-                         C->o, C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
+      const int nvb = (P.P + 255) / 256;
+      if (s == 1)
+        hipLaunchKernelGGL(k_vertex_first, dim3(P.nr + nvb), dim3(256), 0, C->stream, C->g, C->o,
+                           C->S.zeta, C->S.vwet, C->S.vdone, ctrl_args(C, 1));
+      else
+        hipLaunchKernelGGL(k_vertex_stage, dim3(nvb), dim3(256), 0, C->stream, C->g, C->o,
+                           C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
       if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
         hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
                            C->S.din, C->S.gwork);

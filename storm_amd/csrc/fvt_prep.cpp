@@ -708,6 +708,21 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
   }
   S.r_last.resize(NR);
   for (int r = 0; r < NR; ++r) S.r_last[r] = ip[split->cell_ptr[r + 1] - 1];
+  {
+    std::vector<std::vector<int>> sv(NR);
+    for (int pp = 0; pp < P; ++pp) {
+      if (S.vbc[pp] < 0) continue;
+      const int p = S.vperm[pp];
+      const int deg = n2t_ptr[p + 1] - n2t_ptr[p];
+      const int vr = deg > 0 ? cell_reach[n2t[n2t_ptr[p]]] : 0;
+      sv[vr].push_back(pp);
+    }
+    S.r_svptr.assign(1, 0); S.r_svtx.clear();
+    for (int r = 0; r < NR; ++r) {
+      S.r_svtx.insert(S.r_svtx.end(), sv[r].begin(), sv[r].end());
+      S.r_svptr.push_back((int)S.r_svtx.size());
+    }
+  }
   return STORM_OK;
 }
 

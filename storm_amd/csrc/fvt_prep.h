@@ -89,6 +89,9 @@ struct Prep {
   // cell (its local dt drives steady friction), bch_max of dataprepFVT
   int nr = 1;
   std::vector<int> r_cell, r_in, r_out, r_last, r_tile;
+  // per reach, the vertices of its stage strings (vbc >= 0):
+  // r_svtx[r_svptr[r] .. r_svptr[r+1])
+  std::vector<int> r_svptr, r_svtx;
   std::vector<double> r_bch;
   std::vector<double> h0, u0, v0;     // initial state (internal order)
   // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
