@@ -329,14 +329,16 @@ def dambreak(args):
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     gpu_s, it, tt = 0.0, 0, 0.0
+    it0, tt0 = 0, 0.0
     while tt < args.t_end:
-        # steps to go at the last chunk's mean dt, less a margin; the last
-        # few are run one at a time
+        # steps to go at the last chunk's mean dt (dt shrinks as the flow
+        # speeds up), less a margin; the last few are run one at a time
         if it == 0:
             n = 1000
         else:
-            n = int(0.95 * (args.t_end - tt) / (tt / it))
+            n = int(0.9 * (args.t_end - tt) / ((tt - tt0) / (it - it0)))
             n = max(1, min(n, 5000))
+        it0, tt0 = it, tt
         it, tt, _ = s.run(n)
         gpu_s += s.last_run_seconds()
         print("dambreak: step %d, t = %.3f s" % (it, tt), file=sys.stderr, flush=True)

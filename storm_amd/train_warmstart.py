@@ -34,17 +34,17 @@ def log(msg):
 
 
 def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int,
-             gen_time: float = 0.0):
-    """Steady states of training reaches 0..n-1 (None where not converged).
-    gen_time > 0: no further batch is started once that many seconds have
-    passed, so fewer than n reaches may come back."""
+             gen_time: float = 0.0, first: int = 0):
+    """Steady states of training reaches first..first+n-1 (None where not
+    converged).  gen_time > 0: no further batch is started once that many
+    seconds have passed, so fewer than n reaches may come back."""
     states, steps, walls = [], [], []
     t0 = time.perf_counter()
-    for s in range(0, n_reaches, gen_batch):
+    for s in range(first, first + n_reaches, gen_batch):
         if gen_time > 0 and time.perf_counter() - t0 > gen_time:
-            log("generation time limit reached after %d reaches" % s)
+            log("generation time limit reached after %d reaches" % (s - first))
             break
-        ks = list(range(s, min(n_reaches, s + gen_batch)))
+        ks = list(range(s, min(first + n_reaches, s + gen_batch)))
         rs = [W.training_reach(k) for k in ks]
         init = [W.baseline_state(c) for c in rs]
         out, wall = W.steady_states(rs, dict(rstop=rstop, nonfinite_stop=True), max_steps,
@@ -78,27 +78,43 @@ def main(argv=None):
     ap.add_argument("--features", default=str(W.FEATURES),
                     help="node feature set(s), comma-separated: one network per set, "
                          "trained on the same data (out gets a _f<set> suffix when several)")
+    ap.add_argument("--first", type=int, default=0,
+                    help="first training reach (a later phase takes reaches no earlier one saw)")
+    ap.add_argument("--init", default="",
+                    help="continue training this checkpoint (a later phase on new reaches)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
     t0 = time.perf_counter()
-    states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps, a.gen_time)
+    states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps, a.gen_time,
+                                    a.first)
     gen_s = time.perf_counter() - t0
-    ks = [k for k, st in enumerate(states) if st is not None]
+    # keys: reach numbers of the converged steady states
+    ks = [a.first + j for j, st in enumerate(states) if st is not None]
     sets = [int(f) for f in a.features.split(",")]
+    prev = None
+    if a.init:
+        prev = W.WarmStart.load(a.init)
+        sets = [prev.features]
     for fs in sets:
         frames, ys = [], []
         for k in ks:
             c = W.training_reach(k)
             fr = W.frame(c, fs)
-            st = states[k]
+            st = states[k - a.first]
             frames.append(fr)
             ys.append(W.targets(fr, st["qh"], st["qu"], st["qv"]))
         n_val = max(1, int(round(a.val * len(ks))))
         tr_f, tr_y = frames[:-n_val], ys[:-n_val]
         va_f, va_y = frames[-n_val:], ys[-n_val:]
         log("feature set %d: training on %d reaches, validating on %d" % (fs, len(tr_f), len(va_f)))
-        ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed, features=fs)
+        if prev is not None:
+            ws = W.WarmStart(layers=prev.layers, lr=a.lr, seed=a.seed, params=prev.net.params(),
+                             features=fs)
+            ws.history = list(prev.history)
+            prev.close()
+        else:
+            ws = W.WarmStart(layers=6, lr=a.lr, seed=a.seed, features=fs)
         val0 = ws.loss(va_f, va_y)
         t1 = time.perf_counter()
         ws.train_windows(tr_f, tr_y, a.steps, reaches_per_step=a.reaches_per_step,
@@ -108,14 +124,26 @@ def main(argv=None):
         val1 = ws.loss(va_f, va_y)
         zero = float(np.mean([float((y ** 2).mean()) for y in va_y]))
         meta = dict(
-            reaches=len(states), converged=len(ks), rstop=a.rstop, solver=W.STEADY_OPTS,
+            first=a.first, reaches=len(states), converged=len(ks), rstop=a.rstop,
+            solver=W.STEADY_OPTS,
             train_reaches=len(tr_f), val_reaches=len(va_f), val_keys=ks[-n_val:],
-            steps_median=float(np.median([steps[k] for k in ks])),
+            steps_median=float(np.median([steps[k - a.first] for k in ks])),
             generate_s=gen_s, train_s=train_s, steps=ws.history[-1]["step"],
             reaches_per_step=a.reaches_per_step, window=a.window, lr=a.lr,
             lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1,
             val_loss_of_zero_output=zero,
             generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
+        # a later phase keeps the record of the earlier ones
+        phases = []
+        if a.init:
+            pm = W.WarmStart.load(a.init)
+            old = dict(pm.meta)
+            pm.close()
+            phases = old.pop("phases", None) or [{k: v for k, v in old.items()
+                                                  if k not in ("history", "val_keys")}]
+        phases.append({k: v for k, v in meta.items() if k != "val_keys"})
+        meta["phases"] = phases
+        meta["steady_states_total"] = int(sum(ph["converged"] for ph in phases))
         out = a.out if len(sets) == 1 else a.out.replace(".npz", "_f%d.npz" % fs)
         ws.save(out, meta)
         log("saved %s: val loss %.5f (untrained %.5f, zero output %.5f)" % (out, val1, val0, zero))

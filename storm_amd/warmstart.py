@@ -477,6 +477,7 @@ class WarmStart:
         ws = cls(layers=int(meta["layers"]), params=d["params"],
                  features=int(meta.get("features", 1)))
         ws.meta = meta
+        ws.history = list(meta.get("history", []))
         return ws
 
     def close(self):
