@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--size", type=int, default=200000, help="warmstart: triangles per reach")
     ap.add_argument("--rstop", type=float, default=1e-6, help="warmstart: RMS dh tolerance")
     ap.add_argument("--checkpoint", default="", help="warmstart: trained network (.npz)")
+    ap.add_argument("--start", choices=["cold", "plane", "learned"], default="cold",
+                    help="ensemble: initial state (plane and learned: storm_amd/warmstart.py, "
+                         "their set-up inside the timed region)")
     ap.add_argument("--modes", default="cold,plane,learned",
                     help="warmstart: starts to time (the ratio needs cold and learned)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -217,7 +220,8 @@ def main():
 
 
 def ensemble(args):
-    """Cold-start time to steady state of a batched ensemble."""
+    """Time to steady state of a batched ensemble from the generator's cold
+    start, or (--start) from the plane guess or the learned warm start."""
     from storm_amd import cases, fvt, tune
     t0 = time.perf_counter()
     reaches = [cases.ensemble_reach(k, steady=True) for k in range(args.reaches)]
@@ -232,6 +236,17 @@ def ensemble(args):
     b = fvt.BatchSolver(reaches, opts)
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
+    if args.start != "cold":
+        from storm_amd import warmstart as W
+        if args.start == "plane":
+            init = [W.baseline_state(c) for c in reaches]
+        else:
+            ws = W.WarmStart.load(args.checkpoint or W.DEFAULT_CHECKPOINT)
+            init = ws.predict(reaches)
+            ws.close()
+        b.set_state(np.concatenate([x[0] for x in init]), np.concatenate([x[1] for x in init]),
+                    np.concatenate([x[2] for x in init]))
+    init_s = time.perf_counter() - t0
     gpu_s = 0.0
     while True:
         b.run(1000)
@@ -245,7 +260,8 @@ def ensemble(args):
     n = np.array([c.n_elems for c in reaches])
     updates = float((n * it).sum())
     out = {
-        "metric": "time-to-steady (cold start), batched ensemble",
+        "metric": "time-to-steady (%s start), batched ensemble" % args.start,
+        "start": args.start, "init_s": init_s,
         "value": wall, "unit": "s", "n_gpus": 1, "higher_is_better": False,
         "cell_updates_per_s": updates / wall, "gpu_seconds": gpu_s,
         "reaches": len(reaches), "triangles": int(n.sum()),

@@ -77,6 +77,18 @@ extern "C" int probe_tiles(const storm_mesh_t* m, const storm_fvt_opts_t* o, int
       ok = ok && r[1 + j] == P.cnbr[(size_t)j * N + c] && r[4 + j] == P.cvtx[(size_t)j * N + c];
     if (!ok) return fail("bad halo record");
   }
-  stats[0] = nt; stats[1] = maxw; stats[2] = 0; stats[3] = (int)halo;
+  // the per-reach lists of stage-string vertices hold exactly the vertices
+  // with a stage string, once each
+  if ((int)P.r_svptr.size() != P.nr + 1 || P.r_svptr[0] != 0 ||
+      P.r_svptr[P.nr] != (int)P.r_svtx.size())
+    return fail("bad stage-string vertex lists");
+  std::vector<int> listed(P.P, 0);
+  for (int v : P.r_svtx) {
+    if (v < 0 || v >= P.P || P.vbc[v] < 0) return fail("listed vertex without a stage string");
+    listed[v]++;
+  }
+  for (int v = 0; v < P.P; ++v)
+    if ((P.vbc[v] >= 0) != (listed[v] == 1)) return fail("stage-string vertex not listed once");
+  stats[0] = nt; stats[1] = maxw; stats[2] = (int)P.r_svtx.size(); stats[3] = (int)halo;
   return 0;
 }

@@ -40,14 +40,17 @@ def run_probe(lib, case, cap):
 
 @pytest.mark.parametrize("cap", [256, 384, 512])
 def test_small_case_tiles(probe, cap):
+    n_stage = 0
     for name, (case, _) in small_cases().items():
-        nt, maxw, maxv, halo = run_probe(probe, case, cap)
+        nt, maxw, nsv, halo = run_probe(probe, case, cap)
         assert 1 <= nt <= case.n_elems and maxw <= cap, name
+        n_stage += nsv
+    assert n_stage > 0  # some case has a stage string (its vertex list is checked)
 
 
 def test_meander_tiles(probe):
     case = cases.meander()
-    nt, maxw, maxv, halo = run_probe(probe, case, 512)
+    nt, maxw, nsv, halo = run_probe(probe, case, 512)
     # compact Hilbert patches: the halo stays a small share of the tile cells
     assert halo < 0.3 * case.n_elems
     assert maxw <= 512
