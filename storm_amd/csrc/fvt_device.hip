@@ -341,7 +341,9 @@ __device__ void control_block(const Geo& g, const Opts& o, const CtrlArgs& A, in
   Ctrl* c = &A.ctrl[r];
   const double* part = A.part;
   double* sh = A.sh;
-  __shared__ double s_top[NT], s_bot[NT], s_max[NT], s_dt[NT], s_topq[NT], s_botq[NT];
+  constexpr int NW = NT / 64;
+  __shared__ double s_top[NW], s_bot[NW], s_max[NW], s_dt[NW], s_topq[NW], s_botq[NW];
+  __shared__ double s_str[MAX_STRINGS];
   __shared__ int s_go;
   const int t = threadIdx.x;
   const Ctrl cq = *c;  // one load of the whole control block
@@ -349,29 +351,62 @@ __device__ void control_block(const Geo& g, const Opts& o, const CtrlArgs& A, in
   // fixed-order reduction of the per-block partials of the last stage
   // (residual sums, max|phi|) and of the next step's CFL candidates
   double top = ZERO, bot = ZERO, mx = ZERO, dtm = DBL_MAX, topq = ZERO, botq = ZERO;
-  for (int b = ri.tile0 + t; b < ri.tile1; b += NT) {
-    top = top + part[PART * b];
-    bot = bot + part[PART * b + 1];
-    mx = dmax(mx, part[PART * b + 2]);
-    dtm = dmin(dtm, part[PART * b + 3]);
-    topq = topq + part[PART * b + 4];
-    botq = botq + part[PART * b + 5];
-  }
-  s_top[t] = top; s_bot[t] = bot; s_max[t] = mx; s_dt[t] = dtm;
-  s_topq[t] = topq; s_botq[t] = botq;
-  __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      s_top[t] = s_top[t] + s_top[t + w];
-      s_bot[t] = s_bot[t] + s_bot[t + w];
-      s_max[t] = dmax(s_max[t], s_max[t + w]);
-      s_dt[t] = dmin(s_dt[t], s_dt[t + w]);
-      s_topq[t] = s_topq[t] + s_topq[t + w];
-      s_botq[t] = s_botq[t] + s_botq[t + w];
+  // tiles t, t + NT, ... in that order.  UR tiles per round are loaded
+  // unconditionally (an index past the reach's last tile reads the last tile
+  // again and is then dropped), so a round's loads are in flight together;
+  // UR is small because the first vertex pass shares this kernel's register
+  // budget
+  constexpr int UR = 4;
+  const int last = ri.tile1 - 1;
+  for (int b0 = ri.tile0 + t; b0 <= last; b0 += UR * NT) {
+    double2 v[UR][2];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const double2* q = reinterpret_cast<const double2*>(&part[PART * min(b0 + u * NT, last)]);
+      v[u][0] = q[0];
+      v[u][1] = q[1];
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const bool in = b0 + u * NT <= last;
+      top = top + (in ? v[u][0].x : ZERO);
+      bot = bot + (in ? v[u][0].y : ZERO);
+      mx = in ? dmax(mx, v[u][1].x) : mx;
+      dtm = in ? dmin(dtm, v[u][1].y) : dtm;
+    }
   }
+  if (o.stop_cr == 2)
+    for (int b = ri.tile0 + t; b <= last; b += NT) {
+      topq = topq + part[PART * b + 4];
+      botq = botq + part[PART * b + 5];
+    }
+  // a fixed butterfly within each wave (every lane ends with the same
+  // bits), then the waves' results in wave order
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    top = top + __shfl_xor(top, off);
+    bot = bot + __shfl_xor(bot, off);
+    mx = dmax(mx, __shfl_xor(mx, off));
+    dtm = dmin(dtm, __shfl_xor(dtm, off));
+    topq = topq + __shfl_xor(topq, off);
+    botq = botq + __shfl_xor(botq, off);
+  }
+  if ((t & 63) == 0) {
+    const int w = t >> 6;
+    s_top[w] = top; s_bot[w] = bot; s_max[w] = mx; s_dt[w] = dtm;
+    s_topq[w] = topq; s_botq[w] = botq;
+  }
+  __syncthreads();
   if (t == 0) {
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      s_top[0] = s_top[0] + s_top[w];
+      s_bot[0] = s_bot[0] + s_bot[w];
+      s_max[0] = dmax(s_max[0], s_max[w]);
+      s_dt[0] = dmin(s_dt[0], s_dt[w]);
+      s_topq[0] = s_topq[0] + s_topq[w];
+      s_botq[0] = s_botq[0] + s_botq[w];
+    }
     Ctrl cc = cq;  // thread 0 works on a register copy, stored back once
     bool go = true;
     if (cc.pending) {
@@ -420,7 +455,7 @@ __device__ void control_block(const Geo& g, const Opts& o, const CtrlArgs& A, in
     *c = cc;
     s_go = go;
     // the stage of every stage string at the new time, for the pass below
-    for (int k = 0; k < ri.n_out; ++k) s_top[k] = sh[ri.out_base + k];
+    for (int k = 0; k < ri.n_out; ++k) s_str[k] = sh[ri.out_base + k];
   }
   __syncthreads();
   if (!s_go) return;
@@ -430,7 +465,7 @@ __device__ void control_block(const Geo& g, const Opts& o, const CtrlArgs& A, in
   for (int k = g.svptr[r] + t; k < g.svptr[r + 1]; k += NT) {
     const int p = g.svtx[k];
     const double zv = g.vxyz[p].z;
-    A.hvtx[p] = dmax(ZERO, s_top[g.vbc[p] - ri.out_base] - zv);
+    A.hvtx[p] = dmax(ZERO, s_str[g.vbc[p] - ri.out_base] - zv);
   }
 }
 
