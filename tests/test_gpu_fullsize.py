@@ -92,3 +92,27 @@ def test_dambreak_1m_to_300s_conserves_volume():
     h_dry = c.opts["h_dry"]
     assert np.array_equal(st["dry"] != 0, st["h"] < h_dry)
     assert np.mean(st["h"] > h_dry) > 0.5
+
+
+def test_meander_2m_10000_steps():
+    """The meandering reach over all 10,000 steps of its workload (SSP-RK3,
+    inflow ramp, critical outflow, dry floodplain), checked by what holds at
+    any length: every value finite, depths non-negative, the dry mask is
+    h < h_dry (h_wet = h_dry here), and the wetted area has not shrunk below
+    the initially wet channel."""
+    c = cases.meander()
+    h_dry = c.opts["h_dry"]
+    wet0 = float(np.mean(c.h > h_dry))
+    print("[gpu fullsize] meander N=%d, 10000 steps" % c.n_elems, flush=True)
+    s = fvt.FVTSolver(c, dict(istop=10000))
+    it, tt, _ = s.run(10000)
+    st = s.state()
+    s.close()
+    print("  %d steps, t = %.2f s, wet %.3f -> %.3f" % (it, tt, wet0, np.mean(st["h"] > h_dry)),
+          flush=True)
+    assert it == 10000 and tt > 0
+    for k in ("h", "qh", "qu", "qv"):
+        assert np.isfinite(st[k]).all(), k
+    assert (st["qh"] >= 0).all()
+    assert np.array_equal(st["dry"] != 0, st["h"] < h_dry)
+    assert np.mean(st["h"] > h_dry) >= 0.99 * wet0
