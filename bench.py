@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--start", choices=["cold", "plane", "learned"], default="cold",
                     help="ensemble: initial state (plane and learned: storm_amd/warmstart.py, "
                          "their set-up inside the timed region)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="warmstart: time to a fixed accuracy against tightly converged "
+                         "states (on --reaches reaches, default 8) instead of the RMS dh test")
     ap.add_argument("--modes", default="cold,plane,learned",
                     help="warmstart: starts to time (the ratio needs cold and learned)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -146,7 +149,10 @@ def main():
         ensemble(args)
         return
     if args.workload == "warmstart":
-        warmstart(args)
+        if args.accuracy:
+            warmstart_accuracy(args)
+        else:
+            warmstart(args)
         return
     if args.workload == "dambreak":
         dambreak(args)
@@ -373,6 +379,84 @@ def dambreak(args):
         "config": {"workload": "dambreak-1M", "triangles": n,
                    "scheme": "Rusanov, LSQ+Barth-Jespersen, SSP-RK2, CFL 0.5, h_dry 1e-4",
                    "setup_s": round(setup_s, 3)},
+    }
+    print(json.dumps(out))
+
+
+def warmstart_accuracy(args):
+    """Time to a fixed accuracy from each start.  The reference is each
+    reach's state converged to RMS dh <= 1e-9 (from the plane guess).  Each
+    start then runs, with the RMS dh test off, until every reach's RMS error
+    in h against its reference is at most 1 cm and then 1 mm; the state is
+    read back every 500 steps, so step counts are multiples of 500.  The GPU
+    time counts the steps only."""
+    import ctypes as C
+    from storm_amd import fvt
+    from storm_amd import warmstart as W
+    n_r = args.reaches if args.reaches != 256 else 8
+    ckpt = args.checkpoint or W.DEFAULT_CHECKPOINT
+    reaches = [W.heldout_reach(k, args.size) for k in range(n_r)]
+    t0 = time.perf_counter()
+    ref, _ = W.steady_states(reaches, dict(rstop=1e-9, nonfinite_stop=True), 400000,
+                             chunk=5000, init=[W.baseline_state(c) for c in reaches],
+                             log=lambda m: print("reference: %s" % m, file=sys.stderr, flush=True))
+    ref_s = time.perf_counter() - t0
+    keep = [r for r, st in enumerate(ref) if st[0] is not None]
+    reaches = [reaches[r] for r in keep]
+    href = [ref[r][0]["h"] for r in keep]
+    tols = (1e-2, 1e-3)
+    res = {}
+    for mode in args.modes.split(","):
+        if mode == "cold":
+            init = None
+        elif mode == "plane":
+            init = [W.baseline_state(c) for c in reaches]
+        else:
+            ws = W.WarmStart.load(ckpt)
+            init = ws.predict(reaches)
+            ws.close()
+        o = dict(W.STEADY_OPTS)
+        o.update(rstop=0.0, nonfinite_stop=True, istop=args.max_steps)
+        b = fvt.BatchSolver(reaches, o)
+        if init is not None:
+            b.set_state(np.concatenate([x[0] for x in init]), np.concatenate([x[1] for x in init]),
+                        np.concatenate([x[2] for x in init]))
+        first = {tol: [None] * len(reaches) for tol in tols}
+        at_gpu = {tol: None for tol in tols}
+        gpu_s, it = 0.0, 0
+        qh, qu, qv = np.zeros(b.N), np.zeros(b.N), np.zeros(b.N)
+        dp = C.POINTER(C.c_double)
+        while it < args.max_steps:
+            b.run(500)
+            gpu_s += b.last_run_seconds()
+            it = int(b.status()[0].max())
+            b._chk(fvt.load().storm_fvt_get_conserved(b.h, qh.ctypes.data_as(dp),
+                                                       qu.ctypes.data_as(dp),
+                                                       qv.ctypes.data_as(dp)))
+            for r in range(len(reaches)):
+                e = float(np.sqrt(np.mean((qh[b.cell_ptr[r]:b.cell_ptr[r + 1]] - href[r]) ** 2)))
+                for tol in tols:
+                    if first[tol][r] is None and e <= tol:
+                        first[tol][r] = it
+            for tol in tols:
+                if at_gpu[tol] is None and all(x is not None for x in first[tol]):
+                    at_gpu[tol] = gpu_s
+            print("%s: step %d, reaches within 1 mm: %d" %
+                  (mode, it, sum(x is not None for x in first[1e-3])), file=sys.stderr, flush=True)
+            if at_gpu[tols[-1]] is not None:
+                break
+        b.close()
+        res[mode] = {("%g m" % tol): dict(gpu_s_all=at_gpu[tol], steps=first[tol]) for tol in tols}
+        print("%s: %s" % (mode, json.dumps(res[mode])), file=sys.stderr, flush=True)
+    out = {
+        "metric": "GPU time until every reach is within 1 cm / 1 mm RMS of its converged h",
+        "unit": "s", "n_gpus": 1, "higher_is_better": False, "reaches": len(reaches),
+        "triangles_per_reach": args.size, "reference_s": ref_s,
+        "data": "synthetic (storm_amd/warmstart.py heldout_reach, seeds 9009+k)",
+        "config": {"workload": "warmstart-accuracy-%dx%d" % (len(reaches), args.size),
+                   "reference": "RMS dh <= 1e-9 from the plane guess",
+                   "checkpoint": os.path.relpath(ckpt, ROOT)},
+        "results": res,
     }
     print(json.dumps(out))
 
