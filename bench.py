@@ -139,8 +139,9 @@ def main():
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs the FVT step on one GPU only; --gpus %d is not "
-              "supported (DESIGN.md describes how the reach would shard)." % args.gpus,
+        print("bench.py: the FVT step runs on one GPU only; --gpus %d is not supported "
+              "(a reach would shard into contiguous ranges of tiles with a two-cell halo)."
+              % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
