@@ -2,7 +2,8 @@
 // declared in include/storm_fvt.h.
 //
 // Per RK stage (solverFVT.f90:158-331) two launches:
-//   k_vertex  the IDW water surface, depth and wet flag of every vertex
+//   k_vertex_stage (k_vertex_first in the first stage)
+//             the IDW water surface, depth and wet flag of every vertex
 //             (to_vertices.f90:26-41 with the stage strings' nodes of
 //             enforce_bcs);
 //   k_stage   one block per tile (a run of Hilbert-ordered cells and its
@@ -16,8 +17,10 @@
 //     friction if chosen, SSP-RK update, clip and wet/dry recovery; in the
 //     last stage also implicit friction, hvalve, the residual partial sums
 //     and the next step's CFL candidate.
-// Per step one single-block k_control (dt, time, BC tables, stopping).  The
-// Green-Gauss option adds its first pass (k_gauss1) before each stage.
+// Per step the bookkeeping (dt, time, BC tables, stopping; one block per
+// reach) runs as the first blocks of the first stage's vertex launch, and a
+// standalone k_control closes each run.  The Green-Gauss option adds its
+// first pass (k_gauss1) before each stage.
 //
 // Per-cell records are 32-byte structs (static, stage state); per-cell edge
 // data are stored per (slot, cell).
