@@ -17,6 +17,12 @@
  *   storm_fvt_get_vertex      what solverFVT leaves in the dep_vars / RKparams
  *                             modules (h,u,v,phi,drycell,rk*(:,rkorder),hvtx,
  *                             zetavtx; dep_vars.f90:16-57, RKparams.f90:19)
+ *   storm_fvt_get_gradients   delh, the depth gradient of the last stage
+ *                             (cell_grads.f90:24-75, hgradfix2.f90:46-98),
+ *                             which error_est(2) reads at solverFVT.f90:487
+ *   storm_fvt_get_bc_values   qin / hbc as get_time_bcs last left them
+ *                             (solverFVT.f90:462), which the post-loop
+ *                             enforce_bcs reads (to_vertices.f90:65)
  *   storm_fvt_last_error,
  *   storm_fvt_destroy         the error/shutdown role of byebye.f90:29-215
  *
@@ -115,7 +121,7 @@ int storm_fvt_create(const storm_mesh_t *mesh, const storm_fvt_opts_t *opts, sto
  * process.  The arrays of every other entry point are the reaches' arrays
  * laid end to end in reach order (cells, vertices, strings); iter/ttime/rmax
  * of storm_fvt_run are reach 0's.  Results equal those of n separate
- * contexts. */
+ * contexts.  Steady batches take an even rkorder (STORM_E_ARG otherwise). */
 int storm_fvt_create_batch(int32_t n, const storm_mesh_t *meshes, const storm_fvt_opts_t *opts,
                            storm_fvt_t **ctx);
 /* Per reach (arrays of n; any may be NULL): steps taken, time, residual of
@@ -141,6 +147,14 @@ int storm_fvt_get_state(storm_fvt_t *ctx, double *h, double *u, double *v, doubl
 int storm_fvt_get_conserved(storm_fvt_t *ctx, double *qh, double *qu, double *qv);
 /* Vertex depth and water-surface elevation of the last to_vertices. */
 int storm_fvt_get_vertex(storm_fvt_t *ctx, double *hvtx, double *zetavtx);
+/* delh(:)%x, delh(:)%y [n_elems] in input order, as the last stage formed
+ * them from its input state (zero before the first step); either may be
+ * NULL. */
+int storm_fvt_get_gradients(storm_fvt_t *ctx, double *dhx, double *dhy);
+/* The boundary values of the latest step: qin[n_in] (Q, or stage for vtype
+ * 4) and hbc[n_out] (stage; meaningful for htype 1 strings).  NULL skips a
+ * side. */
+int storm_fvt_get_bc_values(storm_fvt_t *ctx, double *qin, double *hbc);
 /* The cell count, the library's internal (Hilbert) cell order as the
  * permutation perm[internal] = input id, and the context's hipStream_t, for
  * callers that order their own GPU work after the time loop. */

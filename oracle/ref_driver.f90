@@ -20,6 +20,8 @@
 !   real64 phi(N,1), phi(N,2), phi(N,3)
 !   int32  drycell(N)                       1 = dry
 !   real64 hvtx(P), zetavtx(P)
+!   real64 uvtx(P), vvtx(P), cdvtx(P), zvtx(P)
+!   real64 delh(N)%x, delh(N)%y, e_est(N)     what error_est(2) read and wrote
 PROGRAM ref_driver
   USE parameters
   USE geometry
@@ -60,5 +62,7 @@ PROGRAM ref_driver
   WRITE (u_out) phi(1:n_elems, 1), phi(1:n_elems, 2), phi(1:n_elems, 3)
   WRITE (u_out) dryi(1:n_elems)
   WRITE (u_out) hvtx(1:n_pts), zetavtx(1:n_pts)
+  WRITE (u_out) uvtx(1:n_pts), vvtx(1:n_pts), cdvtx(1:n_pts), zvtx(1:n_pts)
+  WRITE (u_out) delh(1:n_elems)%x, delh(1:n_elems)%y, e_est(1:n_elems)
   CLOSE (u_out)
 END PROGRAM ref_driver

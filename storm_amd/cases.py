@@ -480,4 +480,9 @@ def read_ref_output(path: str) -> Dict[str, np.ndarray]:
         out["dry"] = np.frombuffer(f.read(4 * n), dtype=np.int32).copy()
         out["hvtx"] = np.frombuffer(f.read(8 * p), dtype=np.float64).copy()
         out["zetavtx"] = np.frombuffer(f.read(8 * p), dtype=np.float64).copy()
+        # the post-loop fields (uvtx, vvtx, cdvtx, zvtx; delh, e_est) follow
+        for k in ("uvtx", "vvtx", "cdvtx", "zvtx"):
+            out[k] = np.frombuffer(f.read(8 * p), dtype=np.float64).copy()
+        for k in ("delhx", "delhy", "e_est"):
+            out[k] = np.frombuffer(f.read(8 * n), dtype=np.float64).copy()
     return out

@@ -1530,6 +1530,23 @@ __global__ void k_final_cells(Geo g, const uint8_t* dry, const uint8_t* vwet, Ce
   if (!flag) { d[i].u = ZERO; d[i].v = ZERO; }
 }
 
+// The depth gradient delh as the last stage left it (cell_grads.f90:24-75 then
+// hgradfix2.f90:46-98 on that stage's input), which error_est reads after the
+// loop (solverFVT.f90:487, error_est.f90:80-82).  The stage kernel keeps it in
+// registers; this pass forms it again from the same input with the same code.
+// dh: [2N], x components then y components.
+__global__ void k_delh(Geo g, Opts o, const CellD* din, const double* gwork, double* dh) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  const int N = g.N;
+  const Tri tr = load_tri(g, i);
+  Faces F;
+  cell_faces(g, o, din, gwork, i, g.cmeta[i], g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i], tr,
+             true, F);
+  dh[i] = F.dhx;
+  dh[N + i] = F.dhy;
+}
+
 // ---------------------------------------------------------------- host side
 #define HIPCHK(x)                                                                       \
   do {                                                                                  \
@@ -1956,6 +1973,12 @@ int storm_fvt_create_batch(int32_t n, const storm_mesh_t* meshes, const storm_fv
   if (!ctx) return STORM_E_ARG;
   *ctx = nullptr;
   if (n < 1 || !meshes || !opts) return STORM_E_ARG;
+  // A reach that stops runs no stage while the others' steps go on swapping
+  // the buffer roles.  Under an odd RK order a step swaps them an odd number
+  // of times (RK1 also flips its slots), so the stopped reach's state would
+  // end in the buffer the others no longer read as the state: batched steady
+  // runs take an even order.
+  if (n > 1 && opts->steady && (opts->rkorder & 1)) return STORM_E_ARG;
   // the reaches laid end to end as one mesh: ids offset, strings and tables
   // appended in reach order
   const storm_mesh_t& m0 = meshes[0];
@@ -2119,6 +2142,15 @@ int storm_fvt_run(storm_fvt_t* C, int32_t nsteps, int32_t* iter, double* ttime, 
     if (C->finalized) return set_err(C, STORM_E_STATE, "run after finalize: call set_state first");
     const int room = std::max(0, C->prep.o.istop - C->iter_host);
     const int n = std::min(nsteps, room);
+    // The host's view of the buffer roles at the start of the call.  Steps
+    // launched after a reach stopped (steady convergence, a non-finite stop)
+    // run no stage, so they write no buffer: the view is rebuilt below from
+    // the steps the reach took.
+    const CellD* din0 = C->S.din;
+    CellD* dout0 = C->S.dout;
+    const CellD* last0 = C->last_din;
+    const int flip0 = C->r1_flip;
+    const int iter0 = C->hctrl[0].iter;
     HIPCHK(hipEventRecord(C->ev0, C->stream));
     int k = 0;
     if (C->prep.o.graph && n >= 2) {
@@ -2153,6 +2185,13 @@ int storm_fvt_run(storm_fvt_t* C, int32_t nsteps, int32_t* iter, double* ttime, 
       all_done = all_done && hc.done;
     }
     if (all_done) C->iter_host = C->prep.o.istop;
+    if (C->prep.nr == 1 && C->hctrl[0].iter - iter0 < n) {
+      C->S.din = din0;
+      C->S.dout = dout0;
+      C->last_din = last0;
+      C->r1_flip = flip0;
+      for (int j = iter0; j < C->hctrl[0].iter; ++j) launch_step(C, false);
+    }
     const Ctrl& h0 = C->hctrl[0];
     if (iter) *iter = h0.iter;
     if (ttime) *ttime = h0.ttime;
@@ -2262,6 +2301,66 @@ int storm_fvt_get_vertex(storm_fvt_t* C, double* hvtx, double* zetavtx) {
       HIPCHK(hipMemcpy(tmp.data(), C->S.zetavtx, sizeof(double) * P.P, hipMemcpyDeviceToHost));
       for (int pp = 0; pp < P.P; ++pp) zetavtx[P.vperm[pp]] = tmp[pp];
     }
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_get_gradients(storm_fvt_t* C, double* dhx, double* dhy) {
+  if (!C) return STORM_E_ARG;
+  try {
+    if (!C->started) start_run(C);
+    Prep& P = C->prep;
+    const size_t N = P.N;
+    if (C->iter_host == 0) {  // no stage has run: cell_grads has not either
+      for (size_t i = 0; i < N; ++i) {
+        if (dhx) dhx[i] = 0.0;
+        if (dhy) dhy[i] = 0.0;
+      }
+      return STORM_OK;
+    }
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * 2 * N));
+    const int bs = 256;
+    // the first Green-Gauss pass of that stage, again: steps launched after a
+    // steady stop run k_gauss1 on inputs that no stage used
+    if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
+      hipLaunchKernelGGL(k_gauss1, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g,
+                         C->last_din, C->S.gwork);
+    hipLaunchKernelGGL(k_delh, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g, C->o,
+                       C->last_din, C->S.gwork, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    if (e == hipSuccess) {
+      if (dhx) e = hipMemcpy(dhx, d, sizeof(double) * N, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && dhy)
+        e = hipMemcpy(dhy, d + N, sizeof(double) * N, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    HIPCHK(e);
+    std::vector<double> tmp(N);
+    for (double* a : {dhx, dhy}) {
+      if (!a) continue;
+      std::memcpy(tmp.data(), a, sizeof(double) * N);
+      for (size_t ii = 0; ii < N; ++ii) a[P.perm[ii]] = tmp[ii];
+    }
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_get_bc_values(storm_fvt_t* C, double* qin, double* hbc) {
+  if (!C) return STORM_E_ARG;
+  try {
+    if (!C->started) start_run(C);
+    HIPCHK(hipStreamSynchronize(C->stream));
+    Prep& P = C->prep;
+    if (qin && P.n_in)
+      HIPCHK(hipMemcpy(qin, C->S.sq, sizeof(double) * P.n_in, hipMemcpyDeviceToHost));
+    if (hbc && P.n_out)
+      HIPCHK(hipMemcpy(hbc, C->S.sh, sizeof(double) * P.n_out, hipMemcpyDeviceToHost));
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }

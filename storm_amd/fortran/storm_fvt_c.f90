@@ -67,6 +67,16 @@ MODULE storm_fvt_c
       TYPE(C_PTR), VALUE :: ctx, hvtx, zetavtx
     END FUNCTION
 
+    INTEGER(C_INT) FUNCTION storm_fvt_get_gradients(ctx, dhx, dhy) BIND(C, name="storm_fvt_get_gradients")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, dhx, dhy
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION storm_fvt_get_bc_values(ctx, qin, hbc) BIND(C, name="storm_fvt_get_bc_values")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, qin, hbc
+    END FUNCTION
+
     FUNCTION storm_fvt_last_error(ctx) BIND(C, name="storm_fvt_last_error")
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: ctx

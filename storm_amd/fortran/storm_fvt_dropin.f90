@@ -5,8 +5,9 @@
 ! them.  They read the mesh, boundary strings, initial data and options from
 ! the reference's modules (geometry, dep_vars, options, vbc_arrays), run the
 ! time step on the GPU through libstorm_fvt (ISO_C_BINDING, storm_fvt_c), and
-! leave h, u, v, phi, drycell, hvtx, zetavtx and rk*(:,rkorder) in the modules
-! where the reference's output routines expect them.  A non-zero status of
+! leave h, u, v, phi, drycell, rk*(:,rkorder), the vertex fields (hvtx,
+! zetavtx, uvtx, vvtx, cdvtx), delh and e_est in the modules where the
+! reference's output routines expect them.  A non-zero status of
 ! the library ends the run through byebye, the reference's fatal-stop path.
 MODULE storm_fvt_dropin_state
   USE, INTRINSIC :: ISO_C_BINDING
@@ -158,35 +159,49 @@ SUBROUTINE dataprepFVT
   cstat = storm_fvt_create(m, o, ctx)
   IF (cstat /= STORM_OK) CALL byebye('storm_fvt_create failed (mesh, options or device).')
 
-  ! Module copies of the cell/vertex data (dataprepFVT.f90:73-140).
+  ! Module copies of the cell and vertex data (dataprepFVT.f90:69-140), made
+  ! with the reference's own interpolation routines; the library forms the
+  ! same fields for itself from the arrays passed above.
+  CALL c2v_setup(nctr2vtx)
   IF (datav(3)) THEN
     zvtx = z_tcp(1:n_pts)
-    CALL vtx2ctr(zvtx, z)
   ELSE
-    z = z_tcp(1:n_elems)
+    CALL ctr2vtx(nctr2vtx, z_tcp, zvtx)
   END IF
+  CALL vtx2ctr(zvtx, z)
   IF (datav(4)) THEN
     CALL vtx2ctr(h_tcp, h)
+    hvtx = h_tcp(1:n_pts)
+    zetavtx = hvtx + zvtx
+    CALL vtx2ctr(zetavtx, zeta)
   ELSE
     h(1:n_elems) = h_tcp(1:n_elems)
+    zeta = z + h(1:n_elems)
+    CALL ctr2vtx(nctr2vtx, zeta, zetavtx)
+    hvtx = zetavtx - zvtx
   END IF
   h(n_elems+1) = zero
   IF (datav(5)) THEN
     CALL vtx2ctr(u_tcp, u)
+    uvtx = u_tcp(1:n_pts)
   ELSE
     u = u_tcp(1:n_elems)
+    CALL ctr2vtx(nctr2vtx, u_tcp, uvtx)
   END IF
   IF (datav(6)) THEN
     CALL vtx2ctr(v_tcp, v)
+    vvtx = v_tcp(1:n_pts)
   ELSE
     v = v_tcp(1:n_elems)
+    CALL ctr2vtx(nctr2vtx, v_tcp, vvtx)
   END IF
   IF (datav(7)) THEN
     CALL vtx2ctr(cd_tcp, cd)
+    cdvtx = cd_tcp(1:n_pts)
   ELSE
     cd = cd_tcp(1:n_elems)
+    CALL ctr2vtx(nctr2vtx, cd_tcp, cdvtx)
   END IF
-  zeta = z + h(1:n_elems)
   rough = cd
   DEALLOCATE(z_tcp, h_tcp, u_tcp, v_tcp, cd_tcp)
   datav = .FALSE.
@@ -201,6 +216,8 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   USE dep_vars
   USE options
   USE RKparams
+  USE constants
+  USE vbc_arrays
   USE storm_fvt_c
   USE storm_fvt_dropin_state
   IMPLICIT NONE
@@ -211,10 +228,10 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   INTEGER(C_INT32_T) :: it
   REAL(C_DOUBLE) :: tt, rm
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: hh(:), uu(:), vv(:), ph(:), qh(:), qu(:), qv(:), &
-                                         hv(:), zv(:)
+                                         hv(:), zv(:), gx(:), gy(:), qb(:), hb(:)
   INTEGER(C_INT8_T), ALLOCATABLE, TARGET :: dr(:)
   INTEGER(C_INT) :: cstat
-  INTEGER :: i
+  INTEGER :: i, k
 
   IF (.NOT. C_ASSOCIATED(ctx)) CALL byebye('solverFVT: dataprepFVT was not called.')
   cstat = storm_fvt_run(ctx, INT(istop, C_INT32_T), it, tt, rm)
@@ -248,6 +265,35 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   hvtx = hv
   zetavtx = zv
   zeta = z + h(1:n_elems)
+
+  ! The rest of the post-loop work (solverFVT.f90:483-487) on the host, with
+  ! the reference's own routines: the vertex velocities of to_vertices.f90:56-65
+  ! under the boundary values of the last step, the vertex drag coefficient,
+  ! and the error estimates from the last stage's depth gradient.
+  ALLOCATE(gx(n_elems), gy(n_elems), qb(MAX(1, n_inflowbdr)), hb(MAX(1, n_outflowbdr)))
+  cstat = storm_fvt_get_bc_values(ctx, C_LOC(qb), C_LOC(hb))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_gradients(ctx, C_LOC(gx), C_LOC(gy))
+  IF (cstat /= STORM_OK) CALL byebye('solverFVT: reading the GPU state failed.')
+  DO k = 1, n_inflowbdr
+    qin(k) = qb(k)
+  END DO
+  DO k = 1, n_outflowbdr
+    IF (htype(k) == 1) hbc(k) = hb(k)
+  END DO
+  CALL ctr2vtx(nctr2vtx, u, uvtx)
+  CALL ctr2vtx(nctr2vtx, v, vvtx)
+  IF (btype == 0) CALL wetdryvfix
+  WHERE (hvtx < h_dry)
+    uvtx = zero
+    vvtx = zero
+  END WHERE
+  CALL enforce_bcs(hvtx, uvtx, vvtx, zvtx, n_pts)
+  CALL ctr2vtx(nctr2vtx, cd, cdvtx)
+  DO i = 1, n_elems
+    delh(i)%x = gx(i)
+    delh(i)%y = gy(i)
+  END DO
+  CALL error_est(2)
   tecplot_print = .TRUE.
   IF (LEN(fn) < 0 .OR. LEN(title) < 0) RETURN  ! names kept for the call surface
 END SUBROUTINE solverFVT

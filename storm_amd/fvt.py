@@ -63,7 +63,7 @@ EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch
            "storm_fvt_reach_status", "storm_fvt_set_state",
            "storm_fvt_set_bc_values", "storm_fvt_run", "storm_fvt_finalize",
            "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
-           "storm_fvt_device_info", "storm_fvt_last_run_seconds", "storm_fvt_last_error",
+           "storm_fvt_get_gradients", "storm_fvt_get_bc_values", "storm_fvt_device_info", "storm_fvt_last_run_seconds", "storm_fvt_last_error",
            "storm_fvt_destroy", "storm_fvt_inspect_molecule"]
 
 _lib = None
@@ -91,6 +91,8 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8)]
     L.storm_fvt_get_conserved.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.storm_fvt_get_vertex.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_get_gradients.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_get_bc_values.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
     L.storm_fvt_last_run_seconds.restype = C.c_double
     L.storm_fvt_last_run_seconds.argtypes = [C.c_void_p]
@@ -259,6 +261,18 @@ class FVTSolver:
         self._chk(load().storm_fvt_get_vertex(self.h, _p(hv, C.c_double), _p(zv, C.c_double)))
         return dict(h=h, u=u, v=v, qh=qh, qu=qu, qv=qv, phi1=phi[:N], phi2=phi[N:2 * N],
                     phi3=phi[2 * N:], dry=dry.astype(np.int32), hvtx=hv, zetavtx=zv)
+
+    def gradients(self):
+        """delh of the last stage (x, y), what error_est reads after the loop."""
+        gx, gy = np.zeros(self.N), np.zeros(self.N)
+        self._chk(load().storm_fvt_get_gradients(self.h, _p(gx, C.c_double), _p(gy, C.c_double)))
+        return gx, gy
+
+    def bc_values(self):
+        """qin, hbc of the latest step (get_time_bcs' module values)."""
+        q, hb = np.zeros(max(self.n_in, 1)), np.zeros(max(self.n_out, 1))
+        self._chk(load().storm_fvt_get_bc_values(self.h, _p(q, C.c_double), _p(hb, C.c_double)))
+        return q[:self.n_in], hb[:self.n_out]
 
     def last_run_seconds(self) -> float:
         return float(load().storm_fvt_last_run_seconds(self.h))

@@ -3,8 +3,8 @@
 Runs the reference SToRM FVT path -- built from the reference's own Fortran
 sources by oracle/Makefile into oracle/_ref/ref_driver (this container only)
 -- on the small seeded cases of tests/cases_small.py and stores its final
-state (h, u, v, rk*(:,rkorder), phi, drycell, hvtx, zetavtx, iter, ttime) as
-.npz fixtures.  The fixtures are data: inputs are regenerated from the case
+state (h, u, v, rk*(:,rkorder), phi, drycell, hvtx, zetavtx, iter, ttime) and
+the post-loop fields (uvtx, vvtx, cdvtx, zvtx, delh, e_est) as .npz fixtures.  The fixtures are data: inputs are regenerated from the case
 generator and seed recorded with them.
 
 Usage (from the repository root):  python tests/golden/make_golden.py [name ...]
@@ -23,7 +23,8 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import oracle as O  # noqa: E402
 from cases_small import small_cases  # noqa: E402
 
-KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx")
+KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx",
+        "uvtx", "vvtx", "cdvtx", "zvtx", "delhx", "delhy", "e_est")
 
 
 def main():

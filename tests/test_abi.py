@@ -1,8 +1,9 @@
 """The C ABI without a GPU: libstorm_fvt.so and libstorm_mpnn.so load, export
 every function include/storm_fvt.h and include/storm_mpnn.h declare, and the
 Python mirrors bind exactly those.  The only calls made are host-only ones:
-the default options and the least-squares molecule of a small mesh (the
-set-up table of elemnt_db.f90, which needs no device)."""
+the default options, the least-squares molecule of a small mesh (the
+set-up table of elemnt_db.f90, which needs no device) and an option check
+that fails before any device work."""
 import ctypes as C
 import os
 import re
@@ -62,3 +63,20 @@ def test_inspect_molecule_on_the_host():
     assert (counts >= deg).all()
     for i in range(N):
         assert i not in cells[ptr[i]:ptr[i + 1]]
+
+
+def test_steady_batch_with_odd_rk_order_is_refused():
+    """A steady reach that stops early keeps its state in place while the
+    other reaches' steps swap the stage buffers; under an odd RK order that
+    swap count is odd, so such batches are refused before any device work."""
+    cs = [cases.flume(nx=6, ny=3, length=30.0, width=9.0, seed=5 + r) for r in range(2)]
+    keep = [{}, {}]
+    meshes = (fvt.Mesh * 2)(*[fvt.mesh_struct(c, k) for c, k in zip(cs, keep)])
+    L = fvt.load()
+    ctx = C.c_void_p()
+    for rk in (1, 3):
+        o = fvt.Opts()
+        L.storm_fvt_default_opts(C.byref(o))
+        o.rkorder, o.steady = rk, 1
+        rc = L.storm_fvt_create_batch(2, meshes, C.byref(o), C.byref(ctx))
+        assert rc == -1 and not ctx.value, rc

@@ -2,7 +2,8 @@
 (read_mfa and the mesh set-up) with dataprepFVT/solverFVT replaced by the
 Fortran shim over libstorm_fvt (oracle/_ref/dropin_driver, linked by
 `make -C oracle dropin`), run on the .run form of each small case and
-compared with the reference's golden outputs."""
+compared with the reference's golden outputs, the post-loop fields
+(vertex velocities, cdvtx, e_est) included."""
 import os
 import subprocess
 import tempfile
@@ -40,3 +41,8 @@ def test_dropin_matches_reference(name):
         a, b = g[k], out[k]
         assert np.abs(a - b).max() <= 1e-9 * max(np.abs(a).max(), 1e-300), k
     assert np.array_equal(g["dry"], out["dry"])
+    # the post-loop fields the reference's writers read: vertex velocities and
+    # drag, the bed at the vertices, delh and the error estimates
+    for k in ("uvtx", "vvtx", "cdvtx", "zvtx", "delhx", "delhy", "e_est"):
+        a, b = g[k], out[k]
+        assert np.abs(a - b).max() <= 1e-9 * max(np.abs(a).max(), 1e-300), k

@@ -1,8 +1,12 @@
 """HIP path vs the CPU oracle on the same seeded cases (full time loop)."""
+import os
+
 import numpy as np
 import pytest
 
 from cases_small import small_cases
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +40,48 @@ def test_step_parity(name):
     for k in ("hvtx", "zetavtx", "phi1", "phi2", "phi3"):
         assert _cmp(so[k], sg[k]) <= 1e-9, (k, _cmp(so[k], sg[k]))
     assert np.array_equal(so["dry"], sg["dry"])
+    # delh of the last stage, against what the reference's error_est read
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    gx, gy = s.gradients()
+    assert _cmp(g["delhx"], gx) <= 1e-9, _cmp(g["delhx"], gx)
+    assert _cmp(g["delhy"], gy) <= 1e-9, _cmp(g["delhy"], gy)
+
+
+@pytest.mark.parametrize("rk", [1, 3])
+def test_steps_after_a_steady_stop_change_nothing(rk):
+    """Steps launched after a steady run has converged run no stage; under
+    an odd RK order each would swap the stage buffers an odd number of
+    times.  State, final to_vertices and delh are those of the last step
+    taken, whether 0, 1 or 2 steps were launched after it."""
+    import oracle as O
+    from storm_amd import cases, fvt
+    case = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41, dt_op=0, cfl=0.3,
+                       istop=1200, q_in=15.0, steady=True, stop_cr=1, rstop=1e-4, rkorder=rk)
+    orc = O.Oracle(case)
+    k = orc.run(1200)["iter"]
+    orc.finalize()
+    so = orc.state()
+    assert k < 1200
+    out = []
+    for n in (k, k + 1, k + 2):
+        print(f"[gpu steady stop] rk={rk} converged at {k}, launched {n}", flush=True)
+        s = fvt.FVTSolver(case)
+        it, _, _ = s.run(n)
+        s.finalize()
+        sg = s.state()
+        gx, gy = s.gradients()
+        s.close()
+        assert it == k
+        for key in ("h", "qh", "qu", "qv", "u", "v"):
+            assert _cmp(so[key], sg[key]) <= RTOL, (n, key, _cmp(so[key], sg[key]))
+        for key in ("hvtx", "zetavtx", "phi1", "phi2", "phi3"):
+            assert _cmp(so[key], sg[key]) <= 1e-9, (n, key, _cmp(so[key], sg[key]))
+        assert np.array_equal(so["dry"], sg["dry"])
+        sg["gx"], sg["gy"] = gx, gy
+        out.append(sg)
+    for o in out[1:]:
+        for key in out[0]:
+            assert np.array_equal(out[0][key], o[key]), key
 
 
 @pytest.mark.parametrize("cap", [384, 512])
