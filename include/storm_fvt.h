@@ -55,7 +55,10 @@ enum {
  * commented alternatives at :138,143-145). */
 enum { STORM_FLUX_AC = 0, STORM_FLUX_HLL = 1, STORM_FLUX_RUSANOV = 2, STORM_FLUX_BGNVC = 3 };
 /* Cell gradients (cell_grads.f90:46-54). */
-enum { STORM_GRAD_LSQ = 0, STORM_GRAD_GAUSS = 1 };
+/* LSQ: lsq_grad_array on t2t (weights 1/d^2); GAUSS: Green-Gauss (A&C eqs
+ * 11-13); HDLS: hdlsq_grad_array on t2tHD, unit weights (ACTIVATE HDLS,
+ * cell_grads.f90:37-42, hdlsq_setup.f90, elemnt_db.f90:136-214). */
+enum { STORM_GRAD_LSQ = 0, STORM_GRAD_GAUSS = 1, STORM_GRAD_HDLS = 2 };
 
 /* Mesh and initial data: the contents of the reference's geometry/dep_vars/
  * vbc_arrays modules after read_mfa (tecplot_read.f90, ioflow.f90). */

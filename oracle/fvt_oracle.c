@@ -168,6 +168,71 @@ static void cde_t2t(int *row, int *n_io) {
   *n_io = n;
 }
 
+/* elemnt_db.f90:136-214: the molecule of the higher-order least squares
+ * (ACTIVATE HDLS), which replaces t2t for the gradients.  Row i lists every
+ * other triangle on i's three vertices, vertex by vertex in n2t order, then
+ * cde_t2t drops repeats.  A row left with fewer than three entries is
+ * augmented as :168-197 write it: the loop index j = 2..nn is used as a
+ * triangle NUMBER (grid(j)), and a candidate is skipped when its column in
+ * the n2t row equals j. */
+static int build_t2thd(orc_t *s, char *err, int errlen) {
+  const int N = s->N, P = s->P;
+  int maxc = 0;
+  for (int p = 0; p < P; ++p) {
+    int c = s->n2t_ptr[p + 1] - s->n2t_ptr[p];
+    if (c > maxc) maxc = c;
+  }
+  const int MAXD = 4 * maxc + 3; /* columns, the count included */
+  int *w = xcalloc((size_t)N * MAXD, sizeof(int));
+#define W(i, k) w[(size_t)(i) * MAXD + (k)]
+  for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      int k = s->vtx[3 * i + j];
+      for (int t = s->n2t_ptr[k]; t < s->n2t_ptr[k + 1]; ++t) {
+        int m = s->n2t[t];
+        if (m == i) continue;
+        W(i, W(i, 0) + 1) = m;
+        W(i, 0) = W(i, 0) + 1;
+      }
+    }
+    int cnt = W(i, 0);
+    cde_t2t(&W(i, 0), &cnt);
+    W(i, 0) = cnt;
+  }
+  for (int i = 0; i < N; ++i) {
+    int n = W(i, 0);
+    if (n > 2) continue;
+    if (n < 1) { free(w); snprintf(err, errlen, "triangle connectivity table incorrect"); return -1; }
+    int nn = W(i, 0) + 1 < 3 ? W(i, 0) + 1 : 3;
+    for (int j = 2; j <= nn; ++j) {     /* j: a 1-based triangle number */
+      for (int k = 0; k < 3; ++k) {
+        int l = s->vtx[3 * (j - 1) + k];
+        int cl = s->n2t_ptr[l + 1] - s->n2t_ptr[l];
+        for (int m = 2; m <= cl + 1; ++m) { /* m: 1-based column of the n2t row */
+          int t = s->n2t[s->n2t_ptr[l] + m - 2];
+          if (t == i || m == j) continue;
+          if (W(i, 0) + 2 > MAXD) { free(w); snprintf(err, errlen, "allocation bounds in T2TW"); return -1; }
+          W(i, W(i, 0) + 1) = t;
+          W(i, 0) = W(i, 0) + 1;
+        }
+      }
+    }
+    int cnt = W(i, 0);
+    cde_t2t(&W(i, 0), &cnt);
+    W(i, 0) = cnt;
+  }
+  free(s->t2t_ptr);
+  free(s->t2t);
+  s->t2t_ptr = xcalloc(N + 1, sizeof(int));
+  for (int i = 0; i < N; ++i) s->t2t_ptr[i + 1] = s->t2t_ptr[i] + W(i, 0);
+  s->t2t = xcalloc(s->t2t_ptr[N] + 1, sizeof(int));
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < W(i, 0); ++k) s->t2t[s->t2t_ptr[i] + k] = W(i, k + 1);
+#undef W
+  free(w);
+  return 0;
+}
+
 static int build(orc_t *s, const orc_case_t *c, char *err, int errlen) {
   const int P = s->P, N = s->N, E = s->E;
   /* nodes and connectivity (tecplot_read.f90:50-143) */
@@ -401,6 +466,7 @@ static int build(orc_t *s, const orc_case_t *c, char *err, int errlen) {
       if (e1 == i) e1 = s->ee[2 * k + 1];
       s->t2t3[3 * i + j] = e1;
     }
+  if (s->o.gradient == ORC_GRAD_HDLS) return build_t2thd(s, err, errlen);
   return 0;
 }
 
@@ -473,14 +539,17 @@ static int dataprep(orc_t *s, const orc_case_t *c) {
     s->tng[2 * k] = s->tng[2 * k] / s->len[k];
     s->tng[2 * k + 1] = s->tng[2 * k + 1] / s->len[k];
   }
-  /* lsq_setup.f90:92-145 (solver 2, weighted type 2) */
+  /* lsq_setup.f90:92-145 (solver 2, weighted type 2), or with ACTIVATE HDLS
+   * hdlsq_setup.f90:39-74 on t2tHD, unweighted (type 1, dataprepFVT.f90:219) */
   for (int i = 0; i < N; ++i) {
     double a = ZERO, b = ZERO, cc = ZERO;
     for (int t = s->t2t_ptr[i]; t < s->t2t_ptr[i + 1]; ++t) {
       int l = s->t2t[t];
       s->lsq_dx[t] = s->xc[l] - s->xc[i];
       s->lsq_dy[t] = s->yc[l] - s->yc[i];
-      s->lsq_w[t] = ONE / (s->lsq_dx[t] * s->lsq_dx[t] + s->lsq_dy[t] * s->lsq_dy[t]);
+      s->lsq_w[t] = s->o.gradient == ORC_GRAD_HDLS
+                        ? ONE
+                        : ONE / (s->lsq_dx[t] * s->lsq_dx[t] + s->lsq_dy[t] * s->lsq_dy[t]);
     }
     for (int t = s->t2t_ptr[i]; t < s->t2t_ptr[i + 1]; ++t) {
       a = a + s->lsq_w[t] * s->lsq_dx[t] * s->lsq_dx[t];
@@ -805,7 +874,8 @@ static void find_dry_cells(orc_t *s, int with_wavec) {
     for (int i = 0; i < s->N; ++i) s->wavec[i] = s->dry[i] ? ZERO : sqrt(G * s->h[i]);
 }
 
-/* lsq_grad_array.f90:60-73 */
+/* lsq_grad_array.f90:60-73; on t2tHD with unit weights the same sums are
+ * hdlsq_grad_array.f90:36-48 */
 static void lsq_grad(const orc_t *s, const double *ph, double *gr) {
   for (int i = 0; i < s->N; ++i) {
     double di = ZERO, ei = ZERO;

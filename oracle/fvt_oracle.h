@@ -39,7 +39,7 @@ typedef struct orc_case {
 } orc_case_t;
 
 enum { ORC_FLUX_AC = 0, ORC_FLUX_HLL = 1, ORC_FLUX_RUSANOV = 2, ORC_FLUX_BGNVC = 3 };
-enum { ORC_GRAD_LSQ = 0, ORC_GRAD_GAUSS = 1 };
+enum { ORC_GRAD_LSQ = 0, ORC_GRAD_GAUSS = 1, ORC_GRAD_HDLS = 2 };
 
 typedef struct orc_opts {
   int32_t rkorder;     /* 1, 2, 3 (RK ORDER) */

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 
 FLUX = {"ac": 0, "hll": 1, "rusanov": 2, "bgnvc": 3}
-GRAD = {"lsq": 0, "gauss": 1}
+GRAD = {"lsq": 0, "gauss": 1, "hdls": 2}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -460,6 +460,8 @@ def write_run(case: Case, dirpath: str, out_prefix: str = "case") -> str:
         f.write("ROUGHNESS %d\n" % int(o["friction"]))
         if o["vclip"]:
             f.write("MAX VELOCITY %.17e\n" % float(o["vceiling"]))
+        if str(o.get("gradient", "lsq")) == "hdls":
+            f.write("ACTIVATE HDLS\n")
     run = os.path.join(dirpath, out_prefix + ".run")
     with open(run, "w") as f:
         f.write("grid.dat\nbcs.dat\n%s.out\n%s.sol\noptions.dat\n"

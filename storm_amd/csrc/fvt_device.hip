@@ -712,11 +712,14 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
       gvx = HALF * (w[4 * N + i] + w0 * w[4 * N + n0] + w1 * w[4 * N + n1] + w2 * w[4 * N + n2]);
       gvy = HALF * (w[5 * N + i] + w0 * w[5 * N + n0] + w1 * w[5 * N + n1] + w2 * w[5 * N + n2]);
     } else {
+      // ACTIVATE HDLS: the t2tHD molecule for every cell, unit weights
+      // (hdlsq_setup.f90:39-74 with type 1, hdlsq_grad_array.f90:36-48)
+      const bool hd = o.gradient == STORM_GRAD_HDLS;
       double dz = ZERO, ez = ZERO, du = ZERO, eu = ZERO, dv = ZERO, ev = ZERO;
       double a = ZERO, b = ZERO, cc = ZERO;
       auto member = [&](double mxc, double myc, double mzeta, double mu, double mv) {
         const double dx = mxc - xci, dy = myc - yci;
-        const double w = ONE / (dx * dx + dy * dy);
+        const double w = ONE / (hd ? ONE : dx * dx + dy * dy);
         const double fz = mzeta - zeta_i;
         const double fu = mu - ui;
         const double fv = mv - vi;
@@ -725,8 +728,8 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
         du = du + w * fu * dx; eu = eu + w * fu * dy;
         dv = dv + w * fv * dx; ev = ev + w * fv * dy;
       };
-      const bool interior = ((slot_bits(meta, 0) | slot_bits(meta, 1) | slot_bits(meta, 2)) &
-                             CMS_KIND) == SK_INTERIOR;
+      const bool interior = !hd && ((slot_bits(meta, 0) | slot_bits(meta, 1) |
+                                     slot_bits(meta, 2)) & CMS_KIND) == SK_INTERIOR;
       if (interior) {
         // elemnt_db.f90:50-68: a cell with three edge neighbours keeps them,
         // in edge-number order, which is the processing order of its sides

@@ -105,6 +105,68 @@ int molecule(int N, int E, const std::vector<int>& ee, std::vector<int>& ptr,
   return STORM_OK;
 }
 
+// elemnt_db.f90:136-214: the higher-order least-squares molecule t2tHD
+// (ACTIVATE HDLS).  Row i: the other triangles on each of i's vertices, in
+// n2t order, then cde_t2t.  Rows left with fewer than three are augmented as
+// :168-197 write it: the loop index j = 2..nn names a TRIANGLE (grid(j)) and
+// a candidate is skipped when its 1-based column in the n2t row equals j.
+// Triangle numbers count from the first cell of i's reach (base), so that a
+// reach in a batch gets the molecule it gets alone.
+int hd_molecule(int N, const int32_t* tri, const std::vector<int>& n2t_ptr,
+                const std::vector<int>& n2t, const std::vector<int>& base,
+                std::vector<int>& ptr, std::vector<int>& cells, std::string& err) {
+  int maxc = 0;
+  for (size_t p = 0; p + 1 < n2t_ptr.size(); ++p) maxc = std::max(maxc, n2t_ptr[p + 1] - n2t_ptr[p]);
+  const int W = 4 * maxc + 3;  // columns, the count included
+  std::vector<int> w((size_t)N * W, 0);
+  auto row = [&](int i) { return &w[(size_t)i * W]; };
+  auto push = [&](int* r, int t) -> bool {
+    if (r[0] + 2 > W) return false;
+    r[++r[0]] = t;
+    return true;
+  };
+  for (int i = 0; i < N; ++i) {
+    int* ri = row(i);
+    for (int j = 0; j < 3; ++j) {
+      const int k = tri[3 * i + j];
+      for (int t = n2t_ptr[k]; t < n2t_ptr[k + 1]; ++t)
+        if (n2t[t] != i && !push(ri, n2t[t])) { err = "allocation bounds in T2TW"; return STORM_E_MESH; }
+    }
+    int c = ri[0];
+    cde_t2t(ri, c);
+    ri[0] = c;
+  }
+  for (int i = 0; i < N; ++i) {
+    int* ri = row(i);
+    const int n = ri[0];
+    if (n > 2) continue;
+    if (n < 1) { err = "triangle connectivity table incorrect"; return STORM_E_MESH; }
+    const int nn = std::min(3, n + 1);
+    for (int j = 2; j <= nn; ++j) {
+      const int tj = base[i] + j - 1;
+      if (tj >= N) { err = "triangle connectivity table incorrect"; return STORM_E_MESH; }
+      for (int k = 0; k < 3; ++k) {
+        const int l = tri[3 * tj + k];
+        const int cl = n2t_ptr[l + 1] - n2t_ptr[l];
+        for (int m = 2; m <= cl + 1; ++m) {
+          const int t = n2t[n2t_ptr[l] + m - 2];
+          if (t == i || m == j) continue;
+          if (!push(ri, t)) { err = "allocation bounds in T2TW"; return STORM_E_MESH; }
+        }
+      }
+    }
+    int c = ri[0];
+    cde_t2t(ri, c);
+    ri[0] = c;
+  }
+  ptr.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) ptr[i + 1] = ptr[i] + row(i)[0];
+  cells.resize(ptr[N]);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < row(i)[0]; ++k) cells[ptr[i] + k] = row(i)[k + 1];
+  return STORM_OK;
+}
+
 // Edge table: given, with the swap rule of tecplot_read.f90:139-142, or built.
 int edge_table(const storm_mesh_t* m, std::vector<int>& ep, std::vector<int>& ee,
                std::string& err) {
@@ -197,7 +259,7 @@ int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int
 int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const ReachSplit* split) {
   if (!m || !o) { S.err = "null mesh or options"; return STORM_E_ARG; }
   if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
-  if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 1 || o->limiter < 0 ||
+  if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 2 || o->limiter < 0 ||
       o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
       o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2) {
     S.err = "invalid option value"; return STORM_E_ARG;
@@ -404,8 +466,15 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
       }
   }
   // elemnt_db.f90:42-128 (molecule) and :226-233 (edge neighbours)
+  // with ACTIVATE HDLS the gradients use t2tHD instead (cell_grads.f90:37-42)
   std::vector<int> t2t_ptr, t2t;
-  rcode = molecule(N, E, ee, t2t_ptr, t2t, S.err);
+  if (o->gradient == STORM_GRAD_HDLS) {
+    std::vector<int> base(N);
+    for (int i = 0; i < N; ++i) base[i] = split->cell_ptr[cell_reach[i]];
+    rcode = hd_molecule(N, tri, n2t_ptr, n2t, base, t2t_ptr, t2t, S.err);
+  } else {
+    rcode = molecule(N, E, ee, t2t_ptr, t2t, S.err);
+  }
   if (rcode) return rcode;
   std::vector<int> t2t3(3 * (size_t)N);
   for (int i = 0; i < N; ++i)
@@ -463,7 +532,9 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     nrm[2 * k] = nrm[2 * k] / len[k]; nrm[2 * k + 1] = nrm[2 * k + 1] / len[k];
     tng[2 * k] = tng[2 * k] / len[k]; tng[2 * k + 1] = tng[2 * k + 1] / len[k];
   }
-  // lsq_setup.f90:105-145 (weighted, type 2)
+  // lsq_setup.f90:105-145 (weighted, type 2), or hdlsq_setup.f90:39-74
+  // (unweighted, type 1) on t2tHD
+  const bool hd = o->gradient == STORM_GRAD_HDLS;
   std::vector<double> mdx(t2t.size()), mdy(t2t.size()), mw(t2t.size()), ai(N), bi(N), ci(N), det(N);
   for (int i = 0; i < N; ++i) {
     double a = ZERO, b = ZERO, c = ZERO;
@@ -471,7 +542,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
       int l = t2t[t];
       mdx[t] = xc[l] - xc[i];
       mdy[t] = yc[l] - yc[i];
-      mw[t] = ONE / (mdx[t] * mdx[t] + mdy[t] * mdy[t]);
+      mw[t] = hd ? ONE : ONE / (mdx[t] * mdx[t] + mdy[t] * mdy[t]);
     }
     for (int t = t2t_ptr[i]; t < t2t_ptr[i + 1]; ++t) {
       a = a + mw[t] * mdx[t] * mdx[t];

@@ -42,8 +42,8 @@ SUBROUTINE dataprepFVT
 
   ! Options the GPU path implements (everything else stops the run).
   IF (opt_solver /= 2) CALL byebye('storm_fvt: only the FVT solver is provided.')
-  IF (activateHDLS .OR. activateRSAVG .OR. wind_terms .OR. culvert) &
-    CALL byebye('storm_fvt: HDLS, RSAVG, wind and culvert options are not provided.')
+  IF (activateRSAVG .OR. wind_terms .OR. culvert) &
+    CALL byebye('storm_fvt: RSAVG, wind and culvert options are not provided.')
   IF (opt_friction /= 0) CALL byebye('storm_fvt: only Manning friction is provided.')
   IF (nctr2vtx /= 3) CALL byebye('storm_fvt: only inverse-distance vertex weights.')
 
@@ -153,6 +153,7 @@ SUBROUTINE dataprepFVT
   o%btype = btype
   o%hvalve = MERGE(1, 0, hvalve)
   o%limiter = opt_limiter
+  o%gradient = MERGE(2, 0, activateHDLS)  ! STORM_GRAD_HDLS or STORM_GRAD_LSQ
   o%vclip = MERGE(1, 0, vclip)
   o%cfl = cfl; o%rstop = rstop; o%h_dry = h_dry; o%h_wet = h_wet; o%vceiling = vceiling
 

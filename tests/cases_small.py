@@ -30,6 +30,15 @@ def small_cases():
         "flume_q5_crit": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=21,
                                       dt_op=0, cfl=0.05, istop=30, in_kind=5, out_kind=2,
                                       q_in=12.0), 30),
+        # higher-order least squares (ACTIVATE HDLS): the vertex-neighbour
+        # molecule, unit weights; a wet/dry front and a meander with banks
+        "dambreak_hdls": (cases.dambreak(nq=14, side=200.0, seed=6, x_dam=80.0, h_res=2.0,
+                                         flux="ac", dt_op=0, cfl=0.02, istop=60, h_dry=1e-3,
+                                         gradient="hdls"), 60),
+        "meander_hdls_rk3": (cases.meander(ns=40, nn=12, length=400.0, width=120.0, amp=20.0,
+                                           wavelength=200.0, chan_w=40.0, chan_d=2.0,
+                                           n_bumps=5, q0=5.0, q1=20.0, t_ramp=10.0, dt_op=0,
+                                           cfl=0.02, istop=60, rkorder=3, gradient="hdls"), 60),
         # steady local time stepping that converges (RMS dh < 1e-4) at step 390
         "flume_steady_converged": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41,
                                                dt_op=0, cfl=0.3, istop=600, q_in=15.0,
