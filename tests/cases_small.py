@@ -39,6 +39,13 @@ def small_cases():
                                            wavelength=200.0, chan_w=40.0, chan_d=2.0,
                                            n_bumps=5, q0=5.0, q1=20.0, t_ramp=10.0, dt_op=0,
                                            cfl=0.02, istop=60, rkorder=3, gradient="hdls"), 60),
+        # a one-quad-wide strip: cell 1 has only two vertex neighbours, so
+        # t2tHD takes the augmentation of elemnt_db.f90:168-197 (whose row
+        # differs from one built from the molecule's own members); unlimited
+        # gradients, so that every cell's gradient enters the fluxes
+        "flume_strip_hdls": (cases.flume(nx=8, ny=1, length=40.0, width=5.0, seed=3, dt_op=0,
+                                         cfl=0.05, istop=30, q_in=6.0, h0=0.5, slope=0.01,
+                                         limiter=1, gradient="hdls"), 30),
         # steady local time stepping that converges (RMS dh < 1e-4) at step 390
         "flume_steady_converged": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41,
                                                dt_op=0, cfl=0.3, istop=600, q_in=15.0,

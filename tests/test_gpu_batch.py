@@ -42,6 +42,25 @@ def test_batch_equals_separate_runs():
     b.close()
 
 
+def test_hdls_batch_equals_separate_runs():
+    """ACTIVATE HDLS in a batch: each reach keeps the t2tHD molecule it has
+    alone, the augmented rows of one-quad-wide strips included (their
+    triangle numbers count from the reach's first cell)."""
+    rs = [cases.flume(nx=8, ny=1, length=40.0, width=5.0, seed=3 + k, q_in=6.0, h0=0.5,
+                      slope=0.01) for k in range(2)] + _reaches(range(2))
+    opts = dict(istop=30, dt_op=0, cfl=0.02, steady=False, limiter=1, gradient="hdls")
+    print(f"[gpu batch hdls] cells {[c.n_elems for c in rs]}", flush=True)
+    b = fvt.BatchSolver(rs, opts)
+    b.run(30)
+    b.finalize()
+    parts = b.split(b.state())
+    for c, p in zip(rs, parts):
+        s = _solo(c, opts, 30)
+        for k in KEYS:
+            assert np.array_equal(p[k], s[k]), k
+    b.close()
+
+
 def test_batch_steady_reaches_retire_independently():
     # reach 3 of this generator blows up in steady mode at CFL 0.3 (NaN
     # residual, in the oracle too), so it is left out here
