@@ -11,12 +11,22 @@
  *   storm_fvt_set_bc_values   the Q / stage values of get_time_bcs.f90:31-77
  *   storm_fvt_run             the time loop solverFVT.f90:129-478 (called at
  *                             solver.f90:57), nsteps at a time
- *   storm_fvt_finalize        the post-loop to_vertices of solverFVT.f90:483
+ *   storm_fvt_finalize        the post-loop work of solverFVT.f90:483-487:
+ *                             to_vertices (to_vertices.f90:1-68 with the whole
+ *                             of enforce_bcs.f90), ctr2vtx of cd, error_est(2)
+ *   storm_fvt_get_prep_fields the cell and vertex copies dataprepFVT.f90:69-140
+ *                             leaves in the modules (z, zvtx, zeta, cd, cdvtx)
  *   storm_fvt_get_state,
  *   storm_fvt_get_conserved,
  *   storm_fvt_get_vertex      what solverFVT leaves in the dep_vars / RKparams
  *                             modules (h,u,v,phi,drycell,rk*(:,rkorder),hvtx,
- *                             zetavtx; dep_vars.f90:16-57, RKparams.f90:19)
+ *                             uvtx,vvtx,zetavtx; dep_vars.f90:16-57,
+ *                             RKparams.f90:19)
+ *   storm_fvt_get_output      cdvtx and e_est after the loop (solverFVT.f90:
+ *                             484-487, error_est.f90:57-84)
+ *   storm_fvt_get_c2v_weights the centre-to-vertex weights of c2v_setup.f90:
+ *                             98-119 (type 3), which the module keeps for
+ *                             later ctr2vtx calls (write_cgns.f90:182,211)
  *   storm_fvt_get_gradients   delh, the depth gradient of the last stage
  *                             (cell_grads.f90:24-75, hgradfix2.f90:46-98),
  *                             which error_est(2) reads at solverFVT.f90:487
@@ -134,22 +144,49 @@ int storm_fvt_reach_status(storm_fvt_t *ctx, int32_t *iter, double *ttime, doubl
 /* Cell-centred h, u, v; restarts the time loop (iter = 0). */
 int storm_fvt_set_state(storm_fvt_t *ctx, const double *h, const double *u, const double *v);
 /* Replace every string's table by one constant: qin[n_in] (Q, or stage for
- * vtype 4) and hbc[n_out] (stage of htype 1 strings).  NULL keeps a side. */
+ * vtype 4) and hbc[n_out] (stage of htype 1 strings).  NULL keeps a side.
+ * A run in progress goes on from its current step and time: the new values
+ * replace the current ones at once (those of htype 1 strings only on the
+ * outflow side) and get_time_bcs keeps them. */
 int storm_fvt_set_bc_values(storm_fvt_t *ctx, const double *qin, const double *hbc);
 /* Advance up to nsteps time steps (fewer when istop is reached or a steady
  * run converges).  iter/ttime/rmax as solverFVT's dummy arguments. */
 int storm_fvt_run(storm_fvt_t *ctx, int32_t nsteps, int32_t *iter, double *ttime, double *rmax);
-/* The to_vertices call that ends solverFVT (zeroes u,v in partially wet
- * cells and refreshes the vertex depths). */
+/* The work that ends solverFVT (solverFVT.f90:483-487): to_vertices (zeroes
+ * u,v in partially wet cells; forms hvtx, zetavtx and the vertex velocities
+ * under enforce_bcs), cdvtx = ctr2vtx(cd) and e_est = error_est(2). */
 int storm_fvt_finalize(storm_fvt_t *ctx);
+/* What dataprepFVT leaves in the modules, in input order (any pointer may be
+ * NULL): the cell bed z (vtx2ctr of the vertex bed), zeta, cd [n_elems];
+ * zvtx, cdvtx [n_pts] (dataprepFVT.f90:73-140; ctr2vtx type 3 where a field
+ * is given at cells, vtx2ctr where at vertices).  Host data: any time. */
+int storm_fvt_get_prep_fields(storm_fvt_t *ctx, double *z, double *zvtx, double *zeta, double *cd,
+                              double *cdvtx);
 /* Cell state in input order; any pointer may be NULL.  phi: [3*n_elems],
  * component-major (phi(:,1), phi(:,2), phi(:,3)). */
 int storm_fvt_get_state(storm_fvt_t *ctx, double *h, double *u, double *v, double *phi,
                         uint8_t *drycell);
 /* rkh, rku, rkv(:,rkorder): h, hu, hv after the last step. */
 int storm_fvt_get_conserved(storm_fvt_t *ctx, double *qh, double *qu, double *qv);
-/* Vertex depth and water-surface elevation of the last to_vertices. */
-int storm_fvt_get_vertex(storm_fvt_t *ctx, double *hvtx, double *zetavtx);
+/* The vertex fields [n_pts] of the last to_vertices; any pointer may be
+ * NULL.  Before any step or set_state: dataprepFVT's (dataprepFVT.f90:
+ * 90-130).  After storm_fvt_finalize: solverFVT's final ones.  After steps
+ * with no finalize since: hvtx of the last stage's vertex pass, and
+ * STORM_E_STATE for the other three (after set_state with no step since:
+ * for all four). */
+int storm_fvt_get_vertex(storm_fvt_t *ctx, double *hvtx, double *uvtx, double *vvtx,
+                         double *zetavtx);
+/* After storm_fvt_finalize (STORM_E_STATE otherwise): cdvtx [n_pts] =
+ * ctr2vtx(cd) (solverFVT.f90:484) and e_est [n_elems] (error_est.f90:57-84,
+ * etype 2); either may be NULL. */
+int storm_fvt_get_output(storm_fvt_t *ctx, double *cdvtx, double *e_est);
+/* The centre-to-vertex weights of c2v_setup.f90:98-119 (type 3) as CSR in
+ * input numbering: vertex p's cells (node_db.f90:60-68 order) are
+ * cells[ptr[p] .. ptr[p+1]), with the normalised weights w (c2v_weights)
+ * and the inverse distances wid (c2v_weights_id).  ptr: [n_pts+1].  Returns
+ * the number of entries (all pointers NULL: only that) or a negative code. */
+int64_t storm_fvt_get_c2v_weights(storm_fvt_t *ctx, int32_t *ptr, int32_t *cells, double *w,
+                                  double *wid);
 /* delh(:)%x, delh(:)%y [n_elems] in input order, as the last stage formed
  * them from its input state (zero before the first step); either may be
  * NULL. */

@@ -84,6 +84,17 @@ struct orc {
   double *qts_t, *qts_v, *hts_t, *hts_v;
   double *qin, *hbc;
   double bch_max;
+  /* enforce_bcs tables: per string node the normal of ibc_arrays.f90:93-135
+   * (inflow, inward) and :183-253 (outflow, outward), per inflow string edge
+   * its length (l_inflow); wall points of walls.f90:150-176 and their
+   * tangents (ibc_arrays.f90:266-330) */
+  double *pn, *lin, *hn;
+  int n_wall, *wall_pts;
+  double *wtang;
+  /* post-loop fields (solverFVT.f90:483-487) */
+  double *uvtx, *vvtx, *cdvtx, *e_est;
+  /* dataprepFVT.f90:69-140's module copies, kept as set-up left them */
+  double *p_zeta, *p_cdvtx, *p_hvtx, *p_uvtx, *p_vvtx, *p_zetavtx;
   /* topology */
   int *n2t_ptr, *n2t;
   int *t2t_ptr, *t2t;
@@ -387,6 +398,83 @@ static int build(orc_t *s, const orc_case_t *c, char *err, int errlen) {
   if (nh) { memcpy(s->hts_t, c->hts_t, sizeof(double) * nh); memcpy(s->hts_v, c->hts_v, sizeof(double) * nh); }
   s->qin = xcalloc(s->n_in + 1, sizeof(double));
   s->hbc = xcalloc(s->n_out + 1, sizeof(double));
+  /* ibc_arrays.f90:93-135: inflow string node normals.  They are formed
+   * before dataprepFVT.f90:176-181 makes the edge normals unit, so each
+   * edge's normal still carries its length. */
+  s->pn = xcalloc(2 * (size_t)nin_nodes + 2, sizeof(double));
+  s->lin = xcalloc((size_t)nin_nodes + 1, sizeof(double));
+  for (int ch = 0; ch < s->n_in; ++ch) {
+    const int a = s->in_nodes_ptr[ch], n = s->in_nodes_ptr[ch + 1] - a;
+    double *pn = s->pn + 2 * (size_t)a;
+    for (int i = 0; i + 1 < n; ++i) {
+      const int e = s->in_edges[s->in_ptr[ch] + i];
+      const int t = edge_in_element(s, e);
+      const int j = local_edge(s, t, e);
+      if (s->gsign[3 * t + j] > 0) { pn[2 * i] = -s->nrm[2 * e]; pn[2 * i + 1] = -s->nrm[2 * e + 1]; }
+      else { pn[2 * i] = s->nrm[2 * e]; pn[2 * i + 1] = s->nrm[2 * e + 1]; }
+      s->lin[a + i] = s->len[e];
+    }
+    if (n > 1) { pn[2 * (n - 1)] = pn[2 * (n - 2)]; pn[2 * (n - 1) + 1] = pn[2 * (n - 2) + 1]; }
+    for (int i = n - 2; i >= 1; --i) {
+      pn[2 * i] = pn[2 * i] + pn[2 * (i - 1)];
+      pn[2 * i + 1] = pn[2 * i + 1] + pn[2 * (i - 1) + 1];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double l = sqrt(pn[2 * i] * pn[2 * i] + pn[2 * i + 1] * pn[2 * i + 1]);
+      pn[2 * i] = pn[2 * i] / l;
+      pn[2 * i + 1] = pn[2 * i + 1] / l;
+    }
+  }
+  /* ibc_arrays.f90:183-253 (outflow string nodes) and :266-330 (wall
+   * points): the sum of the outward normals of the first two boundary
+   * polygon edges through the node, made unit (for a wall point then turned
+   * by +90 degrees into a tangent) */
+  s->hn = xcalloc(2 * (size_t)nout_nodes + 2, sizeof(double));
+  {
+    int *isstr = xcalloc(P, sizeof(int));
+    for (int t = 0; t < nin_nodes; ++t) isstr[c->in_nodes[t]] = 1;
+    for (int t = 0; t < nout_nodes; ++t) isstr[c->out_nodes[t]] = 1;
+    /* walls.f90:150-176 */
+    int *seen = xcalloc(P, sizeof(int));
+    s->wall_pts = xcalloc(2 * (size_t)s->nwall1 + 1, sizeof(int));
+    s->n_wall = 0;
+    for (int i = 0; i < s->nwall1; ++i)
+      for (int j = 0; j < 2; ++j) {
+        const int p = s->ep[2 * s->walledg1[i] + j];
+        if (!seen[p] && !isstr[p]) { seen[p] = 1; s->wall_pts[s->n_wall++] = p; }
+      }
+    free(seen);
+    free(isstr);
+    s->wtang = xcalloc(2 * (size_t)s->n_wall + 2, sizeof(double));
+    for (int q = 0; q < nout_nodes + s->n_wall; ++q) {
+      const int node = (q < nout_nodes) ? s->out_nodes[q] : s->wall_pts[q - nout_nodes];
+      int e1 = -1, e2 = -1, counter = 0;
+      for (int j = 0; j < s->nbpoly; ++j) {
+        const int k = s->bpolygon[j];
+        if (node == s->ep[2 * k] || node == s->ep[2 * k + 1]) {
+          e2 = k;
+          counter = counter + 1;
+          if (counter == 2) break;
+          e1 = k;
+        }
+      }
+      if (counter == 0) { snprintf(err, errlen, "string or wall node %d on no boundary edge", node); return -1; }
+      double nn[2][2];
+      for (int w = 0; w < 2; ++w) {
+        const int e = w ? e2 : e1;
+        const int t = edge_in_element(s, e);
+        const double sg = (s->gsign[3 * t + local_edge(s, t, e)] > 0) ? ONE : -ONE;
+        nn[w][0] = (sg > 0) ? s->nrm[2 * e] : -s->nrm[2 * e];
+        nn[w][1] = (sg > 0) ? s->nrm[2 * e + 1] : -s->nrm[2 * e + 1];
+      }
+      double nx = nn[0][0] + nn[1][0], ny = nn[0][1] + nn[1][1];
+      const double l = sqrt(nx * nx + ny * ny);
+      nx = nx / l;
+      ny = ny / l;
+      if (q < nout_nodes) { s->hn[2 * q] = nx; s->hn[2 * q + 1] = ny; }
+      else { const int w = q - nout_nodes; s->wtang[2 * w] = -ny; s->wtang[2 * w + 1] = nx; }
+    }
+  }
 
   /* node_db.f90:32-68: n2t, cells in ascending order */
   s->n2t_ptr = xcalloc(P + 1, sizeof(int));
@@ -519,9 +607,15 @@ static int dataprep(orc_t *s, const orc_case_t *c) {
     ctr2vtx3(s, s->zeta, s->zetavtx);
     for (int i = 0; i < P; ++i) s->hvtx[i] = s->zetavtx[i] - s->zvtx[i];
   }
-  if (c->u_at_vtx) vtx2ctr(s, c->u, s->u); else memcpy(s->u, c->u, sizeof(double) * N);
-  if (c->v_at_vtx) vtx2ctr(s, c->v, s->v); else memcpy(s->v, c->v, sizeof(double) * N);
-  if (c->cd_at_vtx) vtx2ctr(s, c->cd, s->cd); else memcpy(s->cd, c->cd, sizeof(double) * N);
+  if (c->u_at_vtx) { vtx2ctr(s, c->u, s->u); memcpy(s->p_uvtx, c->u, sizeof(double) * P); }
+  else { memcpy(s->u, c->u, sizeof(double) * N); ctr2vtx3(s, c->u, s->p_uvtx); }
+  if (c->v_at_vtx) { vtx2ctr(s, c->v, s->v); memcpy(s->p_vvtx, c->v, sizeof(double) * P); }
+  else { memcpy(s->v, c->v, sizeof(double) * N); ctr2vtx3(s, c->v, s->p_vvtx); }
+  if (c->cd_at_vtx) { vtx2ctr(s, c->cd, s->cd); memcpy(s->p_cdvtx, c->cd, sizeof(double) * P); }
+  else { memcpy(s->cd, c->cd, sizeof(double) * N); ctr2vtx3(s, c->cd, s->p_cdvtx); }
+  memcpy(s->p_zeta, s->zeta, sizeof(double) * N);
+  memcpy(s->p_hvtx, s->hvtx, sizeof(double) * P);
+  memcpy(s->p_zetavtx, s->zetavtx, sizeof(double) * P);
   free(tmpv);
   /* dataprepFVT.f90:164-173: cell centre -> edge mid-point vectors */
   for (int i = 0; i < N; ++i)
@@ -663,6 +757,9 @@ orc_t *orc_create(const orc_case_t *c, const orc_opts_t *o, char *err, int errle
   s->phi = xcalloc(3 * N, 8);
   s->dry = xcalloc(N, 1);
   s->rk = xcalloc(3 * (size_t)(o->rkorder + 1) * N, 8);
+  s->uvtx = xcalloc(P, 8); s->vvtx = xcalloc(P, 8); s->cdvtx = xcalloc(P, 8); s->e_est = xcalloc(N, 8);
+  s->p_zeta = xcalloc(N, 8); s->p_cdvtx = xcalloc(P, 8); s->p_hvtx = xcalloc(P, 8);
+  s->p_uvtx = xcalloc(P, 8); s->p_vvtx = xcalloc(P, 8); s->p_zetavtx = xcalloc(P, 8);
   if (dataprep(s, c) != 0) { snprintf(err, errlen, "dataprep failed"); orc_destroy(s); return NULL; }
   return s;
 }
@@ -677,7 +774,9 @@ void orc_destroy(orc_t *s) {
                   s->c2v_w, s->c2v_wid, s->lsq_dx, s->lsq_dy, s->lsq_w, s->ai, s->bi, s->ci, s->det,
                   s->csortedz, s->gconn, s->gcx, s->gcy, s->gwt, s->z, s->zvtx, s->cd, s->h, s->u, s->v,
                   s->zeta, s->u_mag, s->wavec, s->rough, s->hvtx, s->zetavtx, s->delh, s->delu, s->delv,
-                  s->delz, s->gwork, s->phi, s->dry, s->rk};
+                  s->delz, s->gwork, s->phi, s->dry, s->rk, s->pn, s->lin, s->hn, s->wall_pts,
+                  s->wtang, s->uvtx, s->vvtx, s->cdvtx, s->e_est, s->p_zeta, s->p_cdvtx,
+                  s->p_hvtx, s->p_uvtx, s->p_vvtx, s->p_zetavtx};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(s);
 }
@@ -991,10 +1090,10 @@ static void enforce_bcs_h(orc_t *s) {
   }
 }
 
-/* to_vertices.f90:26-65 with ctr2vtxWSE.f90:89-105 (type 3) and
- * positivity_vtx.f90:22-27.  The vertex velocities it also produces are
- * output only and are not formed here. */
-static void to_vertices(orc_t *s) {
+/* to_vertices.f90:26-55 with ctr2vtxWSE.f90:89-105 (type 3) and
+ * positivity_vtx.f90:22-27: vertex water surface and depth, and the cell
+ * velocities zeroed in partially wet cells. */
+static void to_vertices_wse(orc_t *s) {
   for (int i = 0; i < s->P; ++i) {
     double id = ZERO, a = ZERO;
     int counter = 0;
@@ -1026,7 +1125,123 @@ static void to_vertices(orc_t *s) {
     for (int j = 0; j < 3; ++j) flag = flag && (s->hvtx[s->vtx[3 * i + j]] > s->o.h_dry);
     if (!flag) { s->u[i] = ZERO; s->v[i] = ZERO; }
   }
+}
+
+/* to_vertices as every RK stage calls it: the vertex velocities it also
+ * forms (:56-64) are output only, so of enforce_bcs only the stage strings'
+ * depths (:37-56), which the wall fluxes read, are formed here. */
+static void to_vertices(orc_t *s) {
+  to_vertices_wse(s);
   enforce_bcs_h(s);
+}
+
+/* vbc_by_h.f90:32-80 for inflow string ch on vertex depths h0: the node
+ * speeds v_inflow (zero at the string's end nodes). */
+static void vbc_by_h(const orc_t *s, int ch, const double *h0, double *vin) {
+  const int a = s->in_nodes_ptr[ch], n = s->in_nodes_ptr[ch + 1] - a;
+  const double qin = s->qin[ch], h_dry = s->o.h_dry;
+  double *hi = xcalloc(n, 8), *ai = xcalloc(n, 8), *ui = xcalloc(n, 8);
+  for (int i = 0; i < n; ++i) hi[i] = h0[s->in_nodes[a + i]];
+  double ht = ZERO, at = ZERO;
+  for (int i = 0; i + 1 < n; ++i) {
+    ai[i] = s->lin[a + i] * (hi[i] + hi[i + 1]) * HALF;
+    ht = ht + (hi[i] + hi[i + 1]) * HALF;
+    at = at + ai[i];
+  }
+  for (int i = 0; i + 1 < n; ++i) ui[i] = HALF * (hi[i] + hi[i + 1]) * qin / ht / at;
+  for (int i = 0; i < n; ++i) vin[i] = ZERO;
+  for (int i = 1; i + 1 < n; ++i)
+    if (hi[i] > h_dry) vin[i] = HALF * (ui[i - 1] + ui[i]);
+  for (int i = 0; i + 1 < n; ++i) ui[i] = HALF * (vin[i] + vin[i + 1]);
+  double q = ZERO;
+  for (int i = 0; i + 1 < n; ++i) q = q + ui[i] * ai[i];
+  for (int i = 1; i + 1 < n; ++i)
+    if (hi[i] > h_dry) vin[i] = vin[i] * qin / q;
+  free(hi); free(ai); free(ui);
+}
+
+/* enforce_bcs.f90:30-171 on the vertex fields (hvtx, uvtx, vvtx, zvtx). */
+static void enforce_bcs(orc_t *s) {
+  double *h0 = s->hvtx, *u0 = s->uvtx, *v0 = s->vvtx;
+  const double h_dry = s->o.h_dry;
+  for (int ch = 0; ch < s->n_out; ++ch) {
+    const int a = s->out_nodes_ptr[ch], b = s->out_nodes_ptr[ch + 1];
+    const int ht = s->out_type[ch];
+    if (ht == 1)
+      for (int t = a; t < b; ++t) {
+        const int j = s->out_nodes[t];
+        h0[j] = fmax2(ZERO, s->hbc[ch] - s->zvtx[j]);
+      }
+    if (ht == 0 || (ht == 1 && s->o.hvalve))
+      for (int t = a; t < b; ++t) {
+        const int j = s->out_nodes[t];
+        const double dot = u0[j] * s->hn[2 * t] + v0[j] * s->hn[2 * t + 1];
+        if (dot < ZERO) { u0[j] = ZERO; v0[j] = ZERO; }
+      }
+  }
+  if (s->o.btype == 0) {
+    for (int i = 0; i < s->n_wall; ++i) { u0[s->wall_pts[i]] = ZERO; v0[s->wall_pts[i]] = ZERO; }
+  } else {
+    for (int i = 0; i < s->n_wall; ++i) {
+      const int p = s->wall_pts[i];
+      const double tx = s->wtang[2 * i], ty = s->wtang[2 * i + 1];
+      const double dot = u0[p] * tx + v0[p] * ty;
+      u0[p] = dot * tx;
+      v0[p] = dot * ty;
+    }
+  }
+  for (int i = 0; i < s->P; ++i)
+    if (h0[i] < h_dry) { u0[i] = ZERO; v0[i] = ZERO; }
+  for (int ch = 0; ch < s->n_in; ++ch) {
+    const int a = s->in_nodes_ptr[ch], n = s->in_nodes_ptr[ch + 1] - a;
+    const int vt = s->in_type[ch];
+    if (vt != 0 && vt != 1 && vt != 5) continue;
+    double *vin = xcalloc(n, 8);
+    if (vt == 0) for (int i = 0; i < n; ++i) vin[i] = s->qin[ch];
+    else vbc_by_h(s, ch, h0, vin);
+    for (int i = 0; i < n; ++i) {
+      const int j = s->in_nodes[a + i];
+      if (h0[j] > h_dry) { u0[j] = vin[i] * s->pn[2 * (a + i)]; v0[j] = vin[i] * s->pn[2 * (a + i) + 1]; }
+    }
+    free(vin);
+  }
+}
+
+/* The to_vertices call that ends solverFVT (solverFVT.f90:483): the water
+ * surface and depth of to_vertices.f90:26-55, then the vertex velocities of
+ * :56-65 (ctr2vtx of u and v, wetdryvfix.f90:30-49 at no-slip walls, zero at
+ * dry vertices) and the whole of enforce_bcs. */
+static void to_vertices_final(orc_t *s) {
+  to_vertices_wse(s);
+  ctr2vtx3(s, s->u, s->uvtx);
+  ctr2vtx3(s, s->v, s->vvtx);
+  if (s->o.btype == 0)
+    for (int i = 0; i < s->P; ++i) {
+      int wett = 0, dryt = 0;
+      for (int j = s->n2t_ptr[i]; j < s->n2t_ptr[i + 1]; ++j) {
+        if (s->dry[s->n2t[j]]) dryt = 1; else wett = 1;
+      }
+      if (wett && dryt) { s->uvtx[i] = ZERO; s->vvtx[i] = ZERO; }
+    }
+  for (int i = 0; i < s->P; ++i)
+    if (s->hvtx[i] < s->o.h_dry) { s->uvtx[i] = ZERO; s->vvtx[i] = ZERO; }
+  enforce_bcs(s);
+}
+
+/* error_est.f90:57-84 (etype 2) */
+static void error_est2(orc_t *s) {
+  for (int i = 0; i < s->N; ++i) {
+    s->e_est[i] = ZERO;
+    if (s->h[i] < s->o.h_dry) continue;
+    int n1 = s->vtx[3 * i], n2 = s->vtx[3 * i + 1], n3 = s->vtx[3 * i + 2];
+    double gzx = (s->zvtx[n1] * (s->y[n2] - s->y[n3]) + s->zvtx[n2] * (s->y[n3] - s->y[n1]) +
+                  s->zvtx[n3] * (s->y[n1] - s->y[n2])) * HALF / s->area[i];
+    double gzy = (s->zvtx[n1] * (s->x[n3] - s->x[n2]) + s->zvtx[n2] * (s->x[n1] - s->x[n3]) +
+                  s->zvtx[n3] * (s->x[n2] - s->x[n1])) * HALF / s->area[i];
+    double l = s->perim[i] * ONE_THIRD;
+    double gx = gzx + s->delh[2 * i], gy = gzy + s->delh[2 * i + 1];
+    s->e_est[i] = sqrt(gx * gx + gy * gy) * l / (s->z[i] + s->h[i]);
+  }
 }
 
 static inline void face(const orc_t *s, int i, int m, double *h, double *u, double *v) {
@@ -1650,9 +1865,40 @@ int orc_run(orc_t *s, int nsteps, int32_t *iter, double *ttime, double *rmax) {
   return taken;
 }
 
+/* The post-loop part of solverFVT.f90:483-487: to_vertices (with the vertex
+ * velocities and the whole of enforce_bcs), ctr2vtx of cd, error_est(2). */
 void orc_finalize(orc_t *s) {
   if (!s->started) start(s);
-  to_vertices(s);
+  to_vertices_final(s);
+  ctr2vtx3(s, s->cd, s->cdvtx);
+  error_est2(s);
+}
+
+void orc_get_output(const orc_t *s, double *uvtx, double *vvtx, double *cdvtx, double *e_est,
+                    double *delhx, double *delhy) {
+  const int N = s->N, P = s->P;
+  if (uvtx) memcpy(uvtx, s->uvtx, sizeof(double) * P);
+  if (vvtx) memcpy(vvtx, s->vvtx, sizeof(double) * P);
+  if (cdvtx) memcpy(cdvtx, s->cdvtx, sizeof(double) * P);
+  if (e_est) memcpy(e_est, s->e_est, sizeof(double) * N);
+  for (int i = 0; i < N; ++i) {
+    if (delhx) delhx[i] = s->delh[2 * i];
+    if (delhy) delhy[i] = s->delh[2 * i + 1];
+  }
+}
+
+void orc_get_prep(const orc_t *s, double *z, double *zvtx, double *zeta, double *cd,
+                  double *cdvtx, double *hvtx, double *uvtx, double *vvtx, double *zetavtx) {
+  const int N = s->N, P = s->P;
+  if (z) memcpy(z, s->z, sizeof(double) * N);
+  if (zvtx) memcpy(zvtx, s->zvtx, sizeof(double) * P);
+  if (zeta) memcpy(zeta, s->p_zeta, sizeof(double) * N);
+  if (cd) memcpy(cd, s->cd, sizeof(double) * N);
+  if (cdvtx) memcpy(cdvtx, s->p_cdvtx, sizeof(double) * P);
+  if (hvtx) memcpy(hvtx, s->p_hvtx, sizeof(double) * P);
+  if (uvtx) memcpy(uvtx, s->p_uvtx, sizeof(double) * P);
+  if (vvtx) memcpy(vvtx, s->p_vvtx, sizeof(double) * P);
+  if (zetavtx) memcpy(zetavtx, s->p_zetavtx, sizeof(double) * P);
 }
 
 void orc_get_state(const orc_t *s, double *h, double *u, double *v, double *qh, double *qu,
@@ -1670,6 +1916,34 @@ void orc_get_state(const orc_t *s, double *h, double *u, double *v, double *qh, 
   if (phi) memcpy(phi, s->phi, sizeof(double) * 3 * N);
   if (hvtx) memcpy(hvtx, s->hvtx, sizeof(double) * s->P);
   if (zetavtx) memcpy(zetavtx, s->zetavtx, sizeof(double) * s->P);
+}
+
+/* Every string's table replaced by one constant (as the library's
+ * storm_fvt_set_bc_values does; no reference counterpart: a caller of the
+ * reference would edit the module tables).  A started run keeps its step and
+ * time; the new values replace the current ones at once (htype 1 strings
+ * only on the outflow side). */
+void orc_set_bc_values(orc_t *s, const double *qin, const double *hbc) {
+  if (qin) {
+    free(s->qts_t); free(s->qts_v);
+    s->qts_t = xcalloc(s->n_in + 1, sizeof(double));
+    s->qts_v = xcalloc(s->n_in + 1, sizeof(double));
+    for (int k = 0; k <= s->n_in; ++k) s->qts_ptr[k] = k;
+    for (int k = 0; k < s->n_in; ++k) {
+      s->qts_v[k] = qin[k];
+      if (s->started) s->qin[k] = qin[k];
+    }
+  }
+  if (hbc) {
+    free(s->hts_t); free(s->hts_v);
+    s->hts_t = xcalloc(s->n_out + 1, sizeof(double));
+    s->hts_v = xcalloc(s->n_out + 1, sizeof(double));
+    for (int k = 0; k <= s->n_out; ++k) s->hts_ptr[k] = k;
+    for (int k = 0; k < s->n_out; ++k) {
+      s->hts_v[k] = hbc[k];
+      if (s->started && s->out_type[k] == 1) s->hbc[k] = hbc[k];
+    }
+  }
 }
 
 void orc_set_state(orc_t *s, const double *h, const double *u, const double *v) {

@@ -67,15 +67,27 @@ void orc_set_pow_perturbation(double p, uint64_t seed);
 /* Advance up to nsteps time steps (solverFVT.f90:129-478); stops early when a
  * steady run converges or iter reaches istop.  Returns steps taken. */
 int orc_run(orc_t *s, int nsteps, int32_t *iter, double *ttime, double *rmax);
-/* The post-loop part of solverFVT.f90:483 (to_vertices), which zeroes cell
- * velocities in partially wet cells. */
+/* The post-loop part of solverFVT.f90:483-487: to_vertices (which zeroes
+ * cell velocities in partially wet cells and forms the vertex fields, the
+ * velocities under enforce_bcs included), ctr2vtx(cd), error_est(2). */
 void orc_finalize(orc_t *s);
+/* After orc_finalize: uvtx, vvtx, cdvtx [P]; e_est, delh (x, y) [N]. */
+void orc_get_output(const orc_t *s, double *uvtx, double *vvtx, double *cdvtx, double *e_est,
+                    double *delhx, double *delhy);
+/* The module copies dataprepFVT.f90:69-140 leaves: z, zeta, cd [N];
+ * zvtx, cdvtx, hvtx, uvtx, vvtx, zetavtx [P]. */
+void orc_get_prep(const orc_t *s, double *z, double *zvtx, double *zeta, double *cd,
+                  double *cdvtx, double *hvtx, double *uvtx, double *vvtx, double *zetavtx);
 /* h,u,v: cell state; qh,qu,qv: RK slot rkorder; phi: [3N] component-major;
  * dry: [N]; hvtx,zetavtx: [P].  Any pointer may be NULL. */
 void orc_get_state(const orc_t *s, double *h, double *u, double *v, double *qh,
                    double *qu, double *qv, double *phi, int32_t *dry,
                    double *hvtx, double *zetavtx);
 void orc_set_state(orc_t *s, const double *h, const double *u, const double *v);
+/* Replace every string's table by one constant; a started run goes on with
+ * the new values from its next step (qin [n_in], hbc [n_out]; NULL keeps a
+ * side). */
+void orc_set_bc_values(orc_t *s, const double *qin, const double *hbc);
 /* Per-routine entry points (pinned against closed forms). */
 void orc_flux(int32_t kind, double nx, double ny, double hl, double ul, double vl,
               double hr, double ur, double vr, double cl, double cr, double h0,

@@ -77,6 +77,9 @@ def lib():
         L.orc_topology.restype = C.c_int64
         L.orc_topology.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.orc_set_pow_perturbation.argtypes = [C.c_double, C.c_uint64]
+        L.orc_get_output.argtypes = [C.c_void_p] + [_dp] * 6
+        L.orc_set_bc_values.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_get_prep.argtypes = [C.c_void_p] + [_dp] * 9
         _lib = L
     return _lib
 
@@ -112,8 +115,15 @@ class Oracle:
         k["tri"] = np.ascontiguousarray(case.tri, dtype=np.int32)
         k["ep"] = np.ascontiguousarray(case.edge_p, dtype=np.int32)
         k["ee"] = np.ascontiguousarray(case.edge_e, dtype=np.int32)
-        for nm in ("zvtx", "h", "u", "v", "cd"):
-            k[nm] = np.ascontiguousarray(getattr(case, nm), dtype=np.float64)
+        # fields given at the other location (Case.alt): the bed at cells,
+        # the rest at vertices
+        alt = getattr(case, "alt", {})
+        at = {"z": "z" not in alt}
+        k["z"] = np.ascontiguousarray(alt["z"] if "z" in alt else case.zvtx, dtype=np.float64)
+        for nm in ("h", "u", "v", "cd"):
+            at[nm] = nm in alt
+            k[nm] = np.ascontiguousarray(alt[nm] if nm in alt else getattr(case, nm),
+                                         dtype=np.float64)
 
         def csr(bs, field):
             ptr = np.zeros(len(bs) + 1, dtype=np.int32)
@@ -136,8 +146,9 @@ class Oracle:
         cs = OrcCase(n_pts=case.n_pts, n_elems=case.n_elems, n_edges=case.n_edges,
                      x=_p(k["x"], C.c_double), y=_p(k["y"], C.c_double),
                      tri=_p(k["tri"], C.c_int32), edge_p=_p(k["ep"], C.c_int32),
-                     edge_e=_p(k["ee"], C.c_int32), z_at_vtx=1, h_at_vtx=0, u_at_vtx=0,
-                     v_at_vtx=0, cd_at_vtx=0, z=_p(k["zvtx"], C.c_double),
+                     edge_e=_p(k["ee"], C.c_int32), z_at_vtx=int(at["z"]), h_at_vtx=int(at["h"]),
+                     u_at_vtx=int(at["u"]), v_at_vtx=int(at["v"]), cd_at_vtx=int(at["cd"]),
+                     z=_p(k["z"], C.c_double),
                      h=_p(k["h"], C.c_double), u=_p(k["u"], C.c_double),
                      v=_p(k["v"], C.c_double), cd=_p(k["cd"], C.c_double),
                      n_in=len(ins), in_type=_p(k["in_type"], C.c_int32),
@@ -182,11 +193,35 @@ class Oracle:
         out.update(phi1=phi[:N], phi2=phi[N:2 * N], phi3=phi[2 * N:], dry=dry, hvtx=hv, zetavtx=zv)
         return out
 
+    def output(self) -> Dict[str, np.ndarray]:
+        """The post-loop fields after finalize(): uvtx, vvtx, cdvtx, e_est,
+        delh (delhx, delhy)."""
+        N, P = self.N, self.P
+        out = {k: np.zeros(P) for k in ("uvtx", "vvtx", "cdvtx")}
+        out.update({k: np.zeros(N) for k in ("e_est", "delhx", "delhy")})
+        lib().orc_get_output(self.h, *[_p(out[k], C.c_double) for k in
+                                       ("uvtx", "vvtx", "cdvtx", "e_est", "delhx", "delhy")])
+        return out
+
+    def prep(self) -> Dict[str, np.ndarray]:
+        """The module copies dataprepFVT leaves (cells: z, zeta, cd; vertices:
+        zvtx, cdvtx, hvtx, uvtx, vvtx, zetavtx)."""
+        N, P = self.N, self.P
+        keys = ("z", "zvtx", "zeta", "cd", "cdvtx", "hvtx", "uvtx", "vvtx", "zetavtx")
+        out = {k: np.zeros(N if k in ("z", "zeta", "cd") else P) for k in keys}
+        lib().orc_get_prep(self.h, *[_p(out[k], C.c_double) for k in keys])
+        return out
+
     def set_state(self, h, u, v):
         h = np.ascontiguousarray(h, dtype=np.float64)
         u = np.ascontiguousarray(u, dtype=np.float64)
         v = np.ascontiguousarray(v, dtype=np.float64)
         lib().orc_set_state(self.h, _p(h, C.c_double), _p(u, C.c_double), _p(v, C.c_double))
+
+    def set_bc_values(self, qin=None, hbc=None):
+        q = None if qin is None else np.ascontiguousarray(qin, dtype=np.float64)
+        hb = None if hbc is None else np.ascontiguousarray(hbc, dtype=np.float64)
+        lib().orc_set_bc_values(self.h, _p(q, C.c_double), _p(hb, C.c_double))
 
     def cell_grads(self) -> np.ndarray:
         g = np.zeros(6 * self.N)
@@ -234,6 +269,10 @@ def run_reference(case, workdir: str, timeout: float = 600.0):
     run = _cases.write_run(case, workdir)
     out = os.path.join(workdir, "ref_out.bin")
     with open(os.path.join(workdir, "ref_log.txt"), "w") as log:
-        subprocess.run([REF_DRIVER, os.path.basename(run), "ref_out.bin"], cwd=workdir,
-                       stdout=log, stderr=subprocess.STDOUT, check=True, timeout=timeout)
-    return _cases.read_ref_output(out)
+        subprocess.run([REF_DRIVER, os.path.basename(run), "ref_out.bin", "ref_prep.bin"],
+                       cwd=workdir, stdout=log, stderr=subprocess.STDOUT, check=True,
+                       timeout=timeout)
+    res = _cases.read_ref_output(out)
+    prep = _cases.read_prep_output(os.path.join(workdir, "ref_prep.bin"), case.n_elems, case.n_pts)
+    res.update({"prep_" + k: v for k, v in prep.items()})
+    return res

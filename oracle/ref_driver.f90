@@ -7,7 +7,12 @@
 ! solver.f90:50,57 (dataprepFVT, solverFVT), without the command-line prompt
 ! of read_data.f90:41-52 and without the final Tecplot/CGNS writers.
 !
-! Usage: ref_driver <case.run> <out.bin>
+! Usage: ref_driver <case.run> <out.bin> [prep.bin]
+!
+! With a third argument the module copies dataprepFVT leaves are written to
+! it right after dataprepFVT (same encoding):
+!   real64 z(N), zvtx(P), zeta(N), cd(N), cdvtx(P), h(N), u(N), v(N),
+!          hvtx(P), uvtx(P), vvtx(P), zetavtx(P)
 !
 ! The wall time of the solverFVT call alone is printed on standard output as
 ! "solverFVT seconds <t>" (the CPU baseline of the benchmark reads it).
@@ -30,7 +35,7 @@ PROGRAM ref_driver
   USE io
   USE RKparams
   IMPLICIT NONE
-  CHARACTER (LEN=250) :: fname, oname
+  CHARACTER (LEN=250) :: fname, oname, pname
   CHARACTER (LEN=160) :: title
   INTEGER :: iter, i, u_out
   INTEGER (KIND=8) :: c0, c1, crate
@@ -42,6 +47,15 @@ PROGRAM ref_driver
   cgns = .FALSE.
   CALL read_mfa(fname, title)
   CALL dataprepFVT
+  IF (COMMAND_ARGUMENT_COUNT() >= 3) THEN
+    CALL GET_COMMAND_ARGUMENT(3, pname)
+    OPEN (NEWUNIT=u_out, FILE=pname, ACCESS='STREAM', FORM='UNFORMATTED', &
+          STATUS='REPLACE')
+    WRITE (u_out) z(1:n_elems), zvtx(1:n_pts), zeta(1:n_elems), cd(1:n_elems), &
+                  cdvtx(1:n_pts), h(1:n_elems), u(1:n_elems), v(1:n_elems), &
+                  hvtx(1:n_pts), uvtx(1:n_pts), vvtx(1:n_pts), zetavtx(1:n_pts)
+    CLOSE (u_out)
+  END IF
   CALL SYSTEM_CLOCK(c0, crate)
   CALL solverFVT(iter, ttime, rmax, fname, title)
   CALL SYSTEM_CLOCK(c1)

@@ -58,6 +58,10 @@ class Case:
     outflows: List[Boundary]
     opts: Dict[str, object]
     meta: Dict[str, object] = dataclasses.field(default_factory=dict)
+    # Inputs given at the other location than the fields above, as a grid
+    # file may hold them (dataprepFVT.f90:73-140 takes either): "z" the bed
+    # at the cells (N); "h", "u", "v", "cd" values at the vertices (P).
+    alt: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
 
     @property
     def n_elems(self) -> int:
@@ -70,6 +74,15 @@ class Case:
     @property
     def n_edges(self) -> int:
         return int(self.edge_p.shape[0])
+
+
+def inputs(case: Case) -> Dict[str, Tuple[np.ndarray, bool]]:
+    """The initial data as the reference reads it: name -> (values, given at
+    the vertices) for z, h, u, v and cd."""
+    out = {"z": (case.alt["z"], False) if "z" in case.alt else (case.zvtx, True)}
+    for k in ("h", "u", "v", "cd"):
+        out[k] = (case.alt[k], True) if k in case.alt else (getattr(case, k), False)
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -405,9 +418,12 @@ def write_run(case: Case, dirpath: str, out_prefix: str = "case") -> str:
     with open(grid, "w") as f:
         f.write('TITLE = "%s"\n' % case.name)
         f.write('VARIABLES = "X" "Y" "Z" "H" "U" "V" "N" "V8" "V9" "V10" "V11"\n')
-        f.write("ZONE N=%d, E=%d, DATAPACKING=BLOCK, ZONETYPE=FETRIANGLE, "
-                "VARLOCATION=([04,05,06,07]=CELLCENTERED)\n" % (P, N))
-        for arr in (case.x, case.y, case.zvtx, case.h, case.u, case.v, case.cd):
+        data = inputs(case)
+        cells = ["%02d" % (3 + j) for j, k in enumerate(("z", "h", "u", "v", "cd"))
+                 if not data[k][1]]
+        loc = (", VARLOCATION=([%s]=CELLCENTERED)" % ",".join(cells)) if cells else ""
+        f.write("ZONE N=%d, E=%d, DATAPACKING=BLOCK, ZONETYPE=FETRIANGLE%s\n" % (P, N, loc))
+        for arr in (case.x, case.y) + tuple(data[k][0] for k in ("z", "h", "u", "v", "cd")):
             f.write(_fmt(arr))
             f.write("\n")
         zeros = "\n".join(["0.0"] * P) + "\n"
@@ -487,4 +503,19 @@ def read_ref_output(path: str) -> Dict[str, np.ndarray]:
             out[k] = np.frombuffer(f.read(8 * p), dtype=np.float64).copy()
         for k in ("delhx", "delhy", "e_est"):
             out[k] = np.frombuffer(f.read(8 * n), dtype=np.float64).copy()
+    return out
+
+
+PREP_KEYS = (("z", "c"), ("zvtx", "p"), ("zeta", "c"), ("cd", "c"), ("cdvtx", "p"), ("h", "c"),
+             ("u", "c"), ("v", "c"), ("hvtx", "p"), ("uvtx", "p"), ("vvtx", "p"),
+             ("zetavtx", "p"))
+
+
+def read_prep_output(path: str, n: int, p: int) -> Dict[str, np.ndarray]:
+    """Read the module copies the reference driver dumps after dataprepFVT."""
+    out: Dict[str, np.ndarray] = {}
+    with open(path, "rb") as f:
+        for k, where in PREP_KEYS:
+            m = n if where == "c" else p
+            out[k] = np.frombuffer(f.read(8 * m), dtype=np.float64).copy()
     return out

@@ -528,7 +528,7 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
   Vtx v;
   v.wet = hv > o.h_dry;
   const int b = g.vbc[p];
-  if (b >= 0) hv = dmax(ZERO, sh[b] - q.z);
+  if (b >= 0 && sh) hv = dmax(ZERO, sh[b] - q.z);
   v.h = hv;
   v.zeta = zeta;
   return v;
@@ -607,7 +607,10 @@ k_vertex_first(Geo g, Opts o, const double* zeta, uint8_t* vwet, const int* vdon
   vertex_stage(g, o, zeta, A.hvtx, vwet, A.ctrl, vdone, A.sh, 1, p);
 }
 
-// All vertices after the time loop (solverFVT.f90:483), with zetavtx.
+// All vertices after the time loop (solverFVT.f90:483), with zetavtx.  With
+// sh null the stage strings' depths are left as interpolated (the final pass
+// sets them in k_out_vertex, after the velocity tests that read the
+// interpolated depths).
 __global__ void __launch_bounds__(256)
 k_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, double* hvtx, uint8_t* vwet,
          double* zetavtx, const double* sh) {
@@ -1550,6 +1553,164 @@ __global__ void k_delh(Geo g, Opts o, const CellD* din, const double* gwork, dou
   dh[N + i] = F.dhy;
 }
 
+// ---------------------------------------------------------------- output fields
+// Tables of the final to_vertices' velocity part (enforce_bcs.f90): per
+// vertex the outflow-string normals its velocity is checked against and its
+// wall tangent; per inflow string its nodes, node normals and edge lengths.
+struct OutTab {
+  const int *vo_ptr, *vwall;
+  const double2 *vo_nrm, *wtang;
+  const int *is_ptr, *is_node;
+  const double2* is_pn;
+  const double* is_len;
+};
+
+// to_vertices.f90:56-65 after the cell pass, for vertex p: the vertex
+// velocities and drag coefficient by ctr2vtx.f90:73-80 (type 3: the
+// inverse-distance weights of c2v_setup.f90:106-119 normalised by their sum,
+// over every incident cell in n2t order), wetdryvfix.f90:30-49 (no-slip
+// walls), zero velocity where the interpolated depth is below h_dry, then
+// the vertex part of enforce_bcs.f90:30-106: the stage strings' depths, the
+// outflow strings' backflow test, the walls and the dry vertices.  hvtx holds
+// the interpolated depth on entry (k_vertex with no strings) and the final
+// depth on exit.  The inflow strings are k_out_inflow's.
+__global__ void __launch_bounds__(256)
+k_out_vertex(Geo g, Opts o, const CellD* d, const uint8_t* dry, OutTab T, const double* sh,
+             double* hvtx, double* uvtx, double* vvtx, double* cdvtx) {
+  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  const double4 q = g.vxyz[p];
+  const int t0 = g.vptr[p], t1 = g.vptr[p + 1];
+  double tdist = ZERO;
+  for (int t = t0; t < t1; ++t) {
+    const CellS sk = ld(&g.cs[g.vcell[t]]);
+    const double xk = sk.xc - q.x, yk = sk.yc - q.y;
+    tdist = tdist + ONE / sqrt(xk * xk + yk * yk);
+  }
+  double a = ZERO, b = ZERO, c = ZERO;
+  bool wett = false, dryt = false;
+  for (int t = t0; t < t1; ++t) {
+    const int k = g.vcell[t];
+    const CellS sk = ld(&g.cs[k]);
+    const CellD dk = ld(&d[k]);
+    const double xk = sk.xc - q.x, yk = sk.yc - q.y;
+    const double w = (ONE / sqrt(xk * xk + yk * yk)) / tdist;
+    a = a + w * dk.u;
+    b = b + w * dk.v;
+    c = c + w * sk.cd;
+    if (dry ? (dry[k] != 0) : (dk.dry != ZERO)) dryt = true; else wett = true;
+  }
+  double u = a, v = b;
+  if (o.btype == 0 && wett && dryt) { u = ZERO; v = ZERO; }
+  const double hpre = hvtx[p];
+  if (hpre < o.h_dry) { u = ZERO; v = ZERO; }
+  const int vb = g.vbc[p];
+  const double h = (vb >= 0) ? dmax(ZERO, sh[vb] - q.z) : hpre;
+  for (int e = T.vo_ptr[p]; e < T.vo_ptr[p + 1]; ++e) {
+    const double2 n = T.vo_nrm[e];
+    if (u * n.x + v * n.y < ZERO) { u = ZERO; v = ZERO; }
+  }
+  const int w = T.vwall[p];
+  if (w >= 0) {
+    if (o.btype == 0) { u = ZERO; v = ZERO; }
+    else {
+      const double2 tg = T.wtang[w];
+      const double dot = u * tg.x + v * tg.y;
+      u = dot * tg.x;
+      v = dot * tg.y;
+    }
+  }
+  if (h < o.h_dry) { u = ZERO; v = ZERO; }
+  hvtx[p] = h;
+  uvtx[p] = u;
+  vvtx[p] = v;
+  cdvtx[p] = c;
+}
+
+// enforce_bcs.f90:117-149 with vbc_by_h.f90:32-80: the velocities of the
+// inflow strings' nodes (vtype 0: the discharge value along the node normal;
+// vtype 1 and 5: the node speeds that carry the string's discharge on the
+// vertex depths), one block per reach, the reach's strings in order (a node
+// on two strings keeps the later one's value).  vtype 1/5 strings have at
+// most QMAX edges; their speeds are formed by thread 0 in string order.
+__global__ void __launch_bounds__(256)
+k_out_inflow(Geo g, Opts o, OutTab T, const double* sq, const double* hvtx, double* uvtx,
+             double* vvtx) {
+  __shared__ double s_v[QMAX + 1], s_u[QMAX + 1], s_a[QMAX + 1];
+  const RInfo ri = g.reach[blockIdx.x];
+  const int t = threadIdx.x;
+  const double h_dry = o.h_dry;
+  for (int ch = ri.in_base; ch < ri.in_base + ri.n_in; ++ch) {
+    const int vt = g.in_type[ch];
+    if (vt != 0 && vt != 1 && vt != 5) continue;
+    const int a0 = T.is_ptr[ch], n = T.is_ptr[ch + 1] - a0;
+    const double qin = sq[ch];
+    if (vt != 0 && t == 0) {
+      // vbc_by_h.f90:45-73
+      double ht = ZERO, at = ZERO;
+      for (int i = 0; i + 1 < n; ++i) {
+        const double h1 = hvtx[T.is_node[a0 + i]], h2 = hvtx[T.is_node[a0 + i + 1]];
+        s_a[i] = T.is_len[a0 + i] * (h1 + h2) * HALF;
+        ht = ht + (h1 + h2) * HALF;
+        at = at + s_a[i];
+      }
+      for (int i = 0; i + 1 < n; ++i) {
+        const double h1 = hvtx[T.is_node[a0 + i]], h2 = hvtx[T.is_node[a0 + i + 1]];
+        s_u[i] = HALF * (h1 + h2) * qin / ht / at;
+      }
+      for (int i = 0; i < n; ++i) s_v[i] = ZERO;
+      for (int i = 1; i + 1 < n; ++i)
+        if (hvtx[T.is_node[a0 + i]] > h_dry) s_v[i] = HALF * (s_u[i - 1] + s_u[i]);
+      for (int i = 0; i + 1 < n; ++i) s_u[i] = HALF * (s_v[i] + s_v[i + 1]);
+      double qq = ZERO;
+      for (int i = 0; i + 1 < n; ++i) qq = qq + s_u[i] * s_a[i];
+      for (int i = 1; i + 1 < n; ++i)
+        if (hvtx[T.is_node[a0 + i]] > h_dry) s_v[i] = s_v[i] * qin / qq;
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += blockDim.x) {
+      const int p = T.is_node[a0 + i];
+      if (hvtx[p] > h_dry) {
+        const double sp = (vt == 0) ? qin : s_v[i];
+        const double2 pn = T.is_pn[a0 + i];
+        uvtx[p] = sp * pn.x;
+        vvtx[p] = sp * pn.y;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// error_est.f90:57-84 (etype 2), for cell i: the water-surface gradient (the
+// bed's P1 gradient plus delh, which cell_grads and hgradfix2 form from the
+// last stage's input) times a third of the perimeter over the water-surface
+// elevation, on cells with h >= h_dry.  dh_in: that stage's input (grad 0:
+// no stage has run); dfin: the final state.
+__global__ void k_out_cells(Geo g, Opts o, const CellD* dh_in, const double* gwork,
+                            const CellD* dfin, int grad, double* e_est) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  const int N = g.N;
+  const double h = dfin[i].h;
+  if (h < o.h_dry) { e_est[i] = ZERO; return; }
+  const Tri tr = load_tri(g, i);
+  Faces F;
+  F.dhx = ZERO;  // before the first stage delh is zero (solverFVT.f90:63)
+  F.dhy = ZERO;
+  if (grad)
+    cell_faces(g, o, dh_in, gwork, i, g.cmeta[i], g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i],
+               tr, true, F);
+  const double area = tri_area(tr);
+  const double gzx = (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                      tr.zv[2] * (tr.y[0] - tr.y[1])) * HALF / area;
+  const double gzy = (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                      tr.zv[2] * (tr.x[1] - tr.x[0])) * HALF / area;
+  const double perim = side<0>(tr).len + side<1>(tr).len + side<2>(tr).len;
+  const double l = perim * ONE_THIRD;
+  const double gx = gzx + F.dhx, gy = gzy + F.dhy;
+  e_est[i] = sqrt(gx * gx + gy * gy) * l / (g.cs[i].z + h);
+}
+
 // ---------------------------------------------------------------- host side
 #define HIPCHK(x)                                                                       \
   do {                                                                                  \
@@ -1602,6 +1763,19 @@ struct storm_fvt {
   int* d_out_type = nullptr;
   int *d_qts_ptr = nullptr, *d_hts_ptr = nullptr;
   double *d_qts_t = nullptr, *d_qts_v = nullptr, *d_hts_t = nullptr, *d_hts_v = nullptr;
+  // entries the Q(t) and H(t) table buffers hold (at least one per string,
+  // so that storm_fvt_set_bc_values writes into them in place)
+  size_t cap_q = 0, cap_h = 0;
+  // the final to_vertices' tables and the post-loop fields (vertex
+  // velocities, vertex drag coefficient, error estimates)
+  OutTab tab{};
+  double *d_uvtx = nullptr, *d_vvtx = nullptr, *d_cdvtx = nullptr, *d_eest = nullptr;
+  // Where the vertex fields of get_vertex come from: dataprepFVT's (until a
+  // step or set_state), the last finalize's, or none that is current (steps
+  // or set_state since); vpass: hvtx on the device is the last stage's
+  // (a step has run since the last set_state)
+  enum { V_PREP, V_FINAL, V_NONE } vsrc = V_PREP;
+  bool vpass = false;
   Tiles tl{};
   int cap = 0;
   std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
@@ -1673,16 +1847,25 @@ void host_time_bcs(const Prep& P, double time, double* qin, double* hbc) {
   }
 }
 
+// The boundary tables on the device: buffers sized for the larger of the
+// given tables and one entry per string, filled from the host copies.
 void upload_tables(storm_fvt* C) {
   Prep& P = C->prep;
   auto& own = C->owned;
+  C->cap_q = std::max<size_t>({P.qts_t.size(), (size_t)P.n_in, 1});
+  C->cap_h = std::max<size_t>({P.hts_t.size(), (size_t)P.n_out, 1});
   C->d_out_type = dupload(own, P.out_type);
   C->d_qts_ptr = dupload(own, P.qts_ptr);
   C->d_hts_ptr = dupload(own, P.hts_ptr);
-  C->d_qts_t = dupload(own, P.qts_t);
-  C->d_qts_v = dupload(own, P.qts_v);
-  C->d_hts_t = dupload(own, P.hts_t);
-  C->d_hts_v = dupload(own, P.hts_v);
+  C->d_qts_t = dalloc<double>(own, C->cap_q);
+  C->d_qts_v = dalloc<double>(own, C->cap_q);
+  C->d_hts_t = dalloc<double>(own, C->cap_h);
+  C->d_hts_v = dalloc<double>(own, C->cap_h);
+  auto put = [](double* d, const std::vector<double>& h) {
+    if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  };
+  put(C->d_qts_t, P.qts_t); put(C->d_qts_v, P.qts_v);
+  put(C->d_hts_t, P.hts_t); put(C->d_hts_v, P.hts_v);
 }
 
 void set_lds_limit(int cap) {
@@ -1781,6 +1964,19 @@ void setup_device(storm_fvt* C) {
   S.sq = dalloc<double>(own, P.n_in);
   S.sh = dalloc<double>(own, P.n_out);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
+  OutTab& T = C->tab;
+  T.vo_ptr = dupload(own, P.vo_ptr);
+  T.vwall = dupload(own, P.vwall);
+  T.vo_nrm = reinterpret_cast<const double2*>(dupload(own, P.vo_nrm));
+  T.wtang = reinterpret_cast<const double2*>(dupload(own, P.wtang));
+  T.is_ptr = dupload(own, P.is_ptr);
+  T.is_node = dupload(own, P.is_node);
+  T.is_pn = reinterpret_cast<const double2*>(dupload(own, P.is_pn));
+  T.is_len = dupload(own, P.is_len);
+  C->d_uvtx = dalloc<double>(own, P.P);
+  C->d_vvtx = dalloc<double>(own, P.P);
+  C->d_cdvtx = dalloc<double>(own, P.P);
+  C->d_eest = dalloc<double>(own, N);
 }
 
 template <int CAP>
@@ -2103,6 +2299,8 @@ int storm_fvt_set_state(storm_fvt_t* C, const double* h, const double* u, const 
     HIPCHK(hipMemcpyAsync(C->d_v0, P.v0.data(), sizeof(double) * P.N, hipMemcpyHostToDevice, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->started = false;
+    C->vsrc = storm_fvt::V_NONE;
+    C->vpass = false;
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
@@ -2113,6 +2311,8 @@ int storm_fvt_set_bc_values(storm_fvt_t* C, const double* qin, const double* hbc
   if (!C) return STORM_E_ARG;
   try {
     Prep& P = C->prep;
+    // no launched step may still read the tables or the current values
+    HIPCHK(hipStreamSynchronize(C->stream));
     if (qin) {
       P.qts_ptr.assign(P.n_in + 1, 0);
       for (int k = 0; k <= P.n_in; ++k) P.qts_ptr[k] = k;
@@ -2125,13 +2325,34 @@ int storm_fvt_set_bc_values(storm_fvt_t* C, const double* qin, const double* hbc
       P.hts_t.assign(P.n_out, 0.0);
       P.hts_v.assign(hbc, hbc + P.n_out);
     }
-    upload_tables(C);
-    if (C->gexec) {  // the graph holds the old tables' addresses
-      HIPCHK(hipStreamSynchronize(C->stream));
-      (void)hipGraphExecDestroy(C->gexec);
-      C->gexec = nullptr;
+    // in place: the buffers hold one entry per string (upload_tables), and a
+    // captured graph keeps their addresses
+    auto put = [](void* d, const void* h, size_t bytes) {
+      if (bytes) HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    };
+    if (qin) {
+      put(C->d_qts_ptr, P.qts_ptr.data(), sizeof(int) * P.qts_ptr.size());
+      put(C->d_qts_t, P.qts_t.data(), sizeof(double) * P.n_in);
+      put(C->d_qts_v, P.qts_v.data(), sizeof(double) * P.n_in);
     }
-    C->started = false;
+    if (hbc) {
+      put(C->d_hts_ptr, P.hts_ptr.data(), sizeof(int) * P.hts_ptr.size());
+      put(C->d_hts_t, P.hts_t.data(), sizeof(double) * P.n_out);
+      put(C->d_hts_v, P.hts_v.data(), sizeof(double) * P.n_out);
+    }
+    // A run in progress goes on from its current step with the new values:
+    // they replace the current ones now (get_time_bcs keeps them from the
+    // next step's end on, and a steady run never calls it again)
+    if (C->started) {
+      if (qin) put(C->S.sq, qin, sizeof(double) * P.n_in);
+      if (hbc && P.n_out) {
+        std::vector<double> sh(P.n_out);
+        HIPCHK(hipMemcpy(sh.data(), C->S.sh, sizeof(double) * P.n_out, hipMemcpyDeviceToHost));
+        for (int k = 0; k < P.n_out; ++k)
+          if (P.out_type[k] == 1) sh[k] = hbc[k];
+        put(C->S.sh, sh.data(), sizeof(double) * P.n_out);
+      }
+    }
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
@@ -2188,6 +2409,9 @@ int storm_fvt_run(storm_fvt_t* C, int32_t nsteps, int32_t* iter, double* ttime, 
       all_done = all_done && hc.done;
     }
     if (all_done) C->iter_host = C->prep.o.istop;
+    bool moved = false;  // some reach took a step in this call
+    for (int r = 0; r < nr; ++r) moved = moved || C->hctrl[r].iter != 0;
+    if (moved && n > 0) { C->vsrc = storm_fvt::V_NONE; C->vpass = true; }
     if (C->prep.nr == 1 && C->hctrl[0].iter - iter0 < n) {
       C->S.din = din0;
       C->S.dout = dout0;
@@ -2211,19 +2435,29 @@ int storm_fvt_finalize(storm_fvt_t* C) {
     if (!C->started) start_run(C);
     Prep& P = C->prep;
     const int bs = 256;
+    const dim3 gv((P.P + bs - 1) / bs), gc((P.N + bs - 1) / bs);
     // to_vertices after the loop (solverFVT.f90:483): the water surface of
     // the last stage's start and the dry flags of the post-step
-    // find_dry_cells (:335)
+    // find_dry_cells (:335); then ctr2vtx of cd (:484) and error_est(2)
+    // (:487) on the last stage's delh
     const bool stepped = C->iter_host > 0;
-    hipLaunchKernelGGL(k_vertex, dim3((P.P + bs - 1) / bs), dim3(bs), 0, C->stream, C->g, C->o,
-                       C->last_din, stepped ? C->S.dry_post : (const uint8_t*)nullptr,
-                       C->S.hvtx, C->S.vwet, C->S.zetavtx, C->S.sh);
-    hipLaunchKernelGGL(k_final_cells, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g,
-                       stepped ? C->S.dry_post : (const uint8_t*)nullptr, C->S.vwet,
-                       const_cast<CellD*>(C->S.din));
+    const uint8_t* dry = stepped ? C->S.dry_post : (const uint8_t*)nullptr;
+    CellD* dfin = const_cast<CellD*>(C->S.din);
+    hipLaunchKernelGGL(k_vertex, gv, dim3(bs), 0, C->stream, C->g, C->o, C->last_din, dry,
+                       C->S.hvtx, C->S.vwet, C->S.zetavtx, (const double*)nullptr);
+    hipLaunchKernelGGL(k_final_cells, gc, dim3(bs), 0, C->stream, C->g, dry, C->S.vwet, dfin);
+    hipLaunchKernelGGL(k_out_vertex, gv, dim3(bs), 0, C->stream, C->g, C->o, dfin, dry, C->tab,
+                       C->S.sh, C->S.hvtx, C->d_uvtx, C->d_vvtx, C->d_cdvtx);
+    hipLaunchKernelGGL(k_out_inflow, dim3(P.nr), dim3(bs), 0, C->stream, C->g, C->o, C->tab,
+                       C->S.sq, C->S.hvtx, C->d_uvtx, C->d_vvtx);
+    if (stepped && C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
+      hipLaunchKernelGGL(k_gauss1, gc, dim3(bs), 0, C->stream, C->g, C->last_din, C->S.gwork);
+    hipLaunchKernelGGL(k_out_cells, gc, dim3(bs), 0, C->stream, C->g, C->o, C->last_din,
+                       C->S.gwork, (const CellD*)dfin, (int)stepped, C->d_eest);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(C->stream));
     C->finalized = true;
+    C->vsrc = storm_fvt::V_FINAL;
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
@@ -2290,24 +2524,91 @@ int storm_fvt_get_conserved(storm_fvt_t* C, double* qh, double* qu, double* qv) 
   return STORM_OK;
 }
 
-int storm_fvt_get_vertex(storm_fvt_t* C, double* hvtx, double* zetavtx) {
+static int copy_vertices(storm_fvt_t* C, const double* d, double* out) {
+  Prep& P = C->prep;
+  std::vector<double> tmp(P.P);
+  HIPCHK(hipMemcpy(tmp.data(), d, sizeof(double) * P.P, hipMemcpyDeviceToHost));
+  for (int pp = 0; pp < P.P; ++pp) out[P.vperm[pp]] = tmp[pp];
+  return STORM_OK;
+}
+
+int storm_fvt_get_vertex(storm_fvt_t* C, double* hvtx, double* uvtx, double* vvtx,
+                         double* zetavtx) {
   if (!C) return STORM_E_ARG;
   try {
-    HIPCHK(hipStreamSynchronize(C->stream));
     Prep& P = C->prep;
-    std::vector<double> tmp(P.P);
-    if (hvtx) {
-      HIPCHK(hipMemcpy(tmp.data(), C->S.hvtx, sizeof(double) * P.P, hipMemcpyDeviceToHost));
-      for (int pp = 0; pp < P.P; ++pp) hvtx[P.vperm[pp]] = tmp[pp];
+    if (C->vsrc == storm_fvt::V_PREP) {  // what dataprepFVT formed
+      dataprep_fields(P, nullptr, nullptr, nullptr, nullptr, nullptr, hvtx, uvtx, vvtx, zetavtx);
+      return STORM_OK;
     }
-    if (zetavtx) {
-      HIPCHK(hipMemcpy(tmp.data(), C->S.zetavtx, sizeof(double) * P.P, hipMemcpyDeviceToHost));
-      for (int pp = 0; pp < P.P; ++pp) zetavtx[P.vperm[pp]] = tmp[pp];
-    }
+    if (C->vsrc == storm_fvt::V_NONE && ((uvtx || vvtx || zetavtx) || (hvtx && !C->vpass)))
+      return set_err(C, STORM_E_STATE,
+                     "the vertex fields are formed by storm_fvt_finalize (steps or set_state since)");
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (hvtx) copy_vertices(C, C->S.hvtx, hvtx);
+    if (uvtx) copy_vertices(C, C->d_uvtx, uvtx);
+    if (vvtx) copy_vertices(C, C->d_vvtx, vvtx);
+    if (zetavtx) copy_vertices(C, C->S.zetavtx, zetavtx);
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
   return STORM_OK;
+}
+
+int storm_fvt_get_prep_fields(storm_fvt_t* C, double* z, double* zvtx, double* zeta, double* cd,
+                              double* cdvtx) {
+  if (!C) return STORM_E_ARG;
+  try {
+    dataprep_fields(C->prep, z, zvtx, zeta, cd, cdvtx, nullptr, nullptr, nullptr, nullptr);
+  } catch (const std::bad_alloc&) {
+    return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_get_output(storm_fvt_t* C, double* cdvtx, double* e_est) {
+  if (!C) return STORM_E_ARG;
+  if (C->vsrc != storm_fvt::V_FINAL)
+    return set_err(C, STORM_E_STATE, "the post-loop fields are formed by storm_fvt_finalize");
+  try {
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (cdvtx) copy_vertices(C, C->d_cdvtx, cdvtx);
+    if (e_est) copy_cells(C, C->d_eest, e_est);
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  }
+  return STORM_OK;
+}
+
+int64_t storm_fvt_get_c2v_weights(storm_fvt_t* C, int32_t* ptr, int32_t* cells, double* w,
+                                  double* wid) {
+  if (!C) return STORM_E_ARG;
+  const Prep& P = C->prep;
+  const size_t PP = (size_t)P.P;
+  const int64_t total = P.vptr[P.P];
+  if (!ptr && !cells && !w && !wid) return total;
+  // input vertex p's row starts where the rows of vertices 0..p-1 end
+  std::vector<int64_t> start(P.P + 1, 0);
+  for (int p = 0; p < P.P; ++p) {
+    const int pp = P.viperm[p];
+    start[p + 1] = start[p] + (P.vptr[pp + 1] - P.vptr[pp]);
+  }
+  for (int p = 0; p < P.P; ++p) {
+    const int pp = P.viperm[p];
+    const int deg = P.vptr[pp + 1] - P.vptr[pp];
+    if (ptr) ptr[p] = (int32_t)start[p];
+    double tdist = 0.0;
+    for (int t = 0; t < deg; ++t) tdist = tdist + P.vwt[(size_t)t * PP + pp];
+    for (int t = 0; t < deg; ++t) {
+      const int64_t at = start[p] + t;
+      const double wi = P.vwt[(size_t)t * PP + pp];
+      if (cells) cells[at] = P.perm[P.vell[(size_t)t * PP + pp]];
+      if (w) w[at] = wi / tdist;
+      if (wid) wid[at] = wi;
+    }
+  }
+  if (ptr) ptr[P.P] = (int32_t)start[P.P];
+  return total;
 }
 
 int storm_fvt_get_gradients(storm_fvt_t* C, double* dhx, double* dhy) {

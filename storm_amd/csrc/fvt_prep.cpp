@@ -410,6 +410,9 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     int kind = (vt == 1) ? SK_IN_Q1 : (vt == 5) ? SK_IN_Q5 : (vt == 4) ? SK_IN_H : SK_NONE;
     if (vt < 0 || vt > 5) { S.err = "invalid value in vtype"; return STORM_E_ARG; }
     if (vt == 2) { S.err = "VTYPE = 2: option not implemented (enforce_bcs.f90:151-155)"; return STORM_E_ARG; }
+    // enforce_bcs.f90:157-163 sets both velocity components of a vtype-3
+    // string from two tables (timeserq, timeserq2); the mesh carries one
+    if (vt == 3) { S.err = "VTYPE = 3 (two velocity tables) is not provided"; return STORM_E_ARG; }
     for (size_t p = 0; p < ce.size(); ++p) {
       if (cnt[ce[p]] != 1) { S.err = "inflow string runs through the interior of the mesh"; return STORM_E_MESH; }
       qbar_e[ce[p]] = 1;
@@ -438,6 +441,98 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     if (cnt[k] == 1 && !wall[k] && e_kind[k] < 0) {
       S.err = "wall edges and inflow/outflow edges do not add up"; return STORM_E_MESH;
     }
+  // enforce_bcs' tables (input ids; mapped to the internal order below).
+  // ibc_arrays runs before dataprepFVT.f90:176-181 makes the edge normals
+  // unit, so nrm still carries each edge's length here.
+  auto edge_sign = [&](int k) {  // grid(t)%edge sign in the cell holding boundary edge k
+    const int t = ee[2 * k];
+    for (int l = 0; l < 3; ++l) if (gedge[3 * t + l] == k) return gsign[3 * t + l];
+    return 0;
+  };
+  // ibc_arrays.f90:93-135: inflow string node normals, l_inflow
+  std::vector<double> in_pn(2 * (size_t)nin_nodes, 0.0), in_len((size_t)nin_nodes, 0.0);
+  for (int ch = 0; ch < m->n_in; ++ch) {
+    const int a = m->in_ptr[ch], n = m->in_ptr[ch + 1] - a;
+    double* pn = &in_pn[2 * (size_t)a];
+    for (int i = 0; i + 1 < n; ++i) {
+      const int k = in_edges[ch][i];
+      const double sg = edge_sign(k) > 0 ? -1.0 : 1.0;
+      pn[2 * i] = sg < 0 ? -nrm[2 * k] : nrm[2 * k];
+      pn[2 * i + 1] = sg < 0 ? -nrm[2 * k + 1] : nrm[2 * k + 1];
+      in_len[a + i] = len[k];
+    }
+    pn[2 * (n - 1)] = pn[2 * (n - 2)];
+    pn[2 * (n - 1) + 1] = pn[2 * (n - 2) + 1];
+    for (int i = n - 2; i >= 1; --i) {
+      pn[2 * i] = pn[2 * i] + pn[2 * (i - 1)];
+      pn[2 * i + 1] = pn[2 * i + 1] + pn[2 * (i - 1) + 1];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double l = std::sqrt(pn[2 * i] * pn[2 * i] + pn[2 * i + 1] * pn[2 * i + 1]);
+      pn[2 * i] = pn[2 * i] / l;
+      pn[2 * i + 1] = pn[2 * i + 1] / l;
+    }
+  }
+  // walls.f90:150-176: wall points (walledg1 endpoints on no string)
+  std::vector<int> wall_pts;
+  {
+    std::vector<char> seen(P, 0);
+    for (int k = 0; k < E; ++k) {
+      if (cnt[k] != 1 || !wall[k]) continue;
+      for (int j = 0; j < 2; ++j) {
+        const int p = ep[2 * k + j];
+        if (!seen[p] && !isbc[p]) { seen[p] = 1; wall_pts.push_back(p); }
+      }
+    }
+  }
+  // ibc_arrays.f90:183-253,266-330: outflow string node normals and wall
+  // tangents from the first two boundary-polygon edges through the node
+  std::vector<std::vector<int>> node_bedges(P);
+  for (int k = 0; k < E; ++k)
+    if (cnt[k] == 1)
+      for (int j = 0; j < 2; ++j) {
+        std::vector<int>& be = node_bedges[ep[2 * k + j]];
+        if (be.size() < 2) be.push_back(k);
+      }
+  auto node_normal = [&](int p, double& nx, double& ny) -> bool {
+    const std::vector<int>& be = node_bedges[p];
+    if (be.empty()) return false;
+    const int e1 = be[0], e2 = be.size() > 1 ? be[1] : be[0];
+    const double s1 = edge_sign(e1) > 0 ? 1.0 : -1.0, s2 = edge_sign(e2) > 0 ? 1.0 : -1.0;
+    const double n1x = s1 > 0 ? nrm[2 * e1] : -nrm[2 * e1], n1y = s1 > 0 ? nrm[2 * e1 + 1] : -nrm[2 * e1 + 1];
+    const double n2x = s2 > 0 ? nrm[2 * e2] : -nrm[2 * e2], n2y = s2 > 0 ? nrm[2 * e2 + 1] : -nrm[2 * e2 + 1];
+    nx = n1x + n2x;
+    ny = n1y + n2y;
+    const double l = std::sqrt(nx * nx + ny * ny);
+    nx = nx / l;
+    ny = ny / l;
+    return true;
+  };
+  std::vector<int> on_vtx;          // (vertex, normal) entries of the velocity check
+  std::vector<double> on_nrm;
+  for (int ch = 0; ch < m->n_out; ++ch) {
+    const int ht = m->out_type[ch];
+    const bool check = ht == 0 || (ht == 1 && o->hvalve);
+    for (int t = m->out_ptr[ch]; t < m->out_ptr[ch + 1]; ++t) {
+      double nx, ny;
+      if (!node_normal(m->out_nodes[t], nx, ny)) {
+        S.err = "outflow string node on no boundary edge"; return STORM_E_MESH;
+      }
+      if (!check) continue;
+      on_vtx.push_back(m->out_nodes[t]);
+      on_nrm.push_back(nx);
+      on_nrm.push_back(ny);
+    }
+  }
+  std::vector<double> wall_tg(2 * wall_pts.size());
+  for (size_t w = 0; w < wall_pts.size(); ++w) {
+    double nx, ny;
+    if (!node_normal(wall_pts[w], nx, ny)) { S.err = "wall point on no boundary edge"; return STORM_E_MESH; }
+    wall_tg[2 * w] = -ny;
+    wall_tg[2 * w + 1] = nx;
+  }
+  node_bedges.clear();
+  node_bedges.shrink_to_fit();
   // time tables
   S.qts_ptr.assign(m->n_in + 1, 0);
   S.hts_ptr.assign(m->n_out + 1, 0);
@@ -511,6 +606,11 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     }
   };
   // dataprepFVT.f90:73-140
+  S.vin_h.clear(); S.vin_u.clear(); S.vin_v.clear(); S.vin_cd.clear();
+  if (m->h_at_vtx) S.vin_h.assign(m->h, m->h + P);
+  if (m->u_at_vtx) S.vin_u.assign(m->u, m->u + P);
+  if (m->v_at_vtx) S.vin_v.assign(m->v, m->v + P);
+  if (m->cd_at_vtx) S.vin_cd.assign(m->cd, m->cd + P);
   std::vector<double> z, zvtx, h, u, v, cd, tmp;
   if (m->z_at_vtx) { vtx2ctr(m->z, z); zvtx.assign(m->z, m->z + P); }
   else { ctr2vtx(m->z, zvtx); vtx2ctr(zvtx.data(), z); }
@@ -753,6 +853,28 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     S.vzb[2 * (size_t)pp] = zvtx[p];
     std::memcpy(&S.vzb[2 * (size_t)pp + 1], &bits, sizeof(bits));
   }
+  // enforce_bcs' tables in the internal vertex order
+  S.vo_ptr.assign(P + 1, 0);
+  for (int v : on_vtx) S.vo_ptr[vip[v] + 1]++;
+  for (int pp = 0; pp < P; ++pp) S.vo_ptr[pp + 1] += S.vo_ptr[pp];
+  S.vo_nrm.assign(on_nrm.size(), 0.0);
+  {
+    std::vector<int> fill(S.vo_ptr.begin(), S.vo_ptr.end() - 1);
+    for (size_t e = 0; e < on_vtx.size(); ++e) {
+      const int at = fill[vip[on_vtx[e]]]++;
+      S.vo_nrm[2 * (size_t)at] = on_nrm[2 * e];
+      S.vo_nrm[2 * (size_t)at + 1] = on_nrm[2 * e + 1];
+    }
+  }
+  S.vwall.assign(P, -1);
+  S.wtang = wall_tg;
+  for (size_t w = 0; w < wall_pts.size(); ++w) S.vwall[vip[wall_pts[w]]] = (int)w;
+  S.is_ptr.assign(m->n_in + 1, 0);
+  S.is_node.resize(nin_nodes);
+  for (int ch = 0; ch < m->n_in; ++ch) S.is_ptr[ch + 1] = m->in_ptr[ch + 1];
+  for (int t = 0; t < nin_nodes; ++t) S.is_node[t] = vip[m->in_nodes[t]];
+  S.is_pn = in_pn;
+  S.is_len = in_len;
   // inflowQbyH strings: (cell, local edge) in string order, with the edge's
   // rc vector, unit normal (as the reference orients it) and length
   S.qs_ptr.assign(m->n_in + 1, 0);
@@ -795,6 +917,66 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     }
   }
   return STORM_OK;
+}
+
+void dataprep_fields(const Prep& S, double* z, double* zvtx, double* zeta, double* cd,
+                     double* cdvtx, double* hvtx, double* uvtx, double* vvtx, double* zetavtx) {
+  const int N = S.N, P = S.P;
+  const size_t PP = (size_t)P;
+  // ctr2vtx type 3 of a cell field (internal order) at every vertex, written
+  // in input vertex order
+  auto ctr2vtx = [&](const std::vector<double>& f, double* out) {
+    for (int pp = 0; pp < P; ++pp) {
+      const int deg = S.vptr[pp + 1] - S.vptr[pp];
+      double tdist = ZERO;
+      for (int t = 0; t < deg; ++t) tdist = tdist + S.vwt[(size_t)t * PP + pp];
+      double a = ZERO;
+      for (int t = 0; t < deg; ++t)
+        a = a + (S.vwt[(size_t)t * PP + pp] / tdist) * f[S.vell[(size_t)t * PP + pp]];
+      out[S.vperm[pp]] = a;
+    }
+  };
+  auto cells_out = [&](const std::vector<double>& f, double* out) {
+    for (int ii = 0; ii < N; ++ii) out[S.perm[ii]] = f[ii];
+  };
+  std::vector<double> zv(P);  // zvtx, input order
+  for (int pp = 0; pp < P; ++pp) zv[S.vperm[pp]] = S.vxyz[4 * (size_t)pp + 2];
+  if (z) cells_out(S.z, z);
+  if (zvtx) std::copy(zv.begin(), zv.end(), zvtx);
+  if (cd) cells_out(S.cd, cd);
+  if (cdvtx) {
+    if (!S.vin_cd.empty()) std::copy(S.vin_cd.begin(), S.vin_cd.end(), cdvtx);
+    else ctr2vtx(S.cd, cdvtx);
+  }
+  if (uvtx) {
+    if (!S.vin_u.empty()) std::copy(S.vin_u.begin(), S.vin_u.end(), uvtx);
+    else ctr2vtx(S.u0, uvtx);
+  }
+  if (vvtx) {
+    if (!S.vin_v.empty()) std::copy(S.vin_v.begin(), S.vin_v.end(), vvtx);
+    else ctr2vtx(S.v0, vvtx);
+  }
+  if (!zeta && !hvtx && !zetavtx) return;
+  std::vector<double> zc(N), zvt(P);  // zeta (internal order), zetavtx (input order)
+  if (!S.vin_h.empty()) {
+    // dataprepFVT.f90:90-96
+    for (int p = 0; p < P; ++p) zvt[p] = S.vin_h[p] + zv[p];
+    for (int ii = 0; ii < N; ++ii) {
+      const int* cv = &S.cvtx[0];
+      const double a = zvt[S.vperm[cv[ii]]], b = zvt[S.vperm[cv[(size_t)N + ii]]],
+                   c = zvt[S.vperm[cv[2 * (size_t)N + ii]]];
+      zc[ii] = (a + b + c) * ONE_THIRD;
+    }
+    if (hvtx) std::copy(S.vin_h.begin(), S.vin_h.end(), hvtx);
+  } else {
+    // dataprepFVT.f90:97-104
+    for (int ii = 0; ii < N; ++ii) zc[ii] = S.z[ii] + S.h0[ii];
+    ctr2vtx(zc, zvt.data());
+    if (hvtx)
+      for (int p = 0; p < P; ++p) hvtx[p] = zvt[p] - zv[p];
+  }
+  if (zeta) cells_out(zc, zeta);
+  if (zetavtx) std::copy(zvt.begin(), zvt.end(), zetavtx);
 }
 
 int build_tiles(Prep& S, int cap) {

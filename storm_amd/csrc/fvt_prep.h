@@ -94,6 +94,22 @@ struct Prep {
   std::vector<int> r_svptr, r_svtx;
   std::vector<double> r_bch;
   std::vector<double> h0, u0, v0;     // initial state (internal order)
+  // The final to_vertices' velocity part (to_vertices.f90:56-65 with
+  // enforce_bcs.f90).  Per vertex (internal order): the outflow-string node
+  // normals of ibc_arrays.f90:183-253 its velocity is checked against
+  // (htype 0, or htype 1 with hvalve) as vo_nrm[2*vo_ptr[p] ..
+  // 2*vo_ptr[p+1]), and the index of its wall tangent (ibc_arrays.f90:
+  // 266-330) in wtang, -1 if it is no wall point (walls.f90:150-176).
+  std::vector<int> vo_ptr, vwall;
+  std::vector<double> vo_nrm, wtang;
+  // Every inflow string's nodes (internal ids) in string order, with the
+  // node normals of ibc_arrays.f90:93-135 (interleaved x, y) and, at each
+  // node but the last, the length of the edge to the next node (l_inflow).
+  std::vector<int> is_ptr, is_node;
+  std::vector<double> is_pn, is_len;
+  // dataprepFVT.f90:73-140's inputs given at vertices, in input order (empty
+  // when given at cells), for the module copies it leaves
+  std::vector<double> vin_h, vin_u, vin_v, vin_cd;
   // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
   // (edge neighbours outside) is listed per tile; local cell ids are 16
   // bits, three to a word (cell slot j at bit 16*j): tile cells 0..T-1, then
@@ -119,6 +135,14 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& P,
 // Tiles of at most `cap` cells + halo cells, none spanning two reaches
 // (needs the cell tables of build_prep).
 int build_tiles(Prep& P, int cap);
+
+// The module copies dataprepFVT.f90:69-140 leaves, in input order (any
+// pointer may be null): cells z, zeta, cd; vertices zvtx, cdvtx, hvtx, uvtx,
+// vvtx, zetavtx.  Cell values given at vertices are averaged by vtx2ctr
+// (vtx2ctr.f90:32-35); vertex values given at cells are interpolated by
+// ctr2vtx type 3 (ctr2vtx.f90:73-80 with c2v_setup.f90:106-119).
+void dataprep_fields(const Prep& P, double* z, double* zvtx, double* zeta, double* cd,
+                     double* cdvtx, double* hvtx, double* uvtx, double* vvtx, double* zetavtx);
 
 // Least-squares molecule only (elemnt_db.f90:42-128), input numbering.
 int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int>& cells,

@@ -62,9 +62,27 @@ MODULE storm_fvt_c
       TYPE(C_PTR), VALUE :: ctx, qh, qu, qv
     END FUNCTION
 
-    INTEGER(C_INT) FUNCTION storm_fvt_get_vertex(ctx, hvtx, zetavtx) BIND(C, name="storm_fvt_get_vertex")
+    INTEGER(C_INT) FUNCTION storm_fvt_get_vertex(ctx, hvtx, uvtx, vvtx, zetavtx) &
+        BIND(C, name="storm_fvt_get_vertex")
       IMPORT :: C_INT, C_PTR
-      TYPE(C_PTR), VALUE :: ctx, hvtx, zetavtx
+      TYPE(C_PTR), VALUE :: ctx, hvtx, uvtx, vvtx, zetavtx
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION storm_fvt_get_prep_fields(ctx, z, zvtx, zeta, cd, cdvtx) &
+        BIND(C, name="storm_fvt_get_prep_fields")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, z, zvtx, zeta, cd, cdvtx
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION storm_fvt_get_output(ctx, cdvtx, e_est) BIND(C, name="storm_fvt_get_output")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, cdvtx, e_est
+    END FUNCTION
+
+    INTEGER(C_INT64_T) FUNCTION storm_fvt_get_c2v_weights(ctx, ptr, cells, w, wid) &
+        BIND(C, name="storm_fvt_get_c2v_weights")
+      IMPORT :: C_INT64_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, ptr, cells, w, wid
     END FUNCTION
 
     INTEGER(C_INT) FUNCTION storm_fvt_get_gradients(ctx, dhx, dhy) BIND(C, name="storm_fvt_get_gradients")

@@ -39,6 +39,9 @@ SUBROUTINE dataprepFVT
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: qts_t(:), qts_v(:), hts_t(:), hts_v(:)
   INTEGER :: i, j, k, n, nq, ierr
   INTEGER(C_INT) :: cstat
+  INTEGER(C_INT64_T) :: nw
+  INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: wptr(:), wcell(:)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: ww(:), wid(:), pc(:,:), pp(:,:)
 
   ! Options the GPU path implements (everything else stops the run).
   IF (opt_solver /= 2) CALL byebye('storm_fvt: only the FVT solver is provided.')
@@ -160,49 +163,48 @@ SUBROUTINE dataprepFVT
   cstat = storm_fvt_create(m, o, ctx)
   IF (cstat /= STORM_OK) CALL byebye('storm_fvt_create failed (mesh, options or device).')
 
-  ! Module copies of the cell and vertex data (dataprepFVT.f90:69-140), made
-  ! with the reference's own interpolation routines; the library forms the
-  ! same fields for itself from the arrays passed above.
-  CALL c2v_setup(nctr2vtx)
-  IF (datav(3)) THEN
-    zvtx = z_tcp(1:n_pts)
-  ELSE
-    CALL ctr2vtx(nctr2vtx, z_tcp, zvtx)
-  END IF
-  CALL vtx2ctr(zvtx, z)
-  IF (datav(4)) THEN
-    CALL vtx2ctr(h_tcp, h)
-    hvtx = h_tcp(1:n_pts)
-    zetavtx = hvtx + zvtx
-    CALL vtx2ctr(zetavtx, zeta)
-  ELSE
-    h(1:n_elems) = h_tcp(1:n_elems)
-    zeta = z + h(1:n_elems)
-    CALL ctr2vtx(nctr2vtx, zeta, zetavtx)
-    hvtx = zetavtx - zvtx
-  END IF
+  ! Module copies of the cell and vertex data (dataprepFVT.f90:69-140) and
+  ! the centre-to-vertex weights of c2v_setup (type 3), as the library formed
+  ! them (kept for later ctr2vtx calls of the output writers).
+  nw = storm_fvt_get_c2v_weights(ctx, C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, C_NULL_PTR)
+  IF (nw < 0) CALL byebye('storm_fvt: reading the set-up fields failed.')
+  ALLOCATE(wptr(n_pts+1), wcell(MAX(1_C_INT64_T, nw)), ww(MAX(1_C_INT64_T, nw)), &
+           wid(MAX(1_C_INT64_T, nw)), pc(n_elems,5), pp(n_pts,6), STAT=ierr)
+  IF (ierr /= 0) CALL alloc_err(ierr)
+  nw = storm_fvt_get_c2v_weights(ctx, C_LOC(wptr), C_LOC(wcell), C_LOC(ww), C_LOC(wid))
+  cstat = storm_fvt_get_prep_fields(ctx, C_LOC(pc(1,1)), C_LOC(pp(1,1)), C_LOC(pc(1,2)), &
+                                    C_LOC(pc(1,3)), C_LOC(pp(1,2)))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_state(ctx, C_LOC(pc(1,4)), C_LOC(pc(1,5)), &
+                                                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR)
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_vertex(ctx, C_LOC(pp(1,3)), C_LOC(pp(1,4)), &
+                                                      C_LOC(pp(1,5)), C_LOC(pp(1,6)))
+  IF (nw < 0 .OR. cstat /= STORM_OK) CALL byebye('storm_fvt: reading the set-up fields failed.')
+  z = pc(:,1)
+  zeta = pc(:,2)
+  cd = pc(:,3)
+  h(1:n_elems) = pc(:,4)
   h(n_elems+1) = zero
-  IF (datav(5)) THEN
-    CALL vtx2ctr(u_tcp, u)
-    uvtx = u_tcp(1:n_pts)
-  ELSE
-    u = u_tcp(1:n_elems)
-    CALL ctr2vtx(nctr2vtx, u_tcp, uvtx)
-  END IF
-  IF (datav(6)) THEN
-    CALL vtx2ctr(v_tcp, v)
-    vvtx = v_tcp(1:n_pts)
-  ELSE
-    v = v_tcp(1:n_elems)
-    CALL ctr2vtx(nctr2vtx, v_tcp, vvtx)
-  END IF
-  IF (datav(7)) THEN
-    CALL vtx2ctr(cd_tcp, cd)
-    cdvtx = cd_tcp(1:n_pts)
-  ELSE
-    cd = cd_tcp(1:n_elems)
-    CALL ctr2vtx(nctr2vtx, cd_tcp, cdvtx)
-  END IF
+  u = pc(:,5)
+  zvtx = pp(:,1)
+  cdvtx = pp(:,2)
+  hvtx = pp(:,3)
+  uvtx = pp(:,4)
+  vvtx = pp(:,5)
+  zetavtx = pp(:,6)
+  cstat = storm_fvt_get_state(ctx, C_NULL_PTR, C_NULL_PTR, C_LOC(pc(1,5)), C_NULL_PTR, C_NULL_PTR)
+  IF (cstat /= STORM_OK) CALL byebye('storm_fvt: reading the set-up fields failed.')
+  v = pc(:,5)
+  k = MAX(2, SIZE(n2t2, 2))
+  ALLOCATE(c2v_weights(n_pts, k-1), c2v_weights_id(n_pts, k-1), STAT=ierr)
+  IF (ierr /= 0) CALL alloc_err(ierr)
+  c2v_weights = zero
+  c2v_weights_id = zero
+  DO i = 1, n_pts
+    DO j = 1, wptr(i+1) - wptr(i)
+      c2v_weights(i, j) = ww(wptr(i) + j)
+      c2v_weights_id(i, j) = wid(wptr(i) + j)
+    END DO
+  END DO
   rough = cd
   DEALLOCATE(z_tcp, h_tcp, u_tcp, v_tcp, cd_tcp)
   datav = .FALSE.
@@ -229,7 +231,8 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   INTEGER(C_INT32_T) :: it
   REAL(C_DOUBLE) :: tt, rm
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: hh(:), uu(:), vv(:), ph(:), qh(:), qu(:), qv(:), &
-                                         hv(:), zv(:), gx(:), gy(:), qb(:), hb(:)
+                                         hv(:), zv(:), uv(:), vv2(:), cv(:), ee(:), gx(:), &
+                                         gy(:), qb(:), hb(:)
   INTEGER(C_INT8_T), ALLOCATABLE, TARGET :: dr(:)
   INTEGER(C_INT) :: cstat
   INTEGER :: i, k
@@ -246,10 +249,16 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   rmax = rm
 
   ALLOCATE(hh(n_elems), uu(n_elems), vv(n_elems), ph(3*n_elems), dr(n_elems), &
-           qh(n_elems), qu(n_elems), qv(n_elems), hv(n_pts), zv(n_pts))
+           qh(n_elems), qu(n_elems), qv(n_elems), hv(n_pts), zv(n_pts), uv(n_pts), &
+           vv2(n_pts), cv(n_pts), ee(n_elems), gx(n_elems), gy(n_elems), &
+           qb(MAX(1, n_inflowbdr)), hb(MAX(1, n_outflowbdr)))
   cstat = storm_fvt_get_state(ctx, C_LOC(hh), C_LOC(uu), C_LOC(vv), C_LOC(ph), C_LOC(dr))
   IF (cstat == STORM_OK) cstat = storm_fvt_get_conserved(ctx, C_LOC(qh), C_LOC(qu), C_LOC(qv))
-  IF (cstat == STORM_OK) cstat = storm_fvt_get_vertex(ctx, C_LOC(hv), C_LOC(zv))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_vertex(ctx, C_LOC(hv), C_LOC(uv), C_LOC(vv2), &
+                                                      C_LOC(zv))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_output(ctx, C_LOC(cv), C_LOC(ee))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_gradients(ctx, C_LOC(gx), C_LOC(gy))
+  IF (cstat == STORM_OK) cstat = storm_fvt_get_bc_values(ctx, C_LOC(qb), C_LOC(hb))
   IF (cstat /= STORM_OK) CALL byebye('solverFVT: reading the GPU state failed.')
   h(1:n_elems) = hh
   u = uu
@@ -259,42 +268,26 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
     phi(i,2) = ph(n_elems + i)
     phi(i,3) = ph(2*n_elems + i)
     drycell(i) = dr(i) /= 0
+    delh(i)%x = gx(i)
+    delh(i)%y = gy(i)
   END DO
   rkh(:,rkorder) = qh
   rku(:,rkorder) = qu
   rkv(:,rkorder) = qv
   hvtx = hv
+  uvtx = uv
+  vvtx = vv2
   zetavtx = zv
+  cdvtx = cv
+  e_est = ee
   zeta = z + h(1:n_elems)
-
-  ! The rest of the post-loop work (solverFVT.f90:483-487) on the host, with
-  ! the reference's own routines: the vertex velocities of to_vertices.f90:56-65
-  ! under the boundary values of the last step, the vertex drag coefficient,
-  ! and the error estimates from the last stage's depth gradient.
-  ALLOCATE(gx(n_elems), gy(n_elems), qb(MAX(1, n_inflowbdr)), hb(MAX(1, n_outflowbdr)))
-  cstat = storm_fvt_get_bc_values(ctx, C_LOC(qb), C_LOC(hb))
-  IF (cstat == STORM_OK) cstat = storm_fvt_get_gradients(ctx, C_LOC(gx), C_LOC(gy))
-  IF (cstat /= STORM_OK) CALL byebye('solverFVT: reading the GPU state failed.')
+  ! the boundary values of the last step (get_time_bcs, solverFVT.f90:462)
   DO k = 1, n_inflowbdr
     qin(k) = qb(k)
   END DO
   DO k = 1, n_outflowbdr
     IF (htype(k) == 1) hbc(k) = hb(k)
   END DO
-  CALL ctr2vtx(nctr2vtx, u, uvtx)
-  CALL ctr2vtx(nctr2vtx, v, vvtx)
-  IF (btype == 0) CALL wetdryvfix
-  WHERE (hvtx < h_dry)
-    uvtx = zero
-    vvtx = zero
-  END WHERE
-  CALL enforce_bcs(hvtx, uvtx, vvtx, zvtx, n_pts)
-  CALL ctr2vtx(nctr2vtx, cd, cdvtx)
-  DO i = 1, n_elems
-    delh(i)%x = gx(i)
-    delh(i)%y = gy(i)
-  END DO
-  CALL error_est(2)
   tecplot_print = .TRUE.
   IF (LEN(fn) < 0 .OR. LEN(title) < 0) RETURN  ! names kept for the call surface
 END SUBROUTINE solverFVT

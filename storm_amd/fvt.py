@@ -22,6 +22,8 @@ from typing import Dict, Optional
 
 import numpy as np
 
+from . import cases
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libstorm_fvt.so")
 
@@ -63,8 +65,13 @@ EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch
            "storm_fvt_reach_status", "storm_fvt_set_state",
            "storm_fvt_set_bc_values", "storm_fvt_run", "storm_fvt_finalize",
            "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
-           "storm_fvt_get_gradients", "storm_fvt_get_bc_values", "storm_fvt_device_info", "storm_fvt_last_run_seconds", "storm_fvt_last_error",
-           "storm_fvt_destroy", "storm_fvt_inspect_molecule"]
+           "storm_fvt_get_prep_fields", "storm_fvt_get_output", "storm_fvt_get_c2v_weights",
+           "storm_fvt_get_gradients", "storm_fvt_get_bc_values", "storm_fvt_device_info",
+           "storm_fvt_last_run_seconds", "storm_fvt_last_error", "storm_fvt_destroy",
+           "storm_fvt_inspect_molecule"]
+
+STORM_E_STATE = -5
+VERTEX_KEYS = ("hvtx", "uvtx", "vvtx", "zetavtx", "cdvtx", "zvtx")
 
 _lib = None
 
@@ -90,7 +97,11 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_finalize.argtypes = [C.c_void_p]
     L.storm_fvt_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8)]
     L.storm_fvt_get_conserved.argtypes = [C.c_void_p, _dp, _dp, _dp]
-    L.storm_fvt_get_vertex.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_get_vertex.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.storm_fvt_get_prep_fields.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.storm_fvt_get_output.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_get_c2v_weights.restype = C.c_int64
+    L.storm_fvt_get_c2v_weights.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
     L.storm_fvt_get_gradients.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_get_bc_values.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
@@ -125,8 +136,9 @@ def mesh_struct(case, keep: dict) -> Mesh:
     k["tri"] = np.ascontiguousarray(case.tri, dtype=np.int32)
     k["ep"] = np.ascontiguousarray(case.edge_p, dtype=np.int32)
     k["ee"] = np.ascontiguousarray(case.edge_e, dtype=np.int32)
-    for nm in ("zvtx", "h", "u", "v", "cd"):
-        k[nm] = np.ascontiguousarray(getattr(case, nm), dtype=np.float64)
+    data = cases.inputs(case)
+    for nm in ("z", "h", "u", "v", "cd"):
+        k[nm] = np.ascontiguousarray(data[nm][0], dtype=np.float64)
     ins, outs = case.inflows, case.outflows
     z1 = np.zeros(1)
     k["in_type"] = np.array([b.kind for b in ins] or [0], dtype=np.int32)
@@ -145,8 +157,10 @@ def mesh_struct(case, keep: dict) -> Mesh:
     return Mesh(n_pts=case.n_pts, n_elems=case.n_elems, n_edges=case.n_edges,
                 x=_p(k["x"], d), y=_p(k["y"], d), tri=_p(k["tri"], i),
                 edge_p=_p(k["ep"], i), edge_e=_p(k["ee"], i),
-                z_at_vtx=1, h_at_vtx=0, u_at_vtx=0, v_at_vtx=0, cd_at_vtx=0,
-                z=_p(k["zvtx"], d), h=_p(k["h"], d), u=_p(k["u"], d), v=_p(k["v"], d),
+                z_at_vtx=int(data["z"][1]), h_at_vtx=int(data["h"][1]),
+                u_at_vtx=int(data["u"][1]), v_at_vtx=int(data["v"][1]),
+                cd_at_vtx=int(data["cd"][1]),
+                z=_p(k["z"], d), h=_p(k["h"], d), u=_p(k["u"], d), v=_p(k["v"], d),
                 cd=_p(k["cd"], d), n_in=len(ins), in_type=_p(k["in_type"], i),
                 in_ptr=_p(k["in_ptr"], i), in_nodes=_p(k["in_nodes"], i),
                 qts_ptr=_p(k["qts_ptr"], i), qts_t=_p(k["qts_t"], d), qts_v=_p(k["qts_v"], d),
@@ -257,10 +271,56 @@ class FVTSolver:
         qh, qu, qv = np.zeros(N), np.zeros(N), np.zeros(N)
         self._chk(load().storm_fvt_get_conserved(self.h, _p(qh, C.c_double), _p(qu, C.c_double),
                                                  _p(qv, C.c_double)))
-        hv, zv = np.zeros(self.P), np.zeros(self.P)
-        self._chk(load().storm_fvt_get_vertex(self.h, _p(hv, C.c_double), _p(zv, C.c_double)))
-        return dict(h=h, u=u, v=v, qh=qh, qu=qu, qv=qv, phi1=phi[:N], phi2=phi[N:2 * N],
-                    phi3=phi[2 * N:], dry=dry.astype(np.int32), hvtx=hv, zetavtx=zv)
+        out = dict(h=h, u=u, v=v, qh=qh, qu=qu, qv=qv, phi1=phi[:N], phi2=phi[N:2 * N],
+                   phi3=phi[2 * N:], dry=dry.astype(np.int32))
+        out.update(self.vertex(missing_ok=True))
+        return out
+
+    def vertex(self, missing_ok: bool = False) -> Dict[str, np.ndarray]:
+        """hvtx, uvtx, vvtx, zetavtx of the last to_vertices (dataprepFVT's
+        before any step, the final ones after finalize).  After steps with no
+        finalize only hvtx is current: with missing_ok the others are left
+        out (else StormError)."""
+        P = self.P
+        a = {k: np.zeros(P) for k in ("hvtx", "uvtx", "vvtx", "zetavtx")}
+        L = load()
+        rc = L.storm_fvt_get_vertex(self.h, *[_p(a[k], C.c_double) for k in a])
+        if rc == 0:
+            return a
+        if rc == STORM_E_STATE and missing_ok:
+            rc = L.storm_fvt_get_vertex(self.h, _p(a["hvtx"], C.c_double), None, None, None)
+            return {"hvtx": a["hvtx"]} if rc == 0 else {}
+        self._chk(rc)
+
+    def prep_fields(self) -> Dict[str, np.ndarray]:
+        """The module copies dataprepFVT leaves: z, zeta, cd (cells), zvtx,
+        cdvtx (vertices); the vertex depth, velocities and surface come from
+        vertex() before the first step."""
+        N, P = self.N, self.P
+        a = dict(z=np.zeros(N), zvtx=np.zeros(P), zeta=np.zeros(N), cd=np.zeros(N),
+                 cdvtx=np.zeros(P))
+        self._chk(load().storm_fvt_get_prep_fields(self.h, *[_p(a[k], C.c_double) for k in
+                                                             ("z", "zvtx", "zeta", "cd", "cdvtx")]))
+        return a
+
+    def output(self) -> Dict[str, np.ndarray]:
+        """After finalize: cdvtx = ctr2vtx(cd) and e_est (error_est(2))."""
+        cd, ee = np.zeros(self.P), np.zeros(self.N)
+        self._chk(load().storm_fvt_get_output(self.h, _p(cd, C.c_double), _p(ee, C.c_double)))
+        return dict(cdvtx=cd, e_est=ee)
+
+    def c2v_weights(self):
+        """(ptr, cells, w, wid): the centre-to-vertex weights as CSR."""
+        L = load()
+        n = L.storm_fvt_get_c2v_weights(self.h, None, None, None, None)
+        if n < 0:
+            self._chk(int(n))
+        ptr = np.zeros(self.P + 1, dtype=np.int32)
+        cells = np.zeros(max(n, 1), dtype=np.int32)
+        w, wid = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        L.storm_fvt_get_c2v_weights(self.h, _p(ptr, C.c_int32), _p(cells, C.c_int32),
+                                    _p(w, C.c_double), _p(wid, C.c_double))
+        return ptr, cells[:n], w[:n], wid[:n]
 
     def gradients(self):
         """delh of the last stage (x, y), what error_est reads after the loop."""
@@ -340,7 +400,7 @@ class BatchSolver(FVTSolver):
         for r in range(self.nr):
             a, b = self.cell_ptr[r], self.cell_ptr[r + 1]
             pa, pb = self.pts_ptr[r], self.pts_ptr[r + 1]
-            out.append({k: (v[pa:pb] if k in ("hvtx", "zetavtx") else v[a:b]) for k, v in st.items()})
+            out.append({k: (v[pa:pb] if k in VERTEX_KEYS else v[a:b]) for k, v in st.items()})
         return out
 
 

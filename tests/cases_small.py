@@ -1,6 +1,25 @@
 """Small seeded cases shared by the parity tests (the oracle runs them in
 well under a second each)."""
+import numpy as np
+
 from storm_amd import cases
+
+
+def vtx_data_case():
+    """A flume whose initial data are given where the reference's other
+    branch of dataprepFVT.f90:73-140 reads them: the bed at the cells, depth,
+    velocities and Manning n at the vertices; free outflow (htype 0)."""
+    c = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=51, dt_op=0, cfl=0.05,
+                    istop=30, q_in=12.0, out_kind=0)
+    rng = np.random.default_rng(5151)
+    zc = c.zvtx[c.tri].mean(axis=1) + rng.normal(0.0, 0.01, c.n_elems)
+    c.alt = {"z": zc,
+             "h": np.maximum(1.2 - c.zvtx, 0.0) + rng.uniform(0.0, 0.05, c.n_pts),
+             "u": 0.2 + rng.normal(0.0, 0.02, c.n_pts),
+             "v": rng.normal(0.0, 0.02, c.n_pts),
+             "cd": rng.uniform(0.025, 0.04, c.n_pts)}
+    c.meta["alt"] = "z at cells (flume zvtx averaged + N(0,0.01^2)); h, u, v, n at vertices; seed 5151"
+    return c
 
 
 def small_cases():
@@ -50,6 +69,14 @@ def small_cases():
         "flume_steady_converged": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=41,
                                                dt_op=0, cfl=0.3, istop=600, q_in=15.0,
                                                steady=True, stop_cr=1, rstop=1e-4), 600),
+        # initial data at the other locations dataprepFVT reads (vtx2ctr,
+        # ctr2vtx of the bed), free outflow
+        "flume_vtx_data": (vtx_data_case(), 30),
+        # a velocity inflow string (vtype 0: no flux routine, enforce_bcs sets
+        # the vertex velocity), slip walls, free outflow
+        "flume_vtype0_slip_free": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=52,
+                                               dt_op=0, cfl=0.05, istop=30, in_kind=0,
+                                               out_kind=0, q_in=0.4, btype=1), 30),
     }
 
 

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"\b(storm_(?:fvt|mpnn)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(storm_(?:fvt|mpnn)_[a-z0-9_]+)\s*\(", text)))
 
 
 @pytest.mark.parametrize("header,lib,mirror", [

@@ -1,0 +1,190 @@
+"""The library's own post-loop and set-up fields against the reference.
+
+* solverFVT's final to_vertices (vertex velocities under enforce_bcs),
+  ctr2vtx of cd and error_est(2) (solverFVT.f90:483-487), after finalize,
+  against the golden outputs of the flang-built reference and the oracle;
+* the module copies dataprepFVT leaves (dataprepFVT.f90:69-140), before any
+  step, against the reference's own dump of them;
+* the centre-to-vertex weights (c2v_setup.f90:98-119, type 3);
+* when each vertex field is current (get_vertex / get_output state rules);
+* boundary values changed in the middle of a run, against the oracle making
+  the same change;
+* a run split into chunks equals the same run in one call, bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cases_small import small_cases
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+# The state matches the reference to far below this (bit for bit on every
+# case so far); the derived fields take the same bound as phi and hvtx.
+RTOL = 1e-9
+
+
+def _cmp(a, b):
+    scale = max(np.abs(a).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+@pytest.mark.parametrize("name", list(small_cases().keys()))
+def test_output_fields_match_reference(name):
+    import oracle as O
+    from storm_amd import fvt
+    case, n = small_cases()[name]
+    print(f"[gpu output] {name} N={case.n_elems} steps={n}", flush=True)
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    s = fvt.FVTSolver(case)
+    s.run(n)
+    s.finalize()
+    vt = s.vertex()
+    out = s.output()
+    gx, gy = s.gradients()
+    s.close()
+    orc = O.Oracle(case)
+    orc.run(n)
+    orc.finalize()
+    oo = orc.output()
+    for k in ("uvtx", "vvtx"):
+        assert _cmp(g[k], vt[k]) <= RTOL, (k, _cmp(g[k], vt[k]))
+        assert _cmp(oo[k], vt[k]) <= RTOL, (k, _cmp(oo[k], vt[k]))
+    for k in ("hvtx", "zetavtx"):
+        assert _cmp(g[k], vt[k]) <= RTOL, (k, _cmp(g[k], vt[k]))
+    for k in ("cdvtx", "e_est"):
+        assert _cmp(g[k], out[k]) <= RTOL, (k, _cmp(g[k], out[k]))
+        assert _cmp(oo[k], out[k]) <= RTOL, (k, _cmp(oo[k], out[k]))
+    # zero exactly where the reference has zero (dry vertices, walls, string
+    # ends, cells below h_dry)
+    for k, a in (("uvtx", vt["uvtx"]), ("vvtx", vt["vvtx"]), ("e_est", out["e_est"])):
+        assert np.array_equal(g[k] == 0.0, a == 0.0), k
+    assert _cmp(g["delhx"], gx) <= RTOL and _cmp(g["delhy"], gy) <= RTOL
+
+
+@pytest.mark.parametrize("name", list(small_cases().keys()))
+def test_dataprep_copies_match_reference(name):
+    """Before any step: z, zeta, cd, h, u, v at the cells and zvtx, cdvtx,
+    hvtx, uvtx, vvtx, zetavtx at the vertices, bit for bit."""
+    from storm_amd import cases, fvt
+    case, _ = small_cases()[name]
+    print(f"[gpu dataprep] {name}", flush=True)
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    s = fvt.FVTSolver(case)
+    got = dict(s.prep_fields())
+    got.update(s.vertex())
+    st = s.state()
+    s.close()
+    for k in ("h", "u", "v"):
+        got[k] = st[k]
+    for k, _ in cases.PREP_KEYS:
+        assert np.array_equal(g["prep_" + k], got[k]), (k, np.abs(g["prep_" + k] - got[k]).max())
+
+
+def test_vertex_field_state_rules():
+    from storm_amd import fvt
+    case, n = small_cases()["flume_rk2"]
+    s = fvt.FVTSolver(case)
+    v0 = s.vertex()          # dataprepFVT's
+    with pytest.raises(fvt.StormError):
+        s.output()           # only after finalize
+    s.run(3)
+    assert set(s.vertex(missing_ok=True)) == {"hvtx"}
+    with pytest.raises(fvt.StormError):
+        s.vertex()
+    with pytest.raises(fvt.StormError):
+        s.output()
+    s.finalize()
+    v1 = s.vertex()
+    assert set(s.output()) == {"cdvtx", "e_est"}
+    assert not np.array_equal(v0["hvtx"], v1["hvtx"])
+    st = s.state()
+    s.set_state(st["h"], st["u"], st["v"])
+    assert s.vertex(missing_ok=True) == {}
+    s.close()
+
+
+def test_c2v_weights_are_the_reference_inverse_distances():
+    """c2v_setup.f90:106-119: w = (1/d) / sum(1/d) over node_db's n2t row
+    (cells in ascending order), d = centroid-to-vertex distance."""
+    from storm_amd import fvt
+    case, _ = small_cases()["meander_crit"]
+    s = fvt.FVTSolver(case)
+    ptr, cells, w, wid = s.c2v_weights()
+    s.close()
+    # the reference's centroid: ONE_THIRD * (x1 + x2 + x3) in that order
+    x, y = [float(a) for a in case.x], [float(a) for a in case.y]
+    xc = [(1.0 / 3.0) * ((x[a] + x[b]) + x[c]) for a, b, c in case.tri]
+    yc = [(1.0 / 3.0) * ((y[a] + y[b]) + y[c]) for a, b, c in case.tri]
+    for p in range(case.n_pts):
+        row = sorted(int(k) for k in np.nonzero((case.tri == p).any(axis=1))[0])
+        assert list(cells[ptr[p]:ptr[p + 1]]) == row
+        dx = [xc[k] - x[p] for k in row]
+        dy = [yc[k] - y[p] for k in row]
+        inv = [1.0 / float(np.sqrt(a * a + b * b)) for a, b in zip(dx, dy)]
+        tdist = 0.0
+        for a in inv:
+            tdist = tdist + a
+        assert list(wid[ptr[p]:ptr[p + 1]]) == inv
+        assert list(w[ptr[p]:ptr[p + 1]]) == [a / tdist for a in inv]
+
+
+@pytest.mark.parametrize("steady", [False, True])
+def test_bc_values_changed_mid_run_match_oracle(steady):
+    """run(k), new discharge and stage, run(m): the run goes on from step k
+    with the new values, as the oracle does with the same change."""
+    import oracle as O
+    from storm_amd import cases, fvt
+    case = cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=61, dt_op=0, cfl=0.05,
+                       istop=60, q_in=12.0, steady=steady, stop_cr=1, rstop=1e-14)
+    print(f"[gpu bc change] steady={steady}", flush=True)
+    s = fvt.FVTSolver(case)
+    orc = O.Oracle(case)
+    s.run(20)
+    orc.run(20)
+    s.set_bc_values(qin=[18.0], hbc=[2.2])
+    orc.set_bc_values(qin=[18.0], hbc=[2.2])
+    it, tt, rm = s.run(25)
+    r = orc.run(25)
+    assert it == r["iter"] == 45
+    assert tt == pytest.approx(r["ttime"], rel=1e-13)
+    q, hb = s.bc_values()
+    assert q[0] == 18.0 and hb[0] == 2.2
+    s.finalize()
+    orc.finalize()
+    sg, so = s.state(), orc.state()
+    s.close()
+    for k in ("h", "qh", "qu", "qv", "u", "v"):
+        assert _cmp(so[k], sg[k]) <= 1e-10, (k, _cmp(so[k], sg[k]))
+    assert np.array_equal(so["dry"], sg["dry"])
+
+
+CHUNK_CASES = [("flume_rk2", 2, 0), ("flume_rk3_slip", 3, 0), ("flume_rk1_steady", 1, 1),
+               ("flume_rk2", 2, 1), ("flume_steady_converged", 2, 0)]
+
+
+@pytest.mark.parametrize("name,rk,graph", CHUNK_CASES)
+def test_chunked_runs_equal_one_run(name, rk, graph):
+    """run(a) + run(b) + run(c) equals run(a + b + c) bit for bit: state,
+    vertex fields, delh, ttime and rmax (each run ends with a bookkeeping
+    launch that the next run's first step must continue exactly)."""
+    from storm_amd import fvt
+    case, n = small_cases()[name]
+    print(f"[gpu chunks] {name} rk={rk} graph={graph}", flush=True)
+    total = min(n, 40)
+    res = []
+    for chunks in ([total], [7, 1, total - 8]):
+        s = fvt.FVTSolver(case, dict(rkorder=rk, graph=graph))
+        for c in chunks:
+            it, tt, rm = s.run(c)
+        gx, gy = s.gradients()
+        s.finalize()
+        st = s.state()
+        st.update(it=np.array(it), tt=np.array(tt), rm=np.array(rm), gx=gx, gy=gy)
+        s.close()
+        res.append(st)
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
