@@ -191,6 +191,16 @@ int64_t storm_fvt_get_c2v_weights(storm_fvt_t *ctx, int32_t *ptr, int32_t *cells
  * them from its input state (zero before the first step); either may be
  * NULL. */
 int storm_fvt_get_gradients(storm_fvt_t *ctx, double *dhx, double *dhy);
+/* storm_fvt_get_gradients without allocating: no reference counterpart;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid it).  Its staging buffer
+ * is a file-scope device array of STORM_FVT_NOALLOC_MAX_CELLS cells (32 MiB
+ * per device the library runs on, held until the process ends), shared by
+ * every call on a device: calls that overlap on one device race.  Contexts
+ * of more cells get STORM_E_ARG; a context with no run started
+ * STORM_E_STATE. */
+#define STORM_FVT_NOALLOC_MAX_CELLS 2097152
+int storm_fvt_get_gradients_noalloc(storm_fvt_t *ctx, double *dhx, double *dhy);
 /* The boundary values of the latest step: qin[n_in] (Q, or stage for vtype
  * 4) and hbc[n_out] (stage; meaningful for htype 1 strings).  NULL skips a
  * side. */

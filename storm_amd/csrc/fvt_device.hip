@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1553,6 +1554,22 @@ __global__ void k_delh(Geo g, Opts o, const CellD* din, const double* gwork, dou
   dh[N + i] = F.dhy;
 }
 
+// delh as k_delh forms it, written in input cell order (dh[perm[i]],
+// dh[N + perm[i]]) so that it can be copied to the caller as it is.
+__global__ void k_delh_input_order(Geo g, Opts o, const CellD* din, const double* gwork,
+                                   const int* perm, double* dh) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  const int N = g.N;
+  const Tri tr = load_tri(g, i);
+  Faces F;
+  cell_faces(g, o, din, gwork, i, g.cmeta[i], g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i], tr,
+             true, F);
+  const int j = perm[i];
+  dh[j] = F.dhx;
+  dh[N + j] = F.dhy;
+}
+
 // ---------------------------------------------------------------- output fields
 // Tables of the final to_vertices' velocity part (enforce_bcs.f90): per
 // vertex the outflow-string normals its velocity is checked against and its
@@ -1711,6 +1728,14 @@ __global__ void k_out_cells(Geo g, Opts o, const CellD* dh_in, const double* gwo
   e_est[i] = sqrt(gx * gx + gy * gy) * l / (g.cs[i].z + h);
 }
 
+// ---------------------------------------------------------------- scratch
+// The scratch buffer of storm_fvt_get_gradients_noalloc: delh of up to
+// NOALLOC_MAX_CELLS cells (x then y components), 32 MiB.  Each device the
+// library runs on holds its own copy until the process ends; every call on
+// a device shares that device's copy.
+constexpr int NOALLOC_MAX_CELLS = STORM_FVT_NOALLOC_MAX_CELLS;
+static __device__ __align__(256) double scratch[2 * (size_t)NOALLOC_MAX_CELLS];
+
 // ---------------------------------------------------------------- host side
 #define HIPCHK(x)                                                                       \
   do {                                                                                  \
@@ -1747,6 +1772,15 @@ T* dalloc(std::vector<void*>& owned, size_t n) {
 
 }  // namespace
 
+namespace {
+// The device address of `scratch` per device, taken once per device at
+// context creation with that device current (the host symbol is only a
+// stand-in for it).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[MAX_DEVICES] = {};
+}  // namespace
+
 struct storm_fvt {
   Prep prep;
   std::string err;
@@ -1776,6 +1810,8 @@ struct storm_fvt {
   // (a step has run since the last set_state)
   enum { V_PREP, V_FINAL, V_NONE } vsrc = V_PREP;
   bool vpass = false;
+  int* d_perm = nullptr;           // internal cell -> input cell
+  double* scratch_ptr = nullptr;   // this device's copy of `scratch`
   Tiles tl{};
   int cap = 0;
   std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
@@ -1964,6 +2000,19 @@ void setup_device(storm_fvt* C) {
   S.sq = dalloc<double>(own, P.n_in);
   S.sh = dalloc<double>(own, P.n_out);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
+  C->d_perm = dupload(own, P.perm);
+  {
+    if (C->device < 0 || C->device >= MAX_DEVICES)
+      throw StormErr(STORM_E_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[C->device]) {
+      void* ptr = nullptr;
+      HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch[C->device] = static_cast<double*>(ptr);
+    }
+    C->scratch_ptr = g_scratch[C->device];
+  }
+  C->err.reserve(256);  // error messages of the no-allocation entry point fit
   OutTab& T = C->tab;
   T.vo_ptr = dupload(own, P.vo_ptr);
   T.vwall = dupload(own, P.vwall);
@@ -2122,6 +2171,12 @@ void ensure_graph(storm_fvt* C) {
 
 int set_err(storm_fvt* C, int code, const std::string& msg) {
   if (C) C->err = msg;
+  return code;
+}
+// A literal message: assigned into the context's reserved string, so this
+// error path allocates nothing.
+int set_err(storm_fvt* C, int code, const char* msg) {
+  if (C) C->err.assign(msg);
   return code;
 }
 
@@ -2652,6 +2707,35 @@ int storm_fvt_get_gradients(storm_fvt_t* C, double* dhx, double* dhy) {
   } catch (const StormErr& e) {
     return set_err(C, e.code, e.msg);
   }
+  return STORM_OK;
+}
+
+int storm_fvt_get_gradients_noalloc(storm_fvt_t* C, double* dhx, double* dhy) {
+  if (!C) return STORM_E_ARG;
+  const Prep& P = C->prep;
+  const size_t N = P.N;
+  if (P.N > NOALLOC_MAX_CELLS)
+    return set_err(C, STORM_E_ARG, "get_gradients_noalloc: more cells than its scratch holds");
+  if (!C->started) return set_err(C, STORM_E_STATE, "get_gradients_noalloc: no run started");
+  if (C->iter_host == 0) {  // no stage has run: cell_grads has not either
+    if (dhx) std::memset(dhx, 0, sizeof(double) * N);
+    if (dhy) std::memset(dhy, 0, sizeof(double) * N);
+    return STORM_OK;
+  }
+  const int bs = 256;
+  const dim3 gc((P.N + bs - 1) / bs);
+  if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
+    hipLaunchKernelGGL(k_gauss1, gc, dim3(bs), 0, C->stream, C->g, C->last_din, C->S.gwork);
+  hipLaunchKernelGGL(k_delh_input_order, gc, dim3(bs), 0, C->stream, C->g, C->o, C->last_din,
+                     C->S.gwork, (const int*)C->d_perm, C->scratch_ptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && dhx)
+    e = hipMemcpyAsync(dhx, C->scratch_ptr, sizeof(double) * N, hipMemcpyDeviceToHost, C->stream);
+  if (e == hipSuccess && dhy)
+    e = hipMemcpyAsync(dhy, C->scratch_ptr + N, sizeof(double) * N, hipMemcpyDeviceToHost,
+                       C->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+  if (e != hipSuccess) return set_err(C, STORM_E_DEVICE, "get_gradients_noalloc: HIP error");
   return STORM_OK;
 }
 

@@ -66,7 +66,8 @@ EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch
            "storm_fvt_set_bc_values", "storm_fvt_run", "storm_fvt_finalize",
            "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
            "storm_fvt_get_prep_fields", "storm_fvt_get_output", "storm_fvt_get_c2v_weights",
-           "storm_fvt_get_gradients", "storm_fvt_get_bc_values", "storm_fvt_device_info",
+           "storm_fvt_get_gradients", "storm_fvt_get_gradients_noalloc",
+           "storm_fvt_get_bc_values", "storm_fvt_device_info",
            "storm_fvt_last_run_seconds", "storm_fvt_last_error", "storm_fvt_destroy",
            "storm_fvt_inspect_molecule"]
 
@@ -103,6 +104,7 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_get_c2v_weights.restype = C.c_int64
     L.storm_fvt_get_c2v_weights.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
     L.storm_fvt_get_gradients.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_get_gradients_noalloc.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_get_bc_values.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
     L.storm_fvt_last_run_seconds.restype = C.c_double
