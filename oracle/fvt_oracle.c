@@ -9,16 +9,23 @@
  * links it.
  *
  * Parity status per function (what pins it):
- *  - set-up, find_dry_cells, lsq_grad_array, limiterBJ, hgradfix2,
- *    to_vertices, invisc_fluxes+flux_ac, wenslp3/weslp2, inflowQbyH,
- *    inflowH, outflowbyH, crit_outflow, free_flow, bed_slope_STD/adjustZb,
- *    RK update and wet/dry recovery, fric_terms_impl, residual, hvalve:
- *    bit-for-bit against the flang-built reference on the committed cases.
- *  - flux_hll / flux_bgnvc (corrected forms of flux_HLL.f90 / flux_bgnvc.f90,
- *    whose reference bodies are defective), flux_rusanov, gauss gradients
- *    (corrected gaussgrad_setup.f90), fric_terms (explicit), the corrected CFL
- *    time step: pinned by closed forms in the tests (consistency, upwind
- *    limits, L*R = I, exactness on linear fields, brute-force minimum).
+ *  - set-up, dataprepFVT's module copies, find_dry_cells, lsq_grad_array,
+ *    limiterBJ, hgradfix2, to_vertices (vertex velocities and enforce_bcs
+ *    included), invisc_fluxes+flux_ac, wenslp3/weslp2, inflowQbyH, inflowH,
+ *    outflowbyH, crit_outflow, free_flow, bed_slope_STD/adjustZb, RK update
+ *    and wet/dry recovery, fric_terms_impl, residual, hvalve, ctr2vtx of cd,
+ *    error_est(2): bit-for-bit against the flang-built reference on the
+ *    committed whole-run cases.
+ *  - flux_rusanov, fric_terms (explicit), fvsources, part_dry, drynwet:
+ *    bit-for-bit against the reference routines themselves, run by the
+ *    per-routine harness ref_units.f90 on seeded inputs.
+ *  - flux_hll (textbook form; flux_HLL.f90's star region and both-wet right
+ *    speed are defective): its dry-bed speeds and upwind fluxes bit-for-bit
+ *    against the reference routine; the star region by closed form.
+ *  - flux_bgnvc (corrected eigenvectors), gauss gradients (corrected
+ *    gaussgrad_setup.f90; ghost cells by the mirror identity), the corrected
+ *    CFL time step: closed forms (consistency, Roe identities, exactness on
+ *    linear fields, brute-force minimum).
  */
 #include "fvt_oracle.h"
 
@@ -1587,13 +1594,125 @@ static void free_flow(orc_t *s) {
 }
 
 /* fric_terms.f90:24-98 (explicit Manning friction; a stage source) */
-static void fric_terms(orc_t *s) {
-  for (int i = 0; i < s->N; ++i) {
-    double r = s->dry[i] ? ZERO : G * s->cd[i] * s->cd[i] / fric_pow(s->h[i]);
-    r = -r * s->u_mag[i];
-    s->phi[s->N + i] = s->phi[s->N + i] + r * s->u[i] * s->area[i];
-    s->phi[2 * s->N + i] = s->phi[2 * s->N + i] + r * s->v[i] * s->area[i];
+/* fric_terms.f90:30-38,87-96 (Manning, opt_friction 0) on n cells. */
+static void fric_explicit(int n, const unsigned char *dry, const double *cd, const double *h,
+                          const double *umag, const double *u, const double *v,
+                          const double *area, double *phi2, double *phi3) {
+  for (int i = 0; i < n; ++i) {
+    double r = dry[i] ? ZERO : G * cd[i] * cd[i] / fric_pow(h[i]);
+    r = -r * umag[i];
+    phi2[i] = phi2[i] + r * u[i] * area[i];
+    phi3[i] = phi3[i] + r * v[i] * area[i];
   }
+}
+
+static void fric_terms(orc_t *s) {
+  fric_explicit(s->N, s->dry, s->cd, s->h, s->u_mag, s->u, s->v, s->area, s->phi + s->N,
+                s->phi + 2 * (size_t)s->N);
+}
+
+void orc_fric_explicit(int32_t n, const int32_t *dry, const double *cd, const double *h,
+                       const double *umag, const double *u, const double *v, const double *area,
+                       double *phi2, double *phi3) {
+  unsigned char *d = xcalloc(n, 1);
+  for (int i = 0; i < n; ++i) d[i] = dry[i] != 0;
+  fric_explicit(n, d, cd, h, umag, u, v, area, phi2, phi3);
+  free(d);
+}
+
+/* The node-based routines of the reference's RDS solver, on this mesh with a
+ * bed z and depth h given at the vertices:
+ *  - fvsources.f90:27-46: the node least-squares bed gradient
+ *    (lsq_gradient.f90:35-47 over n2n, weights 1/d^2 of lsq_setup.f90:32-80,
+ *    solver 1, type 2) times -h, integrated over the median dual
+ *    (integral_md.f90:33-56; its c1 and c2 are default-real literals, and c1
+ *    is 2.037..., not the 11/54 its comment gives);
+ *  - drynwet.f90:28-50 with part_dry.f90:30-43 per triangle (zadjust on):
+ *    the BGN bed of partially dry triangles.  With one dry vertex i the
+ *    reference takes vertices MAX(1,MOD(i+1,4)) and MAX(1,MOD(i+2,4)), which
+ *    for i = 3 are both vertex 1.
+ * source: [3P] component-major; zout: [3N]; pd: [4N] = part_dry's flag, n,
+ * i, j (1-based, 0 where unset). */
+void orc_rds(const orc_t *s, double h_dry, const double *z, const double *h, double *source,
+             double *zout, int32_t *pd) {
+  const int P = s->P, N = s->N, E = s->E;
+  /* node_db.f90:165-185: n2n from the edge table */
+  int *np = xcalloc(P + 1, sizeof(int)), *nn = xcalloc(2 * (size_t)E + 1, sizeof(int));
+  for (int k = 0; k < E; ++k) { np[s->ep[2 * k] + 1]++; np[s->ep[2 * k + 1] + 1]++; }
+  for (int i = 0; i < P; ++i) np[i + 1] += np[i];
+  {
+    int *fill = xcalloc(P, sizeof(int));
+    for (int k = 0; k < E; ++k) {
+      const int p1 = s->ep[2 * k], p2 = s->ep[2 * k + 1];
+      nn[np[p1] + fill[p1]++] = p2;
+      nn[np[p2] + fill[p2]++] = p1;
+    }
+    free(fill);
+  }
+  double *zbx = xcalloc(P, 8), *zby = xcalloc(P, 8);
+  for (int i = 0; i < P; ++i) {
+    double ai = ZERO, bi = ZERO, ci = ZERO, di = ZERO, ei = ZERO;
+    for (int t = np[i]; t < np[i + 1]; ++t) {
+      const int l = nn[t];
+      const double dx = s->x[l] - s->x[i], dy = s->y[l] - s->y[i];
+      const double w = ONE / (dx * dx + dy * dy);
+      ai = ai + w * dx * dx;
+      bi = bi + w * dx * dy;
+      ci = ci + w * dy * dy;
+    }
+    const double det = ai * ci - bi * bi;
+    for (int t = np[i]; t < np[i + 1]; ++t) {
+      const int l = nn[t];
+      const double dx = s->x[l] - s->x[i], dy = s->y[l] - s->y[i];
+      const double w = ONE / (dx * dx + dy * dy);
+      const double dphi = z[l] - z[i];
+      di = di + w * dphi * dx;
+      ei = ei + w * dphi * dy;
+    }
+    const double zx = (ci * di - bi * ei) / det, zy = (ai * ei - di * bi) / det;
+    zbx[i] = -zx * h[i];
+    zby[i] = -zy * h[i];
+  }
+  const double c1 = (double)2.03703703703704f, c2 = (double)6.481481481481481E-2f;
+  for (int i = 0; i < P; ++i) {
+    double ix = ZERO, iy = ZERO;
+    for (int t = s->n2t_ptr[i]; t < s->n2t_ptr[i + 1]; ++t) {
+      const int c = s->n2t[t];
+      int m = 0;
+      while (s->vtx[3 * c + m] != i) ++m;
+      const int j = s->vtx[3 * c + (m + 1) % 3], k = s->vtx[3 * c + (m + 2) % 3];
+      ix = ix + s->area[c] * (c1 * zbx[i] + c2 * (zbx[j] + zbx[k]));
+      iy = iy + s->area[c] * (c1 * zby[i] + c2 * (zby[j] + zby[k]));
+    }
+    source[i] = ZERO;
+    source[P + i] = G * ix;
+    source[2 * (size_t)P + i] = G * iy;
+  }
+  for (int e = 0; e < N; ++e) {
+    const int *v = &s->vtx[3 * e];
+    double *zo = &zout[3 * (size_t)e];
+    int32_t *q = &pd[4 * (size_t)e];
+    for (int m = 0; m < 3; ++m) zo[m] = z[v[m]];
+    int n = 0, fi = 0, fj = 0;
+    for (int m = 1; m <= 3; ++m)
+      if (h[v[m - 1]] < h_dry) { n = n + 1; fj = m; if (n == 1) fi = m; }
+    q[0] = (n != 0 && n != 3); q[1] = n; q[2] = fi; q[3] = fj;
+    if (!q[0]) continue;
+    if (n == 1) {
+      int jj = (fi + 1) % 4, kk = (fi + 2) % 4;
+      if (jj < 1) jj = 1;
+      if (kk < 1) kk = 1;
+      const double dj = z[v[jj - 1]] + h[v[jj - 1]], dk = z[v[kk - 1]] + h[v[kk - 1]];
+      const double dmax = fmax2(dj, dk);
+      zo[fi - 1] = fmin2(dmax, z[v[fi - 1]]);
+    } else {
+      const int kk = 6 - fi - fj;
+      const double dk = z[v[kk - 1]] + h[v[kk - 1]];
+      zo[fi - 1] = fmin2(dk, z[v[fi - 1]]);
+      zo[fj - 1] = fmin2(dk, z[v[fj - 1]]);
+    }
+  }
+  free(np); free(nn); free(zbx); free(zby);
 }
 
 /* bed_slope_STD.f90:23-70 with adjustZb.f90:41-70 */

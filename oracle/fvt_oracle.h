@@ -97,6 +97,15 @@ int32_t orc_equals(double a, double b);
  * grad: [6N] = (dzeta/dx, dzeta/dy, du/dx, du/dy, dv/dx, dv/dy) per cell. */
 void orc_cell_grads(orc_t *s, double *grad);
 double orc_dt_cfl(orc_t *s);
+/* Explicit Manning friction (fric_terms.f90:30-38,87-96) on n cells: phi2,
+ * phi3 are updated in place; the oracle's step runs the same code. */
+void orc_fric_explicit(int32_t n, const int32_t *dry, const double *cd, const double *h,
+                       const double *umag, const double *u, const double *v, const double *area,
+                       double *phi2, double *phi3);
+/* The RDS node routines on this mesh (fvsources; part_dry and drynwet per
+ * triangle) with bed z and depth h at the vertices; see fvt_oracle.c. */
+void orc_rds(const orc_t *s, double h_dry, const double *z, const double *h, double *source,
+             double *zout, int32_t *pd);
 int64_t orc_topology(const orc_t *s, int32_t *t2t_ptr, int32_t *t2t, int32_t *t2t3);
 
 #ifdef __cplusplus

@@ -79,6 +79,8 @@ def lib():
         L.orc_set_pow_perturbation.argtypes = [C.c_double, C.c_uint64]
         L.orc_get_output.argtypes = [C.c_void_p] + [_dp] * 6
         L.orc_set_bc_values.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_fric_explicit.argtypes = [C.c_int32, _ip] + [_dp] * 8
+        L.orc_rds.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _ip]
         L.orc_get_prep.argtypes = [C.c_void_p] + [_dp] * 9
         _lib = L
     return _lib
@@ -218,6 +220,18 @@ class Oracle:
         v = np.ascontiguousarray(v, dtype=np.float64)
         lib().orc_set_state(self.h, _p(h, C.c_double), _p(u, C.c_double), _p(v, C.c_double))
 
+    def rds(self, h_dry, z, h):
+        """fvsources' source (3, P), drynwet's zout (N, 3) and part_dry's
+        (flag, n, i, j) (N, 4) for bed z and depth h at the vertices."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        src = np.zeros(3 * self.P)
+        zo = np.zeros(3 * self.N)
+        pd = np.zeros(4 * self.N, dtype=np.int32)
+        lib().orc_rds(self.h, float(h_dry), _p(z, C.c_double), _p(h, C.c_double),
+                      _p(src, C.c_double), _p(zo, C.c_double), _p(pd, C.c_int32))
+        return src.reshape(3, self.P), zo.reshape(self.N, 3), pd.reshape(self.N, 4)
+
     def set_bc_values(self, qin=None, hbc=None):
         q = None if qin is None else np.ascontiguousarray(qin, dtype=np.float64)
         hb = None if hbc is None else np.ascontiguousarray(hbc, dtype=np.float64)
@@ -250,6 +264,18 @@ def flux(kind: str, nx, ny, hl, ul, vl, hr, ur, vr, cl=0.0, cr=0.0, h0=1e-3):
     f = np.zeros(3)
     lib().orc_flux(FLUX[kind], nx, ny, hl, ul, vl, hr, ur, vr, cl, cr, h0, _p(f, C.c_double))
     return f
+
+
+def fric_explicit(dry, cd, h, umag, u, v, area, phi2, phi3):
+    """fric_terms (Manning) on independent cells; returns updated phi2, phi3."""
+    n = len(h)
+    d = np.ascontiguousarray(dry, dtype=np.int32)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (cd, h, umag, u, v, area)]
+    p2 = np.array(phi2, dtype=np.float64)
+    p3 = np.array(phi3, dtype=np.float64)
+    lib().orc_fric_explicit(n, _p(d, C.c_int32), *[_p(a, C.c_double) for a in arrs],
+                            _p(p2, C.c_double), _p(p3, C.c_double))
+    return p2, p3
 
 
 def equals(a: float, b: float) -> bool:
