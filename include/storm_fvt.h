@@ -201,6 +201,24 @@ int storm_fvt_get_gradients(storm_fvt_t *ctx, double *dhx, double *dhy);
  * STORM_E_STATE. */
 #define STORM_FVT_NOALLOC_MAX_CELLS 2097152
 int storm_fvt_get_gradients_noalloc(storm_fvt_t *ctx, double *dhx, double *dhy);
+/* The node-based wet/dry and source routines of the reference's RDS solver
+ * (not on its FVT path), on the context's mesh for a bed z and depth h given
+ * at the vertices [n_pts] (input order; the context's state is not used):
+ *   storm_fvt_drynwet   drynwet.f90:28-50 with part_dry.f90:30-43 per
+ *                       triangle (zadjust on): zout [3*n_elems], the bed of
+ *                       each triangle's vertices (tri order), lowered at the
+ *                       dry vertices of a partially dry triangle; pd
+ *                       [4*n_elems] or NULL: part_dry's flag, n, i, j
+ *                       (1-based, 0 where unset).  h_dry is the context's.
+ *   storm_fvt_fvsources fvsources.f90:27-46: source [3*n_pts], component-
+ *                       major: 0, then g times the median-dual integral of
+ *                       -h times the node least-squares bed gradient (the
+ *                       n2n molecule of node_db.f90:165-185, weights of
+ *                       lsq_setup.f90:32-80 type 2; integral_md.f90's
+ *                       coefficients as written). */
+int storm_fvt_drynwet(storm_fvt_t *ctx, const double *z, const double *h, double *zout,
+                      int32_t *pd);
+int storm_fvt_fvsources(storm_fvt_t *ctx, const double *z, const double *h, double *source);
 /* The boundary values of the latest step: qin[n_in] (Q, or stage for vtype
  * 4) and hbc[n_out] (stage; meaningful for htype 1 strings).  NULL skips a
  * side. */

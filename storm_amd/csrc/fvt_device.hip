@@ -1728,6 +1728,114 @@ __global__ void k_out_cells(Geo g, Opts o, const CellD* dh_in, const double* gwo
   e_est[i] = sqrt(gx * gx + gy * gy) * l / (g.cs[i].z + h);
 }
 
+// ---------------------------------------------------------------- RDS node routines
+// The node-based routines of the reference's RDS solver on the context's
+// mesh, for a bed z and depth h given at the vertices (internal order).
+
+// part_dry.f90:30-43 and drynwet.f90:28-50 (zadjust on) for triangle i:
+// the bed of its vertices, lowered at the dry vertices of a partially dry
+// triangle to the water surface of its wet vertices (Brufau and
+// Garcia-Navarro 2003).  With one dry vertex, its two others are
+// MAX(1,MOD(i+1,4)) and MAX(1,MOD(i+2,4)) in the reference's 1-based
+// numbering, which for i = 3 are both vertex 1; kept as written.  Written
+// in input cell order: zout[3*perm[i] + m], pd[4*perm[i] + ...] = part_dry's
+// flag, n, i, j (1-based; 0 where unset).
+__global__ void k_drynwet(Geo g, double h_dry, const double* z, const double* h,
+                          const int* perm, double* zout, int* pd) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  const int N = g.N;
+  const int v[3] = {g.cvtx[i], g.cvtx[N + i], g.cvtx[2 * N + i]};
+  double zo[3] = {z[v[0]], z[v[1]], z[v[2]]};
+  int n = 0, fi = 0, fj = 0;
+#pragma unroll
+  for (int m = 1; m <= 3; ++m)
+    if (h[v[m - 1]] < h_dry) { n = n + 1; fj = m; if (n == 1) fi = m; }
+  const bool part = n != 0 && n != 3;
+  if (part && n == 1) {
+    const int jj = max(1, (fi + 1) % 4), kk = max(1, (fi + 2) % 4);
+    const double dj = z[v[jj - 1]] + h[v[jj - 1]], dk = z[v[kk - 1]] + h[v[kk - 1]];
+    const double dm = dmax(dj, dk);
+    zo[fi - 1] = dmin(dm, z[v[fi - 1]]);
+  } else if (part) {
+    const int kk = 6 - fi - fj;
+    const double dk = z[v[kk - 1]] + h[v[kk - 1]];
+    zo[fi - 1] = dmin(dk, z[v[fi - 1]]);
+    zo[fj - 1] = dmin(dk, z[v[fj - 1]]);
+  }
+  const int o = perm[i];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) zout[3 * (size_t)o + m] = zo[m];
+  if (pd) {
+    pd[4 * (size_t)o] = part;
+    pd[4 * (size_t)o + 1] = n;
+    pd[4 * (size_t)o + 2] = fi;
+    pd[4 * (size_t)o + 3] = fj;
+  }
+}
+
+// fvsources.f90:27-32 for vertex p: the least-squares bed gradient over the
+// vertices sharing an edge with p (lsq_gradient.f90:35-47, solver 1, with
+// the weights 1/d^2 and normal-equation sums of lsq_setup.f90:32-80, type 2,
+// as dataprepRDS.f90:93 sets them up), times -h.  zb: [2][P].
+__global__ void k_node_grad(Geo g, const int* nn_ptr, const int* nn_idx, const double* z,
+                            const double* h, double* zb) {
+  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  const double4 q = g.vxyz[p];
+  const int t0 = nn_ptr[p], t1 = nn_ptr[p + 1];
+  double ai = ZERO, bi = ZERO, ci = ZERO, di = ZERO, ei = ZERO;
+  for (int t = t0; t < t1; ++t) {
+    const double4 l = g.vxyz[nn_idx[t]];
+    const double dx = l.x - q.x, dy = l.y - q.y;
+    const double w = ONE / (dx * dx + dy * dy);
+    ai = ai + w * dx * dx;
+    bi = bi + w * dx * dy;
+    ci = ci + w * dy * dy;
+  }
+  const double det = ai * ci - bi * bi;
+  const double zp = z[p];
+  for (int t = t0; t < t1; ++t) {
+    const int k = nn_idx[t];
+    const double4 l = g.vxyz[k];
+    const double dx = l.x - q.x, dy = l.y - q.y;
+    const double w = ONE / (dx * dx + dy * dy);
+    const double dphi = z[k] - zp;
+    di = di + w * dphi * dx;
+    ei = ei + w * dphi * dy;
+  }
+  const double zx = (ci * di - bi * ei) / det, zy = (ai * ei - di * bi) / det;
+  zb[p] = -zx * h[p];
+  zb[g.P + p] = -zy * h[p];
+}
+
+// fvsources.f90:34-44 for vertex p: source(:,2:3) = g times the median-dual
+// integral of zb (integral_md.f90:33-56: per triangle of n2t, in its order,
+// area * (c1 zb(p) + c2 (zb(j) + zb(k))), j and k the triangle's next
+// vertices after p).  c1 and c2 are the routine's default-real literals;
+// c1 = 2.037..., not the 11/54 of its comment.  Written in input vertex
+// order, component-major: src[c*P + vperm[p]].
+__global__ void k_node_integral(Geo g, const double* zb, const int* vperm, double* src) {
+  const int p = xcd_tile() * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  const int N = g.N, P = g.P;
+  const double c1 = (double)2.03703703703704f, c2 = (double)6.481481481481481E-2f;
+  double ix = ZERO, iy = ZERO;
+  for (int t = g.vptr[p]; t < g.vptr[p + 1]; ++t) {
+    const int c = g.vcell[t];
+    const int a0 = g.cvtx[c], a1 = g.cvtx[N + c], a2 = g.cvtx[2 * N + c];
+    const int j = (a0 == p) ? a1 : (a1 == p) ? a2 : a0;
+    const int k = (a0 == p) ? a2 : (a1 == p) ? a0 : a1;
+    const double area = tri_area(load_tri(g, c));
+    ix = ix + area * (c1 * zb[p] + c2 * (zb[j] + zb[k]));
+    iy = iy + area * (c1 * zb[P + p] + c2 * (zb[P + j] + zb[P + k]));
+  }
+  const int o = vperm[p];
+  src[o] = ZERO;
+  src[P + o] = G * ix;
+  src[2 * (size_t)P + o] = G * iy;
+}
+
 // ---------------------------------------------------------------- scratch
 // The scratch buffer of storm_fvt_get_gradients_noalloc: delh of up to
 // NOALLOC_MAX_CELLS cells (x then y components), 32 MiB.  Each device the
@@ -1811,6 +1919,8 @@ struct storm_fvt {
   enum { V_PREP, V_FINAL, V_NONE } vsrc = V_PREP;
   bool vpass = false;
   int* d_perm = nullptr;           // internal cell -> input cell
+  // the node routines' tables, uploaded at their first call
+  int *d_vperm = nullptr, *d_nn_ptr = nullptr, *d_nn_idx = nullptr;
   double* scratch_ptr = nullptr;   // this device's copy of `scratch`
   Tiles tl{};
   int cap = 0;
@@ -2178,6 +2288,29 @@ int set_err(storm_fvt* C, int code, const std::string& msg) {
 int set_err(storm_fvt* C, int code, const char* msg) {
   if (C) C->err.assign(msg);
   return code;
+}
+
+// Device buffers of one call, freed on every way out.
+struct CallBufs {
+  std::vector<void*> p;
+  ~CallBufs() { for (void* q : p) (void)hipFree(q); }
+  template <class T>
+  T* get(size_t n) {
+    T* d = nullptr;
+    HIPCHK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+    p.push_back(d);
+    return d;
+  }
+};
+
+// Vertex values in input order -> a device array in internal order.
+double* vertex_upload(storm_fvt* C, CallBufs& B, const double* a) {
+  const Prep& P = C->prep;
+  std::vector<double> t(P.P);
+  for (int pp = 0; pp < P.P; ++pp) t[pp] = a[P.vperm[pp]];
+  double* d = B.get<double>(P.P);
+  HIPCHK(hipMemcpy(d, t.data(), sizeof(double) * P.P, hipMemcpyHostToDevice));
+  return d;
 }
 
 }  // namespace
@@ -2736,6 +2869,62 @@ int storm_fvt_get_gradients_noalloc(storm_fvt_t* C, double* dhx, double* dhy) {
                        C->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
   if (e != hipSuccess) return set_err(C, STORM_E_DEVICE, "get_gradients_noalloc: HIP error");
+  return STORM_OK;
+}
+
+int storm_fvt_drynwet(storm_fvt_t* C, const double* z, const double* h, double* zout,
+                      int32_t* pd) {
+  if (!C || !z || !h || !zout) return set_err(C, STORM_E_ARG, "null argument");
+  try {
+    const Prep& P = C->prep;
+    CallBufs B;
+    const double* dz = vertex_upload(C, B, z);
+    const double* dh = vertex_upload(C, B, h);
+    double* dzo = B.get<double>(3 * (size_t)P.N);
+    int* dpd = pd ? B.get<int>(4 * (size_t)P.N) : nullptr;
+    const int bs = 256;
+    hipLaunchKernelGGL(k_drynwet, dim3((P.N + bs - 1) / bs), dim3(bs), 0, C->stream, C->g,
+                       C->o.h_dry, dz, dh, (const int*)C->d_perm, dzo, dpd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(zout, dzo, sizeof(double) * 3 * (size_t)P.N, hipMemcpyDeviceToHost));
+    if (pd) HIPCHK(hipMemcpy(pd, dpd, sizeof(int) * 4 * (size_t)P.N, hipMemcpyDeviceToHost));
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_fvsources(storm_fvt_t* C, const double* z, const double* h, double* source) {
+  if (!C || !z || !h || !source) return set_err(C, STORM_E_ARG, "null argument");
+  try {
+    const Prep& P = C->prep;
+    if (!C->d_vperm) {
+      C->d_vperm = dupload(C->owned, P.vperm);
+      C->d_nn_ptr = dupload(C->owned, P.nn_ptr);
+      C->d_nn_idx = dupload(C->owned, P.nn_idx);
+    }
+    CallBufs B;
+    const double* dz = vertex_upload(C, B, z);
+    const double* dh = vertex_upload(C, B, h);
+    double* zb = B.get<double>(2 * (size_t)P.P);
+    double* src = B.get<double>(3 * (size_t)P.P);
+    const int bs = 256;
+    const dim3 gv((P.P + bs - 1) / bs);
+    hipLaunchKernelGGL(k_node_grad, gv, dim3(bs), 0, C->stream, C->g, (const int*)C->d_nn_ptr,
+                       (const int*)C->d_nn_idx, dz, dh, zb);
+    hipLaunchKernelGGL(k_node_integral, gv, dim3(bs), 0, C->stream, C->g, (const double*)zb,
+                       (const int*)C->d_vperm, src);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(source, src, sizeof(double) * 3 * (size_t)P.P, hipMemcpyDeviceToHost));
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
   return STORM_OK;
 }
 

@@ -853,6 +853,29 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     S.vzb[2 * (size_t)pp] = zvtx[p];
     std::memcpy(&S.vzb[2 * (size_t)pp + 1], &bits, sizeof(bits));
   }
+  // node_db.f90:165-185: n2n in the internal vertex order, each row in the
+  // order the edge loop fills it
+  {
+    std::vector<int> cnt_in(P + 1, 0), row_in(2 * (size_t)E), fill(P, 0);
+    for (int k = 0; k < E; ++k) { cnt_in[ep[2 * k] + 1]++; cnt_in[ep[2 * k + 1] + 1]++; }
+    for (int p = 0; p < P; ++p) cnt_in[p + 1] += cnt_in[p];
+    for (int k = 0; k < E; ++k) {
+      const int p1 = ep[2 * k], p2 = ep[2 * k + 1];
+      row_in[cnt_in[p1] + fill[p1]++] = p2;
+      row_in[cnt_in[p2] + fill[p2]++] = p1;
+    }
+    S.nn_ptr.assign(P + 1, 0);
+    for (int pp = 0; pp < P; ++pp) {
+      const int p = S.vperm[pp];
+      S.nn_ptr[pp + 1] = S.nn_ptr[pp] + (cnt_in[p + 1] - cnt_in[p]);
+    }
+    S.nn_idx.resize(2 * (size_t)E);
+    for (int pp = 0; pp < P; ++pp) {
+      const int p = S.vperm[pp];
+      for (int t = cnt_in[p]; t < cnt_in[p + 1]; ++t)
+        S.nn_idx[S.nn_ptr[pp] + (t - cnt_in[p])] = vip[row_in[t]];
+    }
+  }
   // enforce_bcs' tables in the internal vertex order
   S.vo_ptr.assign(P + 1, 0);
   for (int v : on_vtx) S.vo_ptr[vip[v] + 1]++;

@@ -110,6 +110,10 @@ struct Prep {
   // dataprepFVT.f90:73-140's inputs given at vertices, in input order (empty
   // when given at cells), for the module copies it leaves
   std::vector<double> vin_h, vin_u, vin_v, vin_cd;
+  // node_db.f90:165-185: n2n, the vertices sharing an edge with vertex p
+  // (internal ids, in edge-table order) as nn_idx[nn_ptr[p] .. nn_ptr[p+1]),
+  // for the node routines of the RDS solver (storm_fvt_fvsources)
+  std::vector<int> nn_ptr, nn_idx;
   // stage-kernel tiles: tile b owns cells [t_ptr[b], t_ptr[b+1]); its halo
   // (edge neighbours outside) is listed per tile; local cell ids are 16
   // bits, three to a word (cell slot j at bit 16*j): tile cells 0..T-1, then

@@ -67,7 +67,8 @@ EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch
            "storm_fvt_get_state", "storm_fvt_get_conserved", "storm_fvt_get_vertex",
            "storm_fvt_get_prep_fields", "storm_fvt_get_output", "storm_fvt_get_c2v_weights",
            "storm_fvt_get_gradients", "storm_fvt_get_gradients_noalloc",
-           "storm_fvt_get_bc_values", "storm_fvt_device_info",
+           "storm_fvt_get_bc_values", "storm_fvt_drynwet", "storm_fvt_fvsources",
+           "storm_fvt_device_info",
            "storm_fvt_last_run_seconds", "storm_fvt_last_error", "storm_fvt_destroy",
            "storm_fvt_inspect_molecule"]
 
@@ -105,6 +106,8 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_get_c2v_weights.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
     L.storm_fvt_get_gradients.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_get_gradients_noalloc.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_drynwet.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip]
+    L.storm_fvt_fvsources.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.storm_fvt_get_bc_values.argtypes = [C.c_void_p, _dp, _dp]
     L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
     L.storm_fvt_last_run_seconds.restype = C.c_double
@@ -329,6 +332,26 @@ class FVTSolver:
         gx, gy = np.zeros(self.N), np.zeros(self.N)
         self._chk(load().storm_fvt_get_gradients(self.h, _p(gx, C.c_double), _p(gy, C.c_double)))
         return gx, gy
+
+    def drynwet(self, z, h):
+        """drynwet + part_dry per triangle for vertex bed z and depth h:
+        (zout (N, 3), pd (N, 4))."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        zo = np.zeros(3 * self.N)
+        pd = np.zeros(4 * self.N, dtype=np.int32)
+        self._chk(load().storm_fvt_drynwet(self.h, _p(z, C.c_double), _p(h, C.c_double),
+                                           _p(zo, C.c_double), _p(pd, C.c_int32)))
+        return zo.reshape(self.N, 3), pd.reshape(self.N, 4)
+
+    def fvsources(self, z, h):
+        """fvsources' source (3, P) for vertex bed z and depth h."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        src = np.zeros(3 * self.P)
+        self._chk(load().storm_fvt_fvsources(self.h, _p(z, C.c_double), _p(h, C.c_double),
+                                             _p(src, C.c_double)))
+        return src.reshape(3, self.P)
 
     def bc_values(self):
         """qin, hbc of the latest step (get_time_bcs' module values)."""
