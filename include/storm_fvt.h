@@ -119,6 +119,12 @@ typedef struct storm_fvt_opts {
   int32_t nonfinite_stop; /* 1 = a steady run (or reach of a batch) whose residual is
                              not finite stops there, with rmax NaN; 0 = it runs on to
                              MAX ITERATION, as the reference does */
+  int32_t fluxes;    /* interior-edge loop: 0 = invisc_fluxes.f90 (the one the
+                        reference builds); 1 = invisc_fluxes_xsf.f90 (a dry side
+                        keeps its cell depth, the left state takes the right
+                        cell's y-gradients of u and v as written at :65-66, an
+                        ALGORITHM I1 wall zeroes only the mass flux of flux_ac;
+                        needs flux AC and tile capacity 256) */
 } storm_fvt_opts_t;
 
 typedef struct storm_fvt storm_fvt_t;

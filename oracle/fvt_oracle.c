@@ -1258,9 +1258,15 @@ static inline void face(const orc_t *s, int i, int m, double *h, double *u, doub
   *v = s->v[i] + s->delv[2 * i] * rx + s->delv[2 * i + 1] * ry;
 }
 
-/* invisc_fluxes.f90:39-186: interior edges */
+/* invisc_fluxes.f90:39-186: interior edges.  With fluxes = 1, the loop of
+ * invisc_fluxes_xsf.f90:41-164 instead: a dry side keeps its cell depth
+ * (:58,74); the left state takes the y-components of the RIGHT cell's u and
+ * v gradients (delu(j)%y, delv(j)%y at :65-66, kept as written); where
+ * ALGORITHM I1 sets a wall, flux_ac runs on the unchanged states and only
+ * the mass flux is zeroed (:121,125); the flux is always flux_ac. */
 static void interior_fluxes(orc_t *s) {
   const double h_dry = s->o.h_dry, h_wet = s->o.h_wet;
+  const int xsf = s->o.fluxes == 1;
   for (int l = 0; l < s->nflow1; ++l) {
     int k = s->flowedg1[l];
     int i = s->ee[2 * k], j = s->ee[2 * k + 1];
@@ -1270,13 +1276,18 @@ static void interior_fluxes(orc_t *s) {
     double nx = s->nrm[2 * k], ny = s->nrm[2 * k + 1];
     if (s->gsign[3 * i + m] < 0) { nx = -nx; ny = -ny; }
     double hl, ul, vl, hr, ur, vr;
-    if (s->dry[i]) { ul = ZERO; vl = ZERO; hl = ZERO; }
+    if (s->dry[i]) { ul = ZERO; vl = ZERO; hl = xsf ? s->h[i] : ZERO; }
     else {
       face(s, i, m, &hl, &ul, &vl);
       if (hl < ZERO) hl = ZERO;
+      if (xsf) {
+        const double rx = s->rc[6 * i + 2 * m], ry = s->rc[6 * i + 2 * m + 1];
+        ul = s->u[i] + s->delu[2 * i] * rx + s->delu[2 * j + 1] * ry;
+        vl = s->v[i] + s->delv[2 * i] * rx + s->delv[2 * j + 1] * ry;
+      }
     }
     int n = local_edge(s, j, k);
-    if (s->dry[j]) { ur = ZERO; vr = ZERO; hr = ZERO; }
+    if (s->dry[j]) { ur = ZERO; vr = ZERO; hr = xsf ? s->h[j] : ZERO; }
     else {
       face(s, j, n, &hr, &ur, &vr);
       if (hr < ZERO) hr = ZERO;
@@ -1286,11 +1297,14 @@ static void interior_fluxes(orc_t *s) {
       double h0 = fmax2(hr, hl);
       double z0 = HALF * (s->zvtx[s->ep[2 * k]] + s->zvtx[s->ep[2 * k + 1]]);
       h0 = h0 + z0;
-      if (s->dry[i] && s->z[i] + h_wet > h0) { set_wall = 1; hl = hr; }
-      if (s->dry[j] && s->z[j] + h_wet > h0) { set_wall = 1; hr = hl; }
+      if (s->dry[i] && s->z[i] + h_wet > h0) { set_wall = 1; if (!xsf) hl = hr; }
+      if (s->dry[j] && s->z[j] + h_wet > h0) { set_wall = 1; if (!xsf) hr = hl; }
     }
     double f[3];
-    if (set_wall) {
+    if (xsf) {
+      flux_ac(nx, ny, hl, ul, vl, hr, ur, vr, f);
+      if (set_wall) f[0] = ZERO;
+    } else if (set_wall) {
       f[0] = ZERO;
       hl = fmax2(hl, hr);
       f[1] = HALF * G * hl * hl * nx;

@@ -55,6 +55,7 @@ typedef struct orc_opts {
   int32_t friction;    /* 0 = implicit Manning (fric_terms_impl), 1 = explicit (fric_terms) */
   int32_t vclip;
   double cfl, rstop, h_dry, h_wet, vceiling;
+  int32_t fluxes;      /* interior-edge loop: 0 = invisc_fluxes.f90, 1 = invisc_fluxes_xsf.f90 */
 } orc_opts_t;
 
 typedef struct orc orc_t;

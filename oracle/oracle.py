@@ -41,7 +41,10 @@ class OrcOpts(C.Structure):
                 ("hvalve", C.c_int32), ("limiter", C.c_int32), ("flux", C.c_int32),
                 ("gradient", C.c_int32), ("friction", C.c_int32), ("vclip", C.c_int32),
                 ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
-                ("h_wet", C.c_double), ("vceiling", C.c_double)]
+                ("h_wet", C.c_double), ("vceiling", C.c_double), ("fluxes", C.c_int32)]
+
+
+FLUXES = {"std": 0, "xsf": 1}
 
 
 def build(force: bool = False) -> str:
@@ -99,7 +102,7 @@ def opts_struct(o: Dict[str, object]) -> OrcOpts:
                    friction=int(o["friction"]), vclip=int(bool(o["vclip"])),
                    cfl=float(o["cfl"]), rstop=float(o["rstop"]), h_dry=h_dry,
                    h_wet=max(1.0, float(o["c_threshold"])) * h_dry,
-                   vceiling=float(o["vceiling"]))
+                   vceiling=float(o["vceiling"]), fluxes=FLUXES[str(o.get("fluxes", "std"))])
 
 
 class Oracle:
@@ -283,6 +286,8 @@ def equals(a: float, b: float) -> bool:
 
 
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the reference linked with invisc_fluxes_xsf.f90 in place of invisc_fluxes.f90
+REF_DRIVER_XSF = os.path.join(HERE, "_ref", "ref_driver_xsf")
 
 
 def run_reference(case, workdir: str, timeout: float = 600.0):
@@ -295,7 +300,8 @@ def run_reference(case, workdir: str, timeout: float = 600.0):
     run = _cases.write_run(case, workdir)
     out = os.path.join(workdir, "ref_out.bin")
     with open(os.path.join(workdir, "ref_log.txt"), "w") as log:
-        subprocess.run([REF_DRIVER, os.path.basename(run), "ref_out.bin", "ref_prep.bin"],
+        drv = REF_DRIVER_XSF if str(case.opts.get("fluxes", "std")) == "xsf" else REF_DRIVER
+        subprocess.run([drv, os.path.basename(run), "ref_out.bin", "ref_prep.bin"],
                        cwd=workdir, stdout=log, stderr=subprocess.STDOUT, check=True,
                        timeout=timeout)
     res = _cases.read_ref_output(out)

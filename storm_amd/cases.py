@@ -170,7 +170,7 @@ def default_opts(**kw) -> Dict[str, object]:
         rkorder=2, cfl=1.0, dt_op=0, steady=False, stop_cr=0, rstop=1.0e-4,
         istop=0, h_dry=0.001, c_threshold=1.0, btype=0, hvalve=True,
         limiter=2, flux="ac", gradient="lsq", friction=0, vclip=False,
-        vceiling=35.0)
+        vceiling=35.0, fluxes="std")
     o.update(kw)
     return o
 

@@ -104,6 +104,7 @@ struct Geo {
 struct Opts {
   int rkorder, dt_op, steady, stop_cr, btype, hvalve, limiter, flux, gradient, friction, vclip;
   int nonfinite_stop;
+  int xsf;  // interior edges as invisc_fluxes_xsf.f90 forms them
   double cfl, rstop, h_dry, h_wet, vceiling;
 };
 
@@ -673,6 +674,10 @@ struct Faces {
   double dhx, dhy, up, vp, h, umag;  // umag: |u| of the stage input, 0 when dry
   bool dry;
   bool fix;  // hgradfix2 set the gradient: rk hu, hv of slot s-1 are zeroed
+  // XSF faces only: fu, fv hold u + du/dx rcx (v likewise) without the
+  // y-gradient term, which the xsf edge loop takes from the edge's right
+  // cell; rcy of each side and the cell's own du/dy, dv/dy
+  double rcy[3], guy, gvy;
 };
 
 // find_dry_cells' flags (carried in the state), cell_grads.f90:24-75 (least
@@ -680,6 +685,7 @@ struct Faces {
 // sums of lsq_setup.f90:105-145 formed on the fly, or Green-Gauss),
 // hgradfix2.f90:46-98 and the cell pass of to_vertices.f90:44-55, for cell i.
 // vw: the cell's three vertices are wet.
+template <bool XSF = false>
 __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const CellD* din,
                                            const double* gwork, int i, uint64_t meta, int nb0,
                                            int nb1, int nb2, const Tri& tr, bool vw, Faces& F) {
@@ -813,14 +819,24 @@ __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const Ce
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     double fh = hi + dhx * rcx[j] + dhy * rcy[j];
-    double fu = up + gux * rcx[j] + guy * rcy[j];
-    double fv = vp + gvx * rcx[j] + gvy * rcy[j];
+    double fu, fv;
+    if (XSF) {  // the partial sums; the edge adds a y-gradient term
+      fu = up + gux * rcx[j];
+      fv = vp + gvx * rcx[j];
+    } else {
+      fu = up + gux * rcx[j] + guy * rcy[j];
+      fv = vp + gvx * rcx[j] + gvy * rcy[j];
+    }
     if (nb[j] >= 0) {
-      if (dry_i) { fh = ZERO; fu = ZERO; fv = ZERO; }
+      // a dry cell: velocity 0 and depth 0, or under invisc_fluxes_xsf.f90:
+      // 56-58 its cell depth
+      if (dry_i) { fh = XSF ? hi : ZERO; fu = ZERO; fv = ZERO; }
       else if (fh < ZERO) fh = ZERO;
     }
     F.fh[j] = fh; F.fu[j] = fu; F.fv[j] = fv;
+    if (XSF) F.rcy[j] = rcy[j];
   }
+  if (XSF) { F.guy = guy; F.gvy = gvy; }
   F.dhx = dhx; F.dhy = dhy; F.up = up; F.vp = vp; F.h = hi;
   F.umag = dry_i ? ZERO : sqrt(ui * ui + vi * vi);
   F.dry = dry_i;
@@ -1013,11 +1029,16 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
   bool set_wall = false;
   if (dL || dR) {
     const double h0 = dmax(hr, hl) + sd.zm;
-    if (dL && zL + h_wet > h0) { set_wall = true; hl = hr; }
-    if (dR && zR + h_wet > h0) { set_wall = true; hr = hl; }
+    if (dL && zL + h_wet > h0) { set_wall = true; if (!o.xsf) hl = hr; }
+    if (dR && zR + h_wet > h0) { set_wall = true; if (!o.xsf) hr = hl; }
   }
   double f[3];
-  if (set_wall) {
+  if (o.xsf) {
+    // invisc_fluxes_xsf.f90:118-121: flux_ac of the states, mass flux
+    // zeroed at an ALGORITHM I1 wall
+    flux_ac(nx, ny, hl, ul, vl, hr, ur, vr, f);
+    if (set_wall) f[0] = ZERO;
+  } else if (set_wall) {
     f[0] = ZERO;
     hl = dmax(hl, hr);
     f[1] = HALF * G * hl * hl * nx;
@@ -1045,6 +1066,7 @@ constexpr int QMAX = 512;  // edges of one inflowQbyH string
 
 struct Lds {
   double *fh, *fu, *fv;          // [3][CAP] face states, side-major
+  double *rcy, *gy;              // XSF only: [3][CAP] rcy per side, [2][CAP] du/dy, dv/dy
   double *z, *h;                 // [CAP] bed and stage-input depth
   double *vx, *vy, *vz;          // [3][TMAX] vertex coordinates and bed of the owned cells
   uint8_t* dry;                  // [CAP] dry flags
@@ -1055,17 +1077,21 @@ struct Lds {
 // Face states, bed and depth for every cell of the tile and halo; vertex
 // coordinates and bed for the owned cells only: 40.3 KB at capacity 256
 // (three blocks per CU, as many as the registers allow).
-__host__ __device__ constexpr size_t lds_bytes(int cap) {
-  return sizeof(double) * (11 * (size_t)cap + 9 * (size_t)tile_tmax(cap)) + (size_t)cap;
+__host__ __device__ constexpr size_t lds_bytes(int cap, bool xsf = false) {
+  return sizeof(double) * ((xsf ? 16 : 11) * (size_t)cap + 9 * (size_t)tile_tmax(cap)) +
+         (size_t)cap;
 }
 
-__device__ __forceinline__ Lds lds_layout(double* base, int cap) {
+__device__ __forceinline__ Lds lds_layout(double* base, int cap, bool xsf = false) {
   Lds L;
   L.fh = base; L.fu = L.fh + 3 * cap; L.fv = L.fu + 3 * cap;
   L.z = L.fv + 3 * cap; L.h = L.z + cap;
   const int tm = tile_tmax(cap);
   L.vx = L.h + cap; L.vy = L.vx + 3 * tm; L.vz = L.vy + 3 * tm;
-  L.dry = reinterpret_cast<uint8_t*>(L.vz + 3 * tm);
+  // XSF kernels only (lds_bytes(cap, true)): rcy and the y-gradients
+  L.rcy = L.vz + 3 * tm;
+  L.gy = L.rcy + 3 * cap;
+  L.dry = reinterpret_cast<uint8_t*>(xsf ? L.gy + 2 * cap : L.vz + 3 * tm);
   // overlay of the 20*cap cell doubles: 5*QMAX doubles and QMAX flags
   L.qh = base; L.qu = L.qh + QMAX; L.qv = L.qu + QMAX; L.qa = L.qv + QMAX; L.qq = L.qa + QMAX;
   L.qok = reinterpret_cast<uint8_t*>(L.qq + QMAX);
@@ -1193,7 +1219,7 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 //      fixed order by k_control).
 // Three waves per SIMD (about 150 VGPRs, no spills).  Forcing four (128
 // VGPRs) spills the prefetched phase-4 inputs and runs 30 % slower.
-template <int CAP>
+template <int CAP, bool XSF = false>
 __global__ void __launch_bounds__(CAP)
 k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   extern __shared__ double lds_dyn[];
@@ -1241,7 +1267,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   if (st.s == 1 && t == 0 && b == ri.tile0) S.vdone[hd1.w] = done;
   if (done) return;
   const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
-  const Lds L = lds_layout(lds_dyn, CAP);
+  const Lds L = lds_layout(lds_dyn, CAP, XSF);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (t == 0 && o.steady && st.last) {
     s_dlast = last_cell_dt(g, o, S.din, ri.last_cell);
@@ -1268,7 +1294,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       tr.x[2] = q2.x; tr.y[2] = q2.y; tr.zv[2] = q2.z;
     }
     const bool vw = S.vwet[v0] && S.vwet[v1] && S.vwet[v2];
-    cell_faces(g, o, S.din, S.gwork, i, meta, nb0, nb1, nb2, tr, vw, F);
+    cell_faces<XSF>(g, o, S.din, S.gwork, i, meta, nb0, nb1, nb2, tr, vw, F);
     if (own) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -1285,7 +1311,9 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       L.fh[j * CAP + t] = F.fh[j]; L.fu[j * CAP + t] = F.fu[j]; L.fv[j * CAP + t] = F.fv[j];
+      if (XSF) L.rcy[j * CAP + t] = F.rcy[j];
     }
+    if (XSF) { L.gy[t] = F.guy; L.gy[CAP + t] = F.gvy; }
     L.z[t] = own ? pf_cs.z : g.cs[i].z;
     L.h[t] = F.h;
     L.dry[t] = F.dry;
@@ -1311,14 +1339,29 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     const double area = tri_area(tr);
     auto side_flux = [&](uint32_t bits, const Side& sd, int j, double& d0, double& d1,
                          double& d2) {
-      const double fho = L.fh[j * CAP + t], fuo = L.fu[j * CAP + t], fvo = L.fv[j * CAP + t];
+      const double fho = L.fh[j * CAP + t];
+      double fuo = L.fu[j * CAP + t], fvo = L.fv[j * CAP + t];
       if ((bits & CMS_KIND) == SK_INTERIOR) {
         const int k = lid(ln, j), nl = (bits >> CMS_NL) & 3;
         Nbr nb;
         nb.fh = L.fh[nl * CAP + k]; nb.fu = L.fu[nl * CAP + k]; nb.fv = L.fv[nl * CAP + k];
         nb.z = L.z[k]; nb.h = L.h[k]; nb.dry = L.dry[k];
+        if (XSF) {
+          // invisc_fluxes_xsf.f90:65-66,79-80: both states of the edge take
+          // the y-gradients of its right cell (edges%e(2))
+          const int r = ((bits >> CMS_LEFT) & 1) ? k : t;
+          const double gu = L.gy[r], gv = L.gy[CAP + r];
+          fuo = dry_i ? ZERO : fuo + gu * L.rcy[j * CAP + t];
+          fvo = dry_i ? ZERO : fvo + gv * L.rcy[j * CAP + t];
+          nb.fu = nb.dry ? ZERO : nb.fu + gu * L.rcy[nl * CAP + k];
+          nb.fv = nb.dry ? ZERO : nb.fv + gv * L.rcy[nl * CAP + k];
+        }
         interior_flux(o, bits, sd, dry_i, hi, zi, fho, fuo, fvo, nb, d0, d1, d2);
       } else {
+        if (XSF) {  // the cell's own gradient, as invisc_fluxes' boundary routines
+          fuo = fuo + L.gy[t] * L.rcy[j * CAP + t];
+          fvo = fvo + L.gy[CAP + t] * L.rcy[j * CAP + t];
+        }
         const int va = g.cvtx[j * N + i], vb = g.cvtx[(j == 2 ? 0 : j + 1) * N + i];
         const double ha = S.hvtx[va], hb2 = S.hvtx[vb];
         const bool rev = (bits >> CMS_REV) & 1;
@@ -2014,11 +2057,13 @@ void upload_tables(storm_fvt* C) {
   put(C->d_hts_t, P.hts_t); put(C->d_hts_v, P.hts_v);
 }
 
-void set_lds_limit(int cap) {
-  const void* f = (cap == 256) ? reinterpret_cast<const void*>(&k_stage<256>)
+void set_lds_limit(int cap, bool xsf) {
+  const void* f = xsf ? reinterpret_cast<const void*>(&k_stage<256, true>)
+                : (cap == 256) ? reinterpret_cast<const void*>(&k_stage<256>)
                 : (cap == 384) ? reinterpret_cast<const void*>(&k_stage<384>)
                                : reinterpret_cast<const void*>(&k_stage<512>);
-  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(cap)));
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes(cap, xsf)));
 }
 
 void setup_device(storm_fvt* C) {
@@ -2062,6 +2107,7 @@ void setup_device(storm_fvt* C) {
   o.btype = q.btype; o.hvalve = q.hvalve; o.limiter = q.limiter; o.flux = q.flux;
   o.gradient = q.gradient; o.friction = q.friction; o.vclip = q.vclip;
   o.nonfinite_stop = q.nonfinite_stop;
+  o.xsf = q.fluxes == 1;
   o.cfl = q.cfl; o.rstop = q.rstop; o.h_dry = q.h_dry; o.h_wet = q.h_wet; o.vceiling = q.vceiling;
   const size_t N = P.N;
   const int R = P.R;
@@ -2072,6 +2118,8 @@ void setup_device(storm_fvt* C) {
   C->cap = q.block > 0 ? q.block : DEFAULT_CAP;
   if (C->cap != 256 && C->cap != 384 && C->cap != 512)
     throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256, 384 or 512");
+  if (o.xsf && C->cap != 256)
+    throw StormErr(STORM_E_ARG, "the xsf edge loop runs with tile capacity 256");
   if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
   Tiles& tl = C->tl;
   tl.hdr = reinterpret_cast<const int4*>(dupload(own, P.t_hdr));
@@ -2093,7 +2141,7 @@ void setup_device(storm_fvt* C) {
     }
     C->g.reach = dupload(own, ri);
   }
-  set_lds_limit(C->cap);
+  set_lds_limit(C->cap, C->o.xsf != 0);
   State& S = C->S;
   S.gwork = (q.gradient == STORM_GRAD_GAUSS) ? dalloc<double>(own, 6 * N) : nullptr;
   S.hvtx = dalloc<double>(own, P.P);
@@ -2144,10 +2192,10 @@ void launch_init(storm_fvt* C) {
                      C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.zeta, C->S.part);
 }
 
-template <int CAP>
+template <int CAP, bool XSF = false>
 void launch_stage(storm_fvt* C, const Stage& st) {
-  hipLaunchKernelGGL(k_stage<CAP>, dim3(C->tl.nt), dim3(CAP), lds_bytes(CAP), C->stream, C->g,
-                     C->o, C->tl, st, C->S);
+  hipLaunchKernelGGL((k_stage<CAP, XSF>), dim3(C->tl.nt), dim3(CAP), lds_bytes(CAP, XSF),
+                     C->stream, C->g, C->o, C->tl, st, C->S);
 }
 
 void start_run(storm_fvt* C) {
@@ -2240,7 +2288,8 @@ void launch_step(storm_fvt* C, bool launch = true) {
       if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
         hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
                            C->S.din, C->S.gwork);
-      switch (C->cap) {
+      if (C->o.xsf) launch_stage<256, true>(C, st);
+      else switch (C->cap) {
         case 256: launch_stage<256>(C, st); break;
         case 384: launch_stage<384>(C, st); break;
         default: launch_stage<512>(C, st); break;

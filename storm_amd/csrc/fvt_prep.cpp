@@ -261,8 +261,13 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
   if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
   if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 2 || o->limiter < 0 ||
       o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
-      o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2) {
+      o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2 || o->fluxes < 0 ||
+      o->fluxes > 1) {
     S.err = "invalid option value"; return STORM_E_ARG;
+  }
+  // invisc_fluxes_xsf.f90 calls flux_ac only
+  if (o->fluxes == 1 && o->flux != STORM_FLUX_AC) {
+    S.err = "the xsf edge loop uses the Roe flux (flux_ac) only"; return STORM_E_ARG;
   }
   if (!(o->h_dry > 0) || !(o->h_wet >= 0) || !(o->cfl > 0)) { S.err = "invalid threshold or cfl"; return STORM_E_ARG; }
   if (m->n_pts < 3 || m->n_elems < 1 || !m->x || !m->y || !m->tri || !m->z || !m->h || !m->u ||

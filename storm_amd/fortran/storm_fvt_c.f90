@@ -23,7 +23,7 @@ MODULE storm_fvt_c
     INTEGER(C_INT32_T) :: rkorder, dt_op, steady, stop_cr, istop, btype, hvalve, &
                           limiter, flux, gradient, friction, vclip
     REAL(C_DOUBLE) :: cfl, rstop, h_dry, h_wet, vceiling
-    INTEGER(C_INT32_T) :: reorder, block, graph, nonfinite_stop
+    INTEGER(C_INT32_T) :: reorder, block, graph, nonfinite_stop, fluxes
   END TYPE storm_fvt_opts_t
 
   INTERFACE

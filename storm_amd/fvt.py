@@ -28,6 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libstorm_fvt.so")
 
 FLUX = {"ac": 0, "hll": 1, "rusanov": 2, "bgnvc": 3}
+FLUXES = {"std": 0, "xsf": 1}  # invisc_fluxes.f90 or invisc_fluxes_xsf.f90
 GRAD = {"lsq": 0, "gauss": 1, "hdls": 2}
 
 _dp = C.POINTER(C.c_double)
@@ -58,7 +59,7 @@ class Opts(C.Structure):
                 ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
                 ("h_wet", C.c_double), ("vceiling", C.c_double),
                 ("reorder", C.c_int32), ("block", C.c_int32), ("graph", C.c_int32),
-                ("nonfinite_stop", C.c_int32)]
+                ("nonfinite_stop", C.c_int32), ("fluxes", C.c_int32)]
 
 
 EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch",
@@ -201,6 +202,7 @@ def opts_struct(o: Dict[str, object]) -> Opts:
     s.block = int(o.get("block", 0))
     s.graph = int(o.get("graph", 0))
     s.nonfinite_stop = int(bool(o.get("nonfinite_stop", False)))
+    s.fluxes = FLUXES[str(o.get("fluxes", "std"))]
     return s
 
 

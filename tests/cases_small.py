@@ -107,3 +107,29 @@ def option_cases():
                                                  h0=2.0, steady=True, stop_cr=2, rstop=1e-5,
                                                  limiter=0, flux="hll"), 20000),
     }
+
+
+def xsf_cases():
+    """Cases of the invisc_fluxes_xsf.f90 interior-edge loop (option fluxes
+    "xsf"), pinned by the reference linked with that file in place of
+    invisc_fluxes.f90: wet/dry fronts (dry sides keep their depth, ALGORITHM
+    I1 walls), a fully wet flume (the left state's y-gradients taken from the
+    right cell) and a meander with banks at RK3."""
+    return {
+        "dambreak_xsf": (cases.dambreak(nq=14, side=200.0, seed=8, x_dam=80.0, h_res=2.0,
+                                        flux="ac", dt_op=0, cfl=0.02, istop=60, h_dry=1e-3,
+                                        fluxes="xsf"), 60),
+        "flume_xsf": (cases.flume(nx=20, ny=5, length=100.0, width=25.0, seed=9, dt_op=0,
+                                  cfl=0.05, istop=30, q_in=20.0, fluxes="xsf"), 30),
+        "meander_xsf_rk3": (cases.meander(ns=40, nn=12, length=400.0, width=120.0, amp=20.0,
+                                          wavelength=200.0, chan_w=40.0, chan_d=2.0, n_bumps=5,
+                                          q0=5.0, q1=20.0, t_ramp=10.0, dt_op=0, cfl=0.02,
+                                          istop=60, rkorder=3, seed=10, fluxes="xsf"), 60),
+    }
+
+
+def golden_cases():
+    """Every case with a golden fixture from the reference."""
+    out = dict(small_cases())
+    out.update(xsf_cases())
+    return out

@@ -23,7 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle as O  # noqa: E402
 from storm_amd import cases  # noqa: E402
-from cases_small import small_cases  # noqa: E402
+from cases_small import golden_cases  # noqa: E402
 
 KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx",
         "uvtx", "vvtx", "cdvtx", "zvtx", "delhx", "delhy", "e_est")
@@ -32,7 +32,7 @@ KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", 
 def main():
     out_dir = os.path.dirname(os.path.abspath(__file__))
     only = set(sys.argv[1:])
-    for name, (case, n) in small_cases().items():
+    for name, (case, n) in golden_cases().items():
         if only and name not in only:
             continue
         assert int(case.opts["istop"]) == n

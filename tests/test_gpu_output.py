@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from cases_small import small_cases
+from cases_small import golden_cases, small_cases
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -32,11 +32,11 @@ def _cmp(a, b):
     return np.abs(a - b).max() / scale
 
 
-@pytest.mark.parametrize("name", list(small_cases().keys()))
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
 def test_output_fields_match_reference(name):
     import oracle as O
     from storm_amd import fvt
-    case, n = small_cases()[name]
+    case, n = golden_cases()[name]
     print(f"[gpu output] {name} N={case.n_elems} steps={n}", flush=True)
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     s = fvt.FVTSolver(case)

@@ -4,7 +4,7 @@ import os
 import numpy as np
 import pytest
 
-from cases_small import small_cases
+from cases_small import golden_cases, small_cases
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -19,11 +19,11 @@ def _cmp(a, b):
     return np.abs(a - b).max() / scale
 
 
-@pytest.mark.parametrize("name", list(small_cases().keys()))
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
 def test_step_parity(name):
     import oracle as O
     from storm_amd import fvt
-    case, n = small_cases()[name]
+    case, n = golden_cases()[name]
     print(f"[gpu parity] {name} N={case.n_elems} steps={n}", flush=True)
     orc = O.Oracle(case)
     r_o = orc.run(n)

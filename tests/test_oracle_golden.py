@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from cases_small import small_cases
+from cases_small import golden_cases
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx")
@@ -14,10 +14,10 @@ KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", 
 OUT_KEYS = ("uvtx", "vvtx", "cdvtx", "e_est", "delhx", "delhy")
 
 
-@pytest.mark.parametrize("name", list(small_cases().keys()))
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
 def test_oracle_matches_reference_bitwise(name):
     import oracle as O
-    case, n = small_cases()[name]
+    case, n = golden_cases()[name]
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     o = O.Oracle(case)
     r = o.run(n)
@@ -35,13 +35,13 @@ def test_oracle_matches_reference_bitwise(name):
         assert np.array_equal(g[k], out[k]), (k, np.abs(g[k] - out[k]).max())
 
 
-@pytest.mark.parametrize("name", list(small_cases().keys()))
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
 def test_oracle_dataprep_copies_match_reference_bitwise(name):
     """The module copies dataprepFVT.f90:69-140 leaves (ctr2vtx type 3 and
     vtx2ctr of the inputs given at the other location), before any step."""
     import oracle as O
     from storm_amd import cases
-    case, _ = small_cases()[name]
+    case, _ = golden_cases()[name]
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     o = O.Oracle(case)
     p = o.prep()
