@@ -134,6 +134,9 @@ def main():
                          "states (on --reaches reaches, default 8) instead of the RMS dh test")
     ap.add_argument("--modes", default="cold,plane,learned",
                     help="warmstart: starts to time (the ratio needs cold and learned)")
+    ap.add_argument("--wet", type=float, default=None,
+                    help="meander: start with the whole valley under water, this many metres "
+                         "above the valley floor (4 puts every cell under water)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
@@ -168,8 +171,8 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-    case = cases.meander()
-    workload = "meander-2M-rk3"
+    case = cases.meander(wet=args.wet)
+    workload = "meander-2M-rk3" if args.wet is None else "meander-2M-rk3-wet"
     t0 = time.perf_counter()
     solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph, block=args.block))
     setup_s = time.perf_counter() - t0
@@ -178,6 +181,7 @@ def main():
     it, ttime, rmax = solver.run(args.steps)
     wall = time.perf_counter() - t0
     gpu_s = solver.last_run_seconds()
+    wet_frac = float((solver.state()["h"] > case.opts["h_dry"]).mean())
     n = case.n_elems
     value = n * args.steps / wall
     step_s = gpu_s / args.steps
@@ -199,6 +203,10 @@ def main():
         "config": {"workload": workload, "triangles": n, "vertices": case.n_pts,
                    "scheme": "Roe flux_ac, LSQ+Barth-Jespersen, SSP-RK3, CFL 0.9",
                    "boundaries": "Q ramp 500->3000 m3/s inflow, critical outflow, walls",
+                   "initial": ("channel at bank-full, floodplain dry" if args.wet is None else
+                               "whole valley under water, stage %g m above the floor"
+                               % args.wet),
+                   "wet_fraction": round(wet_frac, 4),
                    "setup_s": round(setup_s, 3), "sim_time_s": ttime},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM,

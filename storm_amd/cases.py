@@ -279,7 +279,8 @@ def meander(ns: int = 4000, nn: int = 250, length: float = 20000.0,
             chan_w: float = 80.0, chan_d: float = 4.0, valley_slope: float = 5e-4,
             n_bumps: int = 200, seed: int = 2002, n_chan: float = 0.035,
             n_fp: float = 0.06, q0: float = 500.0, q1: float = 3000.0,
-            t_ramp: float = 3600.0, name: str = "meander", **opt_kw) -> Case:
+            t_ramp: float = 3600.0, name: str = "meander", wet: Optional[float] = None,
+            **opt_kw) -> Case:
     """Sinusoidal meandering channel.
 
     Grid in valley coordinates (s, n), s in [0, L], n in [-W/2, W/2], mapped
@@ -288,7 +289,10 @@ def meander(ns: int = 4000, nn: int = 250, length: float = 20000.0,
     chan_d below the valley floor z_v(s) = valley_slope*(L - s); the
     floodplain carries n_bumps Gaussian bumps (height U[0.5,2] m, sigma
     U[20,80] m, centres uniform over the floodplain).  Initial water at bank
-    full in the channel (stage z_v), floodplain dry.  Inflow string on s=0
+    full in the channel (stage z_v), floodplain dry; with ``wet`` the stage is
+    z_v + wet over the whole valley instead (wet = 4 puts every cell under
+    water).
+    Inflow string on s=0
     across the channel only (vtype 1, Q ramp q0 -> q1 over t_ramp), critical
     outflow (htype 2) across the whole section at s=L, walls elsewhere.
     Random draws, in order: s-jitter, n-jitter, diagonals, bump centres s,
@@ -332,7 +336,10 @@ def meander(ns: int = 4000, nn: int = 250, length: float = 20000.0,
     nc = nrm[tri].sum(axis=1) / 3.0
     zc = zvtx[tri].sum(axis=1) / 3.0
     stage = valley_slope * (length - sc)
-    h = np.where(np.abs(nc) < half_c, np.maximum(stage - zc, 0.0), 0.0)
+    if wet is None:
+        h = np.where(np.abs(nc) < half_c, np.maximum(stage - zc, 0.0), 0.0)
+    else:  # the whole valley under water, wet metres above the floor
+        h = np.maximum(stage + wet - zc, 0.0)
     cd = np.where(np.abs(nc) < half_c, n_chan, n_fp)
     u = np.zeros(n_cells)
     v = np.zeros(n_cells)

@@ -685,13 +685,34 @@ struct Faces {
 // sums of lsq_setup.f90:105-145 formed on the fly, or Green-Gauss),
 // hgradfix2.f90:46-98 and the cell pass of to_vertices.f90:44-55, for cell i.
 // vw: the cell's three vertices are wet.
-template <bool XSF = false>
+// SKIPDRY (the stage kernel): a dry cell whose sides are interior, wall or
+// flux-free sides returns at once with zero faces.  Its interior faces are
+// zero whatever its gradients (invisc_fluxes.f90:60-64), the wall routines
+// return for dry cells, and its bed-slope term is skipped, so no flux or
+// update reads the gradients it would form; the other fields are what the
+// full path gives a dry cell.  (The xsf loop reads a dry right cell's
+// y-gradients, so it takes the full path.)
+template <bool XSF = false, bool SKIPDRY = false>
 __device__ __forceinline__ void cell_faces(const Geo& g, const Opts& o, const CellD* din,
                                            const double* gwork, int i, uint64_t meta, int nb0,
                                            int nb1, int nb2, const Tri& tr, bool vw, Faces& F) {
   const int N = g.N;
   const int nb[3] = {nb0, nb1, nb2};
   const CellD di = ld(&din[i]);
+  if (SKIPDRY && !XSF && di.dry != ZERO) {
+    const uint32_t k0 = slot_bits(meta, 0) & CMS_KIND, k1 = slot_bits(meta, 1) & CMS_KIND,
+                   k2 = slot_bits(meta, 2) & CMS_KIND;
+    const auto plain = [](uint32_t k) { return k == SK_INTERIOR || k == SK_WALL || k == SK_NONE; };
+    if (plain(k0) && plain(k1) && plain(k2)) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { F.fh[j] = ZERO; F.fu[j] = ZERO; F.fv[j] = ZERO; }
+      F.dhx = ZERO; F.dhy = ZERO;
+      F.up = di.u; F.vp = di.v; F.h = di.h; F.umag = ZERO;
+      F.dry = true;
+      F.fix = false;
+      return;
+    }
+  }
   const CellS si = ld(&g.cs[i]);
   CellD dn[3];
   CellS sn[3];
@@ -1294,7 +1315,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       tr.x[2] = q2.x; tr.y[2] = q2.y; tr.zv[2] = q2.z;
     }
     const bool vw = S.vwet[v0] && S.vwet[v1] && S.vwet[v2];
-    cell_faces<XSF>(g, o, S.din, S.gwork, i, meta, nb0, nb1, nb2, tr, vw, F);
+    cell_faces<XSF, true>(g, o, S.din, S.gwork, i, meta, nb0, nb1, nb2, tr, vw, F);
     if (own) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -1376,8 +1397,13 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
 #pragma unroll 1  // one side at a time: fewer live registers
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
-      double d0, d1, d2;
-      side_flux(slot_bits(meta, j), side_dyn<CAP>(L, t, j), j, d0, d1, d2);
+      const uint32_t bits = slot_bits(meta, j), kind = bits & CMS_KIND;
+      // no flux between two dry cells (invisc_fluxes.f90:47) nor along a dry
+      // cell's wall (wenslp3_fluxes.f90:40, weslp2_fluxes.f90:40): the side's
+      // geometry is not formed
+      const bool none = dry_i && (kind == SK_WALL || (kind == SK_INTERIOR && L.dry[lid(ln, j)]));
+      double d0 = ZERO, d1 = ZERO, d2 = ZERO;
+      if (!none) side_flux(bits, side_dyn<CAP>(L, t, j), j, d0, d1, d2);
       p0 = p0 + d0; p1 = p1 + d1; p2 = p2 + d2;
     }
     if (o.friction == 1) {
