@@ -125,6 +125,13 @@ typedef struct storm_fvt_opts {
                         cell's y-gradients of u and v as written at :65-66, an
                         ALGORITHM I1 wall zeroes only the mass flux of flux_ac;
                         needs flux AC and tile capacity 256) */
+  int32_t rsavg;     /* ACTIVATE RSAVG: residual smoothing after the source
+                        terms (solverFVT.f90:238-256): rsavg_n Gauss-Seidel
+                        sweeps in input cell order over elemnt_db's t2t with
+                        the factors of :59-60; needs tile capacity 256 */
+  int32_t rsavg_n;   /* nRSAVG (reference default 2) */
+  double rsavg_eps;  /* epsRSAVG (reference default 0.5), taken as
+                        min(|eps|, 1) as read_options.f90:351 does */
 } storm_fvt_opts_t;
 
 typedef struct storm_fvt storm_fvt_t;
@@ -225,6 +232,29 @@ int storm_fvt_get_gradients_noalloc(storm_fvt_t *ctx, double *dhx, double *dhy);
 int storm_fvt_drynwet(storm_fvt_t *ctx, const double *z, const double *h, double *zout,
                       int32_t *pd);
 int storm_fvt_fvsources(storm_fvt_t *ctx, const double *z, const double *h, double *source);
+/* Wind forcing terms (replaces dataprepFVT.f90:237-243 -> windforcing.f90:
+ * 43-86, FVT case; applied at solverFVT.f90:218-224): n == n_elems values
+ * per cell or n == n_pts per vertex (input order; vertex values are averaged
+ * to the cells by vtx2ctr.f90:32-35) of the coefficient c and the wind
+ * components wx, wy.  Each wet cell's momentum residuals gain
+ * c w |w| / rho (rho = 998.6, constants.f90:29), with no area factor, as the
+ * reference adds them.  n = 0 turns the terms off.  Takes effect from the
+ * next step. */
+int storm_fvt_set_wind(storm_fvt_t *ctx, int64_t n, const double *c, const double *wx,
+                       const double *wy);
+/* Culverts (replaces dataprepFVT.f90:309 -> cvprep.f90:29-58 on the data
+ * ioculvert.f90:84-131 reads; discharges by cvdisch.f90:26-58 every stage,
+ * applied at solverFVT.f90:227-233).  n culverts: pin [3n] inlet x, y and
+ * invert elevation; pout [2n] outlet x, y; ntab [n] (>= 1) points of each
+ * performance curve; head, q: the curves end to end (ascending heads above
+ * the invert, discharges).  Each point is placed in the first cell (input
+ * order) that holds it; STORM_E_ARG names a point in no cell.  n = 0 removes
+ * them.  One reach only (a batch of several takes STORM_E_ARG). */
+int storm_fvt_set_culverts(storm_fvt_t *ctx, int32_t n, const double *pin, const double *pout,
+                           const int32_t *ntab, const double *head, const double *q);
+/* The culverts' cells (cvtrigin, cvtrigout: 0-based input cell ids) and the
+ * discharges cvsrc of the latest stage run [n]; any pointer may be NULL. */
+int storm_fvt_get_culverts(storm_fvt_t *ctx, int32_t *cell_in, int32_t *cell_out, double *src);
 /* The boundary values of the latest step: qin[n_in] (Q, or stage for vtype
  * 4) and hbc[n_out] (stage; meaningful for htype 1 strings).  NULL skips a
  * side. */

@@ -39,6 +39,7 @@
 #define HALF 0.5
 #define ONE 1.0
 #define ZERO 0.0
+#define RHO 998.60  /* constants.f90:29 */
 static const double ONE_THIRD = 1.0 / 3.0;
 
 /* Sensitivity probe (test use only, off by default): with probability p per
@@ -115,6 +116,15 @@ struct orc {
   int *gconn;              /* [3N] gauss */
   double *gcx, *gcy, *gwt; /* [3N] gauss */
   double rka[3][3], rkb[3];
+  /* elemnt_db.f90:42-128's t2t kept for the residual smoothing, which reads
+   * it whatever the gradients use (solverFVT.f90:249) */
+  int *et2t_ptr, *et2t;
+  /* wind (windforcing.f90:64-86): per cell, NULL when off */
+  double *wcoef1, *wcoef2;
+  /* culverts: cvprep.f90's cells and offsets, ioculvert.f90's curves */
+  int ncv, *cvin, *cvout, *cvtab_ptr;
+  double *cvpin, *cvrin, *cvhead, *cvq, *cvsrc;
+  double *phi0;            /* [3N] unsmoothed residual */
   /* state */
   double *z, *zvtx, *cd;
   double *h, *u, *v, *zeta, *u_mag, *wavec, *rough;
@@ -561,6 +571,10 @@ static int build(orc_t *s, const orc_case_t *c, char *err, int errlen) {
       if (e1 == i) e1 = s->ee[2 * k + 1];
       s->t2t3[3 * i + j] = e1;
     }
+  s->et2t_ptr = xcalloc(N + 1, sizeof(int));
+  memcpy(s->et2t_ptr, s->t2t_ptr, sizeof(int) * (N + 1));
+  s->et2t = xcalloc(s->t2t_ptr[N] + 1, sizeof(int));
+  memcpy(s->et2t, s->t2t, sizeof(int) * s->t2t_ptr[N]);
   if (s->o.gradient == ORC_GRAD_HDLS) return build_t2thd(s, err, errlen);
   return 0;
 }
@@ -783,7 +797,9 @@ void orc_destroy(orc_t *s) {
                   s->zeta, s->u_mag, s->wavec, s->rough, s->hvtx, s->zetavtx, s->delh, s->delu, s->delv,
                   s->delz, s->gwork, s->phi, s->dry, s->rk, s->pn, s->lin, s->hn, s->wall_pts,
                   s->wtang, s->uvtx, s->vvtx, s->cdvtx, s->e_est, s->p_zeta, s->p_cdvtx,
-                  s->p_hvtx, s->p_uvtx, s->p_vvtx, s->p_zetavtx};
+                  s->p_hvtx, s->p_uvtx, s->p_vvtx, s->p_zetavtx, s->et2t_ptr, s->et2t,
+                  s->wcoef1, s->wcoef2, s->cvin, s->cvout, s->cvtab_ptr, s->cvpin, s->cvrin,
+                  s->cvhead, s->cvq, s->cvsrc, s->phi0};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(s);
 }
@@ -1858,6 +1874,144 @@ static void start(orc_t *s) {
 }
 
 /* one time step: solverFVT.f90:132-476.  Returns 1 when the run stops. */
+/* windforcing.f90:64-86 (opt_solver 2, 3): cell values as given, vertex
+ * values averaged by vtx2ctr; wcoef = wcoef0 w |w| / rho */
+int32_t orc_set_wind(orc_t *s, int32_t n, const double *c, const double *wx, const double *wy) {
+  const int N = s->N;
+  free(s->wcoef1); free(s->wcoef2);
+  s->wcoef1 = s->wcoef2 = NULL;
+  if (n == 0) return 0;
+  if (n != N && n != s->P) return -1;
+  double *w0 = xcalloc(N, sizeof(double));
+  s->wcoef1 = xcalloc(N, sizeof(double));
+  s->wcoef2 = xcalloc(N, sizeof(double));
+  if (n == N) {
+    memcpy(w0, c, sizeof(double) * N);
+    memcpy(s->wcoef1, wx, sizeof(double) * N);
+    memcpy(s->wcoef2, wy, sizeof(double) * N);
+  } else {
+    vtx2ctr(s, c, w0);
+    vtx2ctr(s, wx, s->wcoef1);
+    vtx2ctr(s, wy, s->wcoef2);
+  }
+  for (int i = 0; i < N; ++i) {
+    double mag = sqrt(s->wcoef1[i] * s->wcoef1[i] + s->wcoef2[i] * s->wcoef2[i]);
+    s->wcoef1[i] = w0[i] * s->wcoef1[i] * mag / RHO;
+    s->wcoef2[i] = w0[i] * s->wcoef2[i] * mag / RHO;
+  }
+  free(w0);
+  return 0;
+}
+
+/* pto_in_triangle.f90:20-43 */
+static int pto_in_triangle(const orc_t *s, int e, double x, double y) {
+  const double tarea = s->area[e];
+  const int a = s->vtx[3 * e], b = s->vtx[3 * e + 1], c = s->vtx[3 * e + 2];
+  const double x1 = s->x[a], y1 = s->y[a], x2 = s->x[b], y2 = s->y[b], x3 = s->x[c], y3 = s->y[c];
+  const double sv = HALF / tarea * (y1 * x3 - x1 * y3 + (y3 - y1) * x + (x1 - x3) * y);
+  const double t = HALF / tarea * (x1 * y2 - y1 * x2 + (y1 - y2) * x + (x2 - x1) * y);
+  if (sv < ZERO) return 0;
+  if (t < ZERO) return 0;
+  if (ONE - sv - t < ZERO) return 0;
+  return 1;
+}
+
+/* ioculvert.f90's data, located by cvprep.f90:29-58: the first triangle (in
+ * cell order) holding each inlet and outlet point, and the inlet's offset
+ * from that triangle's centroid */
+int32_t orc_set_culverts(orc_t *s, int32_t n, const double *pin, const double *pout,
+                         const int32_t *ntab, const double *head, const double *q) {
+  void *old[] = {s->cvin, s->cvout, s->cvtab_ptr, s->cvpin, s->cvrin, s->cvhead, s->cvq, s->cvsrc};
+  for (size_t k = 0; k < sizeof(old) / sizeof(old[0]); ++k) free(old[k]);
+  s->cvin = s->cvout = s->cvtab_ptr = NULL;
+  s->cvpin = s->cvrin = s->cvhead = s->cvq = s->cvsrc = NULL;
+  s->ncv = 0;
+  if (n <= 0) return 0;
+  s->cvin = xcalloc(n, sizeof(int));
+  s->cvout = xcalloc(n, sizeof(int));
+  s->cvtab_ptr = xcalloc(n + 1, sizeof(int));
+  s->cvpin = xcalloc(3 * n, sizeof(double));
+  s->cvrin = xcalloc(2 * n, sizeof(double));
+  s->cvsrc = xcalloc(n, sizeof(double));
+  for (int i = 0; i < n; ++i) s->cvtab_ptr[i + 1] = s->cvtab_ptr[i] + ntab[i];
+  s->cvhead = xcalloc(s->cvtab_ptr[n] + 1, sizeof(double));
+  s->cvq = xcalloc(s->cvtab_ptr[n] + 1, sizeof(double));
+  memcpy(s->cvhead, head, sizeof(double) * s->cvtab_ptr[n]);
+  memcpy(s->cvq, q, sizeof(double) * s->cvtab_ptr[n]);
+  memcpy(s->cvpin, pin, sizeof(double) * 3 * n);
+  s->ncv = n;
+  for (int i = 0; i < n; ++i) {
+    int j, found = -1;
+    for (j = 0; j < s->N; ++j)
+      if (pto_in_triangle(s, j, pin[3 * i], pin[3 * i + 1])) { found = j; break; }
+    if (found < 0) { s->ncv = 0; return i + 1; }
+    s->cvin[i] = found;
+    s->cvrin[2 * i] = pin[3 * i] - s->xc[found];
+    s->cvrin[2 * i + 1] = pin[3 * i + 1] - s->yc[found];
+    found = -1;
+    for (j = 0; j < s->N; ++j)
+      if (pto_in_triangle(s, j, pout[2 * i], pout[2 * i + 1])) { found = j; break; }
+    if (found < 0) { s->ncv = 0; return -(i + 1); }
+    s->cvout[i] = found;
+  }
+  return 0;
+}
+
+void orc_get_culverts(const orc_t *s, int32_t *cin, int32_t *cout, double *src) {
+  for (int i = 0; i < s->ncv; ++i) {
+    if (cin) cin[i] = s->cvin[i];
+    if (cout) cout[i] = s->cvout[i];
+    if (src) src[i] = s->cvsrc[i];
+  }
+}
+
+/* cvdisch.f90:26-58: the discharge of each culvert from the water surface
+ * reconstructed at its inlet (delh + delz is the stage gradient) */
+static void cvdisch(orc_t *s) {
+  for (int i = 0; i < s->ncv; ++i) {
+    s->cvsrc[i] = ZERO;
+    const int e = s->cvin[i];
+    if (s->dry[e]) continue;
+    double head = s->zeta[e] + (s->delh[2 * e] + s->delz[2 * e]) * s->cvrin[2 * i] +
+                  (s->delh[2 * e + 1] + s->delz[2 * e + 1]) * s->cvrin[2 * i + 1];
+    head = head - s->cvpin[3 * i + 2];
+    if (head < s->o.h_dry) continue;
+    const double *hh = s->cvhead + s->cvtab_ptr[i], *qq = s->cvq + s->cvtab_ptr[i];
+    const int k = s->cvtab_ptr[i + 1] - s->cvtab_ptr[i];
+    if (head < hh[0]) {
+      s->cvsrc[i] = qq[0];
+    } else if (head >= hh[k - 1]) {
+      s->cvsrc[i] = qq[k - 1];
+    } else {
+      for (int j = 0; j < k - 1; ++j)
+        if (head < hh[j + 1]) {
+          double t = (head - hh[j]) / (hh[j + 1] - hh[j]);
+          s->cvsrc[i] = qq[j + 1] * t + qq[j] * (ONE - t);
+          break;
+        }
+    }
+  }
+}
+
+/* solverFVT.f90:238-256: Gauss-Seidel sweeps of the residual Laplacian in
+ * cell order over elemnt_db's t2t; fac1, fac2 of :59-60 */
+static void rsavg(orc_t *s) {
+  const int N = s->N;
+  const double eps = s->o.rsavg_eps;
+  const double fac1 = ONE / (ONE + eps * 3.0);
+  const double fac2 = eps / (ONE + eps * 3.0);
+  if (!s->phi0) s->phi0 = xcalloc(3 * (size_t)N, sizeof(double));
+  memcpy(s->phi0, s->phi, sizeof(double) * 3 * N);
+  for (int k = 0; k < s->o.rsavg_n; ++k)
+    for (int i = 0; i < N; ++i) {
+      for (int v = 0; v < 3; ++v) s->phi[v * N + i] = fac1 * s->phi0[v * N + i];
+      for (int m = s->et2t_ptr[i]; m < s->et2t_ptr[i + 1]; ++m) {
+        const int j = s->et2t[m];
+        for (int v = 0; v < 3; ++v) s->phi[v * N + i] = s->phi[v * N + i] + fac2 * s->phi[v * N + j];
+      }
+    }
+}
+
 static int step(orc_t *s) {
   const int N = s->N, R = s->o.rkorder;
   const double h_dry = s->o.h_dry;
@@ -1884,6 +2038,21 @@ static int step(orc_t *s) {
     free_flow(s);
     if (s->o.friction == 1) fric_terms(s);
     bed_slope_std(s);
+    if (s->wcoef1) {  /* solverFVT.f90:218-224 */
+      for (int i = 0; i < N; ++i) {
+        if (s->dry[i]) continue;
+        s->phi[N + i] = s->phi[N + i] + s->wcoef1[i];
+        s->phi[2 * N + i] = s->phi[2 * N + i] + s->wcoef2[i];
+      }
+    }
+    if (s->ncv > 0) {  /* solverFVT.f90:227-233 */
+      cvdisch(s);
+      for (int i = 0; i < s->ncv; ++i) {
+        s->phi[s->cvin[i]] = s->phi[s->cvin[i]] - s->cvsrc[i];
+        s->phi[s->cvout[i]] = s->phi[s->cvout[i]] + s->cvsrc[i];
+      }
+    }
+    if (s->o.rsavg) rsavg(s);
     /* solverFVT.f90:259-283 */
     for (int i = 0; i < N; ++i) {
       if (s->o.steady) {

@@ -56,6 +56,9 @@ typedef struct orc_opts {
   int32_t vclip;
   double cfl, rstop, h_dry, h_wet, vceiling;
   int32_t fluxes;      /* interior-edge loop: 0 = invisc_fluxes.f90, 1 = invisc_fluxes_xsf.f90 */
+  int32_t rsavg;       /* ACTIVATE RSAVG: residual smoothing (solverFVT.f90:238-256) */
+  int32_t rsavg_n;     /* nRSAVG sweeps */
+  double rsavg_eps;    /* epsRSAVG */
 } orc_opts_t;
 
 typedef struct orc orc_t;
@@ -89,6 +92,18 @@ void orc_set_state(orc_t *s, const double *h, const double *u, const double *v);
  * the new values from its next step (qin [n_in], hbc [n_out]; NULL keeps a
  * side). */
 void orc_set_bc_values(orc_t *s, const double *qin, const double *hbc);
+/* Wind forcing (windforcing.f90:64-86, FVT case): n == N values per cell or
+ * n == P per vertex (averaged to the cells by vtx2ctr); wcoef = c w |w| / rho.
+ * n = 0 turns it off.  Returns 0, or -1 for another n. */
+int32_t orc_set_wind(orc_t *s, int32_t n, const double *c, const double *wx, const double *wy);
+/* Culverts (ioculvert.f90's data, cvprep.f90's location): pin [3n] inlet x, y,
+ * invert z; pout [2n] outlet x, y; ntab [n]; head, q: the curves end to end.
+ * Returns 0, or the 1-based number of a culvert whose inlet (negative: outlet)
+ * lies in no triangle. */
+int32_t orc_set_culverts(orc_t *s, int32_t n, const double *pin, const double *pout,
+                         const int32_t *ntab, const double *head, const double *q);
+/* cvtrigin, cvtrigout [n] (0-based cells) and the last stage's cvsrc [n]. */
+void orc_get_culverts(const orc_t *s, int32_t *cin, int32_t *cout, double *src);
 /* Per-routine entry points (pinned against closed forms). */
 void orc_flux(int32_t kind, double nx, double ny, double hl, double ul, double vl,
               double hr, double ur, double vr, double cl, double cr, double h0,

@@ -41,7 +41,8 @@ class OrcOpts(C.Structure):
                 ("hvalve", C.c_int32), ("limiter", C.c_int32), ("flux", C.c_int32),
                 ("gradient", C.c_int32), ("friction", C.c_int32), ("vclip", C.c_int32),
                 ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
-                ("h_wet", C.c_double), ("vceiling", C.c_double), ("fluxes", C.c_int32)]
+                ("h_wet", C.c_double), ("vceiling", C.c_double), ("fluxes", C.c_int32),
+                ("rsavg", C.c_int32), ("rsavg_n", C.c_int32), ("rsavg_eps", C.c_double)]
 
 
 FLUXES = {"std": 0, "xsf": 1}
@@ -85,6 +86,11 @@ def lib():
         L.orc_fric_explicit.argtypes = [C.c_int32, _ip] + [_dp] * 8
         L.orc_rds.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _ip]
         L.orc_get_prep.argtypes = [C.c_void_p] + [_dp] * 9
+        L.orc_set_wind.restype = C.c_int32
+        L.orc_set_wind.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
+        L.orc_set_culverts.restype = C.c_int32
+        L.orc_set_culverts.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _ip, _dp, _dp]
+        L.orc_get_culverts.argtypes = [C.c_void_p, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -102,7 +108,9 @@ def opts_struct(o: Dict[str, object]) -> OrcOpts:
                    friction=int(o["friction"]), vclip=int(bool(o["vclip"])),
                    cfl=float(o["cfl"]), rstop=float(o["rstop"]), h_dry=h_dry,
                    h_wet=max(1.0, float(o["c_threshold"])) * h_dry,
-                   vceiling=float(o["vceiling"]), fluxes=FLUXES[str(o.get("fluxes", "std"))])
+                   vceiling=float(o["vceiling"]), fluxes=FLUXES[str(o.get("fluxes", "std"))],
+                   rsavg=int(bool(o.get("rsavg", False))), rsavg_n=int(o.get("rsavg_n", 2)),
+                   rsavg_eps=float(o.get("rsavg_eps", 0.5)))
 
 
 class Oracle:
@@ -170,6 +178,39 @@ class Oracle:
         if not self.h:
             raise RuntimeError("oracle: " + err.value.decode())
         self.N, self.P = case.n_elems, case.n_pts
+        if getattr(case, "wind", None) is not None:
+            self.set_wind(*case.wind)
+        if getattr(case, "culverts", None):
+            self.set_culverts(case.culverts)
+
+    def set_wind(self, c, wx, wy):
+        """Wind data per cell (N values) or per vertex (P values)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (c, wx, wy)]
+        self._keep["wind"] = a
+        if lib().orc_set_wind(self.h, len(a[0]), *[_p(x, C.c_double) for x in a]) != 0:
+            raise ValueError("wind data must have one value per cell or per vertex")
+
+    def set_culverts(self, culverts):
+        n = len(culverts)
+        pin = np.array([cv.inlet for cv in culverts], dtype=np.float64).reshape(-1)
+        pout = np.array([cv.outlet for cv in culverts], dtype=np.float64).reshape(-1)
+        ntab = np.array([len(cv.head) for cv in culverts], dtype=np.int32)
+        head = np.concatenate([np.asarray(cv.head, dtype=np.float64) for cv in culverts])
+        q = np.concatenate([np.asarray(cv.q, dtype=np.float64) for cv in culverts])
+        rc = lib().orc_set_culverts(self.h, n, _p(pin, C.c_double), _p(pout, C.c_double),
+                                    _p(ntab, C.c_int32), _p(head, C.c_double), _p(q, C.c_double))
+        if rc != 0:
+            raise ValueError("culvert %d: %s point in no triangle" % (abs(rc), "inlet" if rc > 0 else "outlet"))
+        self.ncv = n
+
+    def culverts(self):
+        """cvtrigin, cvtrigout (0-based cells) and the last stage's cvsrc."""
+        n = getattr(self, "ncv", 0)
+        cin = np.zeros(max(n, 1), dtype=np.int32)
+        cout = np.zeros(max(n, 1), dtype=np.int32)
+        src = np.zeros(max(n, 1))
+        lib().orc_get_culverts(self.h, _p(cin, C.c_int32), _p(cout, C.c_int32), _p(src, C.c_double))
+        return cin[:n], cout[:n], src[:n]
 
     def __del__(self):
         if getattr(self, "h", None):

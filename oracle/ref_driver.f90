@@ -4,6 +4,7 @@
 !
 ! It follows the reference's own main-program sequence for a ".run" case,
 ! Source_xsy-64bit/SToRM.f90:64-66 -> read_data.f90:80-83 (read_mfa) ->
+! read_data.f90:92 (ioculvert, when the options name a culvert file) ->
 ! solver.f90:50,57 (dataprepFVT, solverFVT), without the command-line prompt
 ! of read_data.f90:41-52 and without the final Tecplot/CGNS writers.
 !
@@ -46,6 +47,7 @@ PROGRAM ref_driver
   CALL GET_COMMAND_ARGUMENT(2, oname)
   cgns = .FALSE.
   CALL read_mfa(fname, title)
+  IF (culvert) CALL ioculvert   ! read_data.f90:92
   CALL dataprepFVT
   IF (COMMAND_ARGUMENT_COUNT() >= 3) THEN
     CALL GET_COMMAND_ARGUMENT(3, pname)

@@ -42,6 +42,17 @@ class Boundary:
 
 
 @dataclasses.dataclass
+class Culvert:
+    """One culvert as ioculvert.f90:84-131 reads it: inlet point and invert
+    elevation, outlet point, and the performance curve (head above the
+    invert -> discharge)."""
+    inlet: Tuple[float, float, float]   # x, y, invert z
+    outlet: Tuple[float, float]         # x, y
+    head: np.ndarray                    # ascending heads
+    q: np.ndarray                       # discharge at each head
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     x: np.ndarray
@@ -62,6 +73,11 @@ class Case:
     # file may hold them (dataprepFVT.f90:73-140 takes either): "z" the bed
     # at the cells (N); "h", "u", "v", "cd" values at the vertices (P).
     alt: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
+    # Wind forcing data as windforcing.f90 reads it: coefficient, x and y
+    # wind components, each per cell (N values) or per vertex (P values).
+    wind: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None
+    # Culverts (ioculvert.f90); a single reach only.
+    culverts: List[Culvert] = dataclasses.field(default_factory=list)
 
     @property
     def n_elems(self) -> int:
@@ -170,7 +186,7 @@ def default_opts(**kw) -> Dict[str, object]:
         rkorder=2, cfl=1.0, dt_op=0, steady=False, stop_cr=0, rstop=1.0e-4,
         istop=0, h_dry=0.001, c_threshold=1.0, btype=0, hvalve=True,
         limiter=2, flux="ac", gradient="lsq", friction=0, vclip=False,
-        vceiling=35.0, fluxes="std")
+        vceiling=35.0, fluxes="std", rsavg=False, rsavg_n=2, rsavg_eps=0.5)
     o.update(kw)
     return o
 
@@ -485,11 +501,72 @@ def write_run(case: Case, dirpath: str, out_prefix: str = "case") -> str:
             f.write("MAX VELOCITY %.17e\n" % float(o["vceiling"]))
         if str(o.get("gradient", "lsq")) == "hdls":
             f.write("ACTIVATE HDLS\n")
+        if o.get("rsavg"):
+            # read_options.f90:347-351 reads values after the keyword from
+            # column 13 (inside the keyword), so only the bare form runs: the
+            # defaults epsRSAVG = 0.5, nRSAVG = 2
+            if int(o.get("rsavg_n", 2)) != 2 or float(o.get("rsavg_eps", 0.5)) != 0.5:
+                raise ValueError("the reference reads only the bare ACTIVATE RSAVG (0.5, 2)")
+            f.write("ACTIVATE RSAVG\n")
+        # file names are upper case: readline.f90 upper-cases every line
+        if case.wind is not None:
+            f.write("WIND WIND.DAT\n")
+        if case.culverts:
+            f.write("CULV CULV.DAT\n")
+    if case.wind is not None:
+        write_wind(case, os.path.join(dirpath, "WIND.DAT"))
+    if case.culverts:
+        write_culverts(case, os.path.join(dirpath, "CULV.DAT"))
     run = os.path.join(dirpath, out_prefix + ".run")
     with open(run, "w") as f:
         f.write("grid.dat\nbcs.dat\n%s.out\n%s.sol\noptions.dat\n"
                 % (out_prefix, out_prefix))
     return run
+
+
+def _short(v: float) -> str:
+    s = repr(float(v))
+    if len(s) > 12:
+        raise ValueError("value %r does not fit the reference's 40-column lines" % v)
+    return s
+
+
+def write_wind(case: Case, path: str) -> None:
+    """windforcing.f90:43-63: the count, then one "c wx wy" line per cell or
+    vertex.  readline.f90 keeps 40 columns, so every value must print in at
+    most 12 characters (wind_data() rounds them so)."""
+    c, wx, wy = case.wind
+    with open(path, "w") as f:
+        f.write("%d\n" % len(c))
+        for a, b, d in zip(c, wx, wy):
+            f.write("%s %s %s\n" % (_short(a), _short(b), _short(d)))
+
+
+def write_culverts(case: Case, path: str) -> None:
+    """ioculvert.f90:84-131: the count; per culvert the inlet "x y z" line,
+    the outlet "x y" line, the number of curve points and "head q" lines."""
+    with open(path, "w") as f:
+        f.write("%d\n" % len(case.culverts))
+        for cv in case.culverts:
+            f.write("%s %s %s\n" % tuple(_short(v) for v in cv.inlet))
+            f.write("%s %s\n" % tuple(_short(v) for v in cv.outlet))
+            f.write("%d\n" % len(cv.head))
+            for hh, qq in zip(cv.head, cv.q):
+                f.write("%s %s\n" % (_short(hh), _short(qq)))
+
+
+def wind_data(rng: np.random.Generator, n: int, c: float = 1.5e-3, speed: float = 12.0,
+              direction: float = 0.6) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Random wind data for n cells or vertices: coefficient c(1 + U(-0.2,
+    0.2)), components speed(cos, sin)(direction + N(0, 0.3^2)) times U(0.5,
+    1.5); each value rounded to 4 significant digits, so that it prints in
+    the reference's 40-column lines and reads back to the same double."""
+    def r4(a):
+        return np.array([float("%.4g" % v) for v in a])
+    cc = r4(c * (1.0 + rng.uniform(-0.2, 0.2, n)))
+    ang = direction + rng.normal(0.0, 0.3, n)
+    mag = speed * rng.uniform(0.5, 1.5, n)
+    return cc, r4(mag * np.cos(ang)), r4(mag * np.sin(ang))
 
 
 def read_ref_output(path: str) -> Dict[str, np.ndarray]:

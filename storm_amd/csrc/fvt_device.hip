@@ -99,12 +99,17 @@ struct Geo {
   const int *svptr, *svtx;  // per reach, the vertices of its stage strings
   const RInfo* reach;
   int n_in, n_out, nr;  // string totals, reaches
+  const double2* wind;  // per cell wcoef1, wcoef2 (windforcing.f90), when Opts::wind
+  const int2* cv_io;    // per culvert its inlet and outlet cells (internal ids)
+  const double* cvsrc;  // per culvert the discharge of the current stage
 };
 
 struct Opts {
   int rkorder, dt_op, steady, stop_cr, btype, hvalve, limiter, flux, gradient, friction, vclip;
   int nonfinite_stop;
   int xsf;  // interior edges as invisc_fluxes_xsf.f90 forms them
+  int wind;  // wind terms on (Geo::wind)
+  int ncv;   // culverts (Geo::cv_io, cvsrc)
   double cfl, rstop, h_dry, h_wet, vceiling;
 };
 
@@ -124,6 +129,7 @@ struct State {
   double *gwork, *hvtx, *zetavtx, *phi, *part;
   uint8_t *dry_post, *vwet;
   double *sq, *sh;    // current Q (or stage) of every inflow string, stage of every outflow string
+  double *phi0, *phis;  // residual smoothing: the residual before and after it
   Ctrl* ctrl;         // one per reach
   // per reach, `done` as the first stage of the latest step saw it (that
   // stage's kernel writes it), for the next step's first vertex pass, which
@@ -1240,7 +1246,11 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 //      fixed order by k_control).
 // Three waves per SIMD (about 150 VGPRs, no spills).  Forcing four (128
 // VGPRs) spills the prefetched phase-4 inputs and runs 30 % slower.
-template <int CAP, bool XSF = false>
+// MODE 0 runs it whole.  Residual smoothing splits it around k_rsavg:
+// MODE 1 stops after the residual (fluxes and sources) of every tile cell
+// and writes it to S.phi0; MODE 2 skips phases 2-3 and the residual, reads
+// the smoothed residual from S.phis and runs the rest of phase 4.
+template <int CAP, bool XSF = false, int MODE = 0>
 __global__ void __launch_bounds__(CAP)
 k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   extern __shared__ double lds_dyn[];
@@ -1277,7 +1287,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     ci = i;
     nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
     v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
-  } else if (t < T + H) {
+  } else if (MODE != 2 && t < T + H) {
     const int4 r0 = tl.hrec[2 * (hb + t - T)], r1 = tl.hrec[2 * (hb + t - T) + 1];
     ci = r0.x; nb0 = r0.y; nb1 = r0.z; nb2 = r0.w;
     v0 = r1.x; v1 = r1.y; v2 = r1.z;
@@ -1290,19 +1300,29 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
   const Lds L = lds_layout(lds_dyn, CAP, XSF);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
-  if (t == 0 && o.steady && st.last) {
+  if (MODE != 1 && t == 0 && o.steady && st.last) {
     s_dlast = last_cell_dt(g, o, S.din, ri.last_cell);
     // k_control advances the reach's ttime by it
     if (b == ri.tile0) S.ctrl[hd1.w].delt_last = s_dlast;
   }
 
+  Faces F;
+  if constexpr (MODE == 2) {
+    // what phase 4 reads of the faces: the stage input's depth, dry flag
+    // and |u| (hgradfix2's zeroed slot values come from the slots themselves)
+    if (t < T) {
+      const CellD di = ld(&S.din[ci]);
+      F.h = di.h;
+      F.dry = di.dry != ZERO;
+      F.umag = F.dry ? ZERO : sqrt(di.u * di.u + di.v * di.v);
+    }
+  } else {
   // 2. inflow discharge strings (their buffers overlay the cell arrays)
   for (uint64_t qm = qmask; qm; qm &= qm - 1)
     bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, ri, __builtin_ctzll(qm), L, s_at, s_null,
                     s_skip);
 
   // 3. face states of the tile and halo cells
-  Faces F;
   if (t < T + H) {
     const bool own = t < T;
     const int i = ci;
@@ -1340,6 +1360,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     L.dry[t] = F.dry;
   }
   __syncthreads();
+  }  // MODE != 2
 
   // 4. update of the tile cells
   double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX, topq = ZERO, botq = ZERO;
@@ -1349,15 +1370,28 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     const uint64_t ln = pf_ln;
     const CellS si = pf_cs;
     Tri tr;
+    if constexpr (MODE == 2) {
+      const double4 q0 = g.vxyz[v0], q1 = g.vxyz[v1], q2 = g.vxyz[v2];
+      tr.x[0] = q0.x; tr.y[0] = q0.y; tr.zv[0] = q0.z;
+      tr.x[1] = q1.x; tr.y[1] = q1.y; tr.zv[1] = q1.z;
+      tr.x[2] = q2.x; tr.y[2] = q2.y; tr.zv[2] = q2.z;
+    } else {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      constexpr int TM = tile_tmax(CAP);
-      tr.x[k] = L.vx[k * TM + t]; tr.y[k] = L.vy[k * TM + t]; tr.zv[k] = L.vz[k * TM + t];
+      for (int k = 0; k < 3; ++k) {
+        constexpr int TM = tile_tmax(CAP);
+        tr.x[k] = L.vx[k * TM + t]; tr.y[k] = L.vy[k * TM + t]; tr.zv[k] = L.vz[k * TM + t];
+      }
     }
-    const double dhx = F.dhx, dhy = F.dhy, hi = F.h, umag_in = F.umag;
+    const double hi = F.h, umag_in = F.umag;
     const bool dry_i = F.dry;
-    const double zi = si.z, xci = si.xc, yci = si.yc;
+    const double zi = si.z;
     const double area = tri_area(tr);
+    double p0 = ZERO, p1 = ZERO, p2 = ZERO;
+    if constexpr (MODE == 2) {
+      p0 = S.phis[i]; p1 = S.phis[N + i]; p2 = S.phis[2 * N + i];
+    } else {
+    const double dhx = F.dhx, dhy = F.dhy;
+    const double xci = si.xc, yci = si.yc;
     auto side_flux = [&](uint32_t bits, const Side& sd, int j, double& d0, double& d1,
                          double& d2) {
       const double fho = L.fh[j * CAP + t];
@@ -1393,7 +1427,6 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     // the sides in the reference's accumulation order (invisc_fluxes:
     // interior edges by number, walls, boundary routines), each added to phi
     // as it is formed
-    double p0 = ZERO, p1 = ZERO, p2 = ZERO;
 #pragma unroll 1  // one side at a time: fewer live registers
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
@@ -1448,6 +1481,23 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       p1 = p1 - G * hi * gx * area;
       p2 = p2 - G * hi * gy * area;
     }
+    if (o.wind && !dry_i) {  // solverFVT.f90:218-224
+      const double2 w = g.wind[i];
+      p1 = p1 + w.x;
+      p2 = p2 + w.y;
+    }
+    if (o.ncv && ((meta >> CM_CULVERT) & 1)) {  // solverFVT.f90:227-233, culvert order
+      for (int m = 0; m < o.ncv; ++m) {
+        const int2 io = g.cv_io[m];
+        if (io.x == i) p0 = p0 - g.cvsrc[m];
+        if (io.y == i) p0 = p0 + g.cvsrc[m];
+      }
+    }
+    if constexpr (MODE == 1) {
+      S.phi0[i] = p0; S.phi0[N + i] = p1; S.phi0[2 * N + i] = p2;
+    }
+    }  // MODE != 2
+    if constexpr (MODE != 1) {
     const double perim = side<0>(tr).len + side<1>(tr).len + side<2>(tr).len;
     // solverFVT.f90:259-283
     double delt = c->dt;
@@ -1550,8 +1600,9 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       const double wc = dn ? ZERO : sqrt(G * hn);
       cand = o.cfl * area / ((um2 + wc + SMALL) * perim);
     }
+    }  // MODE != 1
   }
-  if (!st.last) return;
+  if (MODE == 1 || !st.last) return;
   // per-tile partials: a fixed butterfly within each wave (every lane ends
   // with the same bits), then the waves' results in wave order
   const bool q2 = o.stop_cr == 2;
@@ -1637,6 +1688,107 @@ __global__ void k_delh_input_order(Geo g, Opts o, const CellD* din, const double
   const int j = perm[i];
   dh[j] = F.dhx;
   dh[N + j] = F.dhy;
+}
+
+// ---------------------------------------------------------------- culverts
+// Culvert tables: per culvert the inlet's offset from its cell's centroid,
+// the invert elevation, and the performance curve head[tptr[m] ..
+// tptr[m+1]) -> q.
+struct CvTab {
+  const double2* rin;
+  const double* zinv;
+  const int* tptr;
+  const double *head, *q;
+};
+
+// cvdisch.f90:26-58, one thread per culvert, before the stage kernel: the
+// water surface at the inlet point reconstructed from the inlet cell's
+// stage input with that stage's depth gradient (delh after cell_grads and
+// hgradfix2, formed by the stage kernel's own code) plus the bed gradient of
+// bed_slope_VTX.f90:32-33, so delh + delz as the reference adds them.
+__global__ void k_cv_src(Geo g, Opts o, const CellD* din, const double* gwork, CvTab cv,
+                         double* cvsrc) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= o.ncv) return;
+  const int N = g.N;
+  const int i = g.cv_io[m].x;
+  const CellD di = ld(&din[i]);
+  double src = ZERO;
+  if (di.dry == ZERO) {
+    const Tri tr = load_tri(g, i);
+    Faces F;
+    cell_faces(g, o, din, gwork, i, g.cmeta[i], g.cnbr[i], g.cnbr[N + i], g.cnbr[2 * N + i], tr,
+               true, F);
+    const double area = tri_area(tr);
+    const double nzx = HALF * (tr.zv[0] * (tr.y[1] - tr.y[2]) + tr.zv[1] * (tr.y[2] - tr.y[0]) +
+                               tr.zv[2] * (tr.y[0] - tr.y[1]));
+    const double nzy = HALF * (tr.zv[0] * (tr.x[2] - tr.x[1]) + tr.zv[1] * (tr.x[0] - tr.x[2]) +
+                               tr.zv[2] * (tr.x[1] - tr.x[0]));
+    const Recip ra = recip(area, nzx);
+    const double dzx = qdiv(ra, nzx, area), dzy = qdiv(ra, nzy, area);
+    const double2 r = cv.rin[m];
+    const double zeta = di.h + g.cs[i].z;
+    double head = zeta + (F.dhx + dzx) * r.x + (F.dhy + dzy) * r.y;
+    head = head - cv.zinv[m];
+    if (!(head < o.h_dry)) {
+      const int a = cv.tptr[m], k = cv.tptr[m + 1] - a;
+      const double *hh = cv.head + a, *qq = cv.q + a;
+      if (head < hh[0]) {
+        src = qq[0];
+      } else if (head >= hh[k - 1]) {
+        src = qq[k - 1];
+      } else {
+        for (int j = 0; j < k - 1; ++j)
+          if (head < hh[j + 1]) {
+            const double tt = (head - hh[j]) / (hh[j + 1] - hh[j]);
+            src = qq[j + 1] * tt + qq[j] * (ONE - tt);
+            break;
+          }
+      }
+    }
+  }
+  cvsrc[m] = src;
+}
+
+// ---------------------------------------------------------------- residual smoothing
+// The level schedule of the Gauss-Seidel sweep (see rsavg_schedule).
+struct RsTab {
+  const int *lptr, *cell, *nptr, *nbr, *rlev;
+};
+constexpr int RS_THREADS = 256;
+
+// solverFVT.f90:238-256: nsweep sweeps of phi(i) = fac1 phi0(i) + fac2 phi(j)
+// over i's t2t row in row order, one block per reach walking its levels.
+// A level's cells read cells of lower levels (already written in this
+// sweep) and of higher levels (still holding the previous sweep, or phi0 in
+// the first), so one barrier per level gives the serial sweep's values.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rsavg(RsTab rs, const Ctrl* ctrl, int N, int nsweep, double fac1, double fac2,
+        const double* phi0, double* phis) {
+  const int r = blockIdx.x;
+  if (ctrl[r].done) return;
+  const int l0 = rs.rlev[r], l1 = rs.rlev[r + 1];
+  for (int sw = 0; sw < nsweep; ++sw) {
+    const bool first = sw == 0;
+    for (int l = l0; l < l1; ++l) {
+      const int k1 = rs.lptr[l + 1];
+      for (int k = rs.lptr[l] + (int)threadIdx.x; k < k1; k += RS_THREADS) {
+        const int i = rs.cell[k];
+        double q0 = fac1 * phi0[i], q1 = fac1 * phi0[N + i], q2 = fac1 * phi0[2 * N + i];
+        const int m1 = rs.nptr[k + 1];
+        for (int m = rs.nptr[k]; m < m1; ++m) {
+          const int e = rs.nbr[m];
+          const int j = e & 0x7fffffff;
+          const double* src = (first && e < 0) ? phi0 : phis;
+          q0 = q0 + fac2 * src[j];
+          q1 = q1 + fac2 * src[N + j];
+          q2 = q2 + fac2 * src[2 * N + j];
+        }
+        phis[i] = q0; phis[N + i] = q1; phis[2 * N + i] = q2;
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // ---------------------------------------------------------------- output fields
@@ -1991,6 +2143,20 @@ struct storm_fvt {
   // the node routines' tables, uploaded at their first call
   int *d_vperm = nullptr, *d_nn_ptr = nullptr, *d_nn_idx = nullptr;
   double* scratch_ptr = nullptr;   // this device's copy of `scratch`
+  uint64_t* d_cmeta = nullptr;     // g.cmeta (culvert flags are set in it)
+  // wind terms (storm_fvt_set_wind): per cell wcoef1, wcoef2
+  double2* d_wind = nullptr;
+  // culverts (storm_fvt_set_culverts): their input cells, device tables and
+  // the buffers that hold them (replaced by the next call)
+  std::vector<int> cv_in, cv_out;
+  CvTab cv{};
+  double* d_cvsrc = nullptr;
+  std::vector<void*> cv_owned;
+  // residual smoothing (opts rsavg): level schedule, factors, sweeps
+  bool rsavg = false;
+  RsTab rs{};
+  double rs_fac1 = 0.0, rs_fac2 = 0.0;
+  int rs_n = 0;
   Tiles tl{};
   int cap = 0;
   std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
@@ -2014,6 +2180,7 @@ struct storm_fvt {
     if (stream) (void)hipStreamSynchronize(stream);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (void* p : owned) (void)hipFree(p);
+    for (void* p : cv_owned) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -2105,7 +2272,8 @@ void setup_device(storm_fvt* C) {
   Geo& g = C->g;
   g.N = P.N; g.P = P.P; g.R = P.R;
   g.cvtx = dupload(own, P.cvtx); g.cnbr = dupload(own, P.cnbr);
-  g.cmeta = dupload(own, P.cmeta);
+  C->d_cmeta = dupload(own, P.cmeta);
+  g.cmeta = C->d_cmeta;
   {
     std::vector<CellS> cs(P.N);
     for (int i = 0; i < P.N; ++i) cs[i] = CellS{P.xc[i], P.yc[i], P.z[i], P.cd[i]};
@@ -2134,6 +2302,8 @@ void setup_device(storm_fvt* C) {
   o.gradient = q.gradient; o.friction = q.friction; o.vclip = q.vclip;
   o.nonfinite_stop = q.nonfinite_stop;
   o.xsf = q.fluxes == 1;
+  o.wind = 0;
+  o.ncv = 0;
   o.cfl = q.cfl; o.rstop = q.rstop; o.h_dry = q.h_dry; o.h_wet = q.h_wet; o.vceiling = q.vceiling;
   const size_t N = P.N;
   const int R = P.R;
@@ -2146,6 +2316,8 @@ void setup_device(storm_fvt* C) {
     throw StormErr(STORM_E_ARG, "block (tile capacity) must be 256, 384 or 512");
   if (o.xsf && C->cap != 256)
     throw StormErr(STORM_E_ARG, "the xsf edge loop runs with tile capacity 256");
+  if (q.rsavg && C->cap != 256)
+    throw StormErr(STORM_E_ARG, "residual smoothing runs with tile capacity 256");
   if (build_tiles(P, C->cap) != STORM_OK) throw StormErr(STORM_E_MESH, P.err);
   Tiles& tl = C->tl;
   tl.hdr = reinterpret_cast<const int4*>(dupload(own, P.t_hdr));
@@ -2185,6 +2357,22 @@ void setup_device(storm_fvt* C) {
   S.sh = dalloc<double>(own, P.n_out);
   C->d_h0 = dupload(own, P.h0); C->d_u0 = dupload(own, P.u0); C->d_v0 = dupload(own, P.v0);
   C->d_perm = dupload(own, P.perm);
+  if (q.rsavg) {
+    // solverFVT.f90:59-60 with epsRSAVG as read_options.f90:351 leaves it
+    const double eps = std::min(std::fabs(q.rsavg_eps), 1.0);
+    C->rsavg = true;
+    C->rs_fac1 = 1.0 / (1.0 + eps * 3.0);
+    C->rs_fac2 = eps / (1.0 + eps * 3.0);
+    C->rs_n = q.rsavg_n;
+    RsTab& rs = C->rs;
+    rs.lptr = dupload(own, P.rs_lptr);
+    rs.cell = dupload(own, P.rs_cell);
+    rs.nptr = dupload(own, P.rs_nptr);
+    rs.nbr = dupload(own, P.rs_nbr);
+    rs.rlev = dupload(own, P.r_lev);
+    S.phi0 = dalloc<double>(own, 3 * N);
+    S.phis = (C->rs_n > 0) ? dalloc<double>(own, 3 * N) : S.phi0;
+  }
   {
     if (C->device < 0 || C->device >= MAX_DEVICES)
       throw StormErr(STORM_E_DEVICE, "device index beyond the scratch table");
@@ -2218,10 +2406,22 @@ void launch_init(storm_fvt* C) {
                      C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.zeta, C->S.part);
 }
 
-template <int CAP, bool XSF = false>
+template <int CAP, bool XSF = false, int MODE = 0>
 void launch_stage(storm_fvt* C, const Stage& st) {
-  hipLaunchKernelGGL((k_stage<CAP, XSF>), dim3(C->tl.nt), dim3(CAP), lds_bytes(CAP, XSF),
-                     C->stream, C->g, C->o, C->tl, st, C->S);
+  hipLaunchKernelGGL((k_stage<CAP, XSF, MODE>), dim3(C->tl.nt), dim3(CAP),
+                     MODE == 2 ? 0 : lds_bytes(CAP, XSF), C->stream, C->g, C->o, C->tl, st, C->S);
+}
+
+// The stage with residual smoothing: the residual of every cell, the sweeps,
+// then the update from the smoothed residual.
+void launch_smoothed_stage(storm_fvt* C, const Stage& st) {
+  if (C->o.xsf) launch_stage<256, true, 1>(C, st);
+  else launch_stage<256, false, 1>(C, st);
+  if (C->rs_n > 0)
+    hipLaunchKernelGGL(k_rsavg, dim3(C->prep.nr), dim3(RS_THREADS), 0, C->stream, C->rs,
+                       C->S.ctrl, C->prep.N, C->rs_n, C->rs_fac1, C->rs_fac2, C->S.phi0,
+                       C->S.phis);
+  launch_stage<256, false, 2>(C, st);
 }
 
 void start_run(storm_fvt* C) {
@@ -2314,7 +2514,11 @@ void launch_step(storm_fvt* C, bool launch = true) {
       if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
         hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
                            C->S.din, C->S.gwork);
-      if (C->o.xsf) launch_stage<256, true>(C, st);
+      if (C->o.ncv)
+        hipLaunchKernelGGL(k_cv_src, dim3((C->o.ncv + 63) / 64), dim3(64), 0, C->stream, C->g,
+                           C->o, C->S.din, C->S.gwork, C->cv, C->d_cvsrc);
+      if (C->rsavg) launch_smoothed_stage(C, st);
+      else if (C->o.xsf) launch_stage<256, true>(C, st);
       else switch (C->cap) {
         case 256: launch_stage<256>(C, st); break;
         case 384: launch_stage<384>(C, st); break;
@@ -2401,6 +2605,7 @@ void storm_fvt_default_opts(storm_fvt_opts_t* o) {
   o->friction = 0; o->vclip = 0;
   o->cfl = 1.0; o->rstop = (double)1.0e-4f; o->h_dry = 0.001; o->h_wet = 0.001; o->vceiling = 35.0;
   o->reorder = 1; o->block = 0; o->graph = 0;
+  o->rsavg = 0; o->rsavg_n = 2; o->rsavg_eps = 0.5;
 }
 
 static int create_ctx(const storm_mesh_t* mesh, const storm_fvt_opts_t* opts,
@@ -2999,6 +3204,125 @@ int storm_fvt_fvsources(storm_fvt_t* C, const double* z, const double* h, double
     return set_err(C, e.code, e.msg);
   } catch (const std::bad_alloc&) {
     return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
+  return STORM_OK;
+}
+
+// A captured step graph holds the kernel arguments of its capture: the
+// source-term tables set after it need a new capture.
+static void drop_graph(storm_fvt* C) {
+  if (C->gexec) {
+    (void)hipGraphExecDestroy(C->gexec);
+    C->gexec = nullptr;
+  }
+}
+
+int storm_fvt_set_wind(storm_fvt_t* C, int64_t n, const double* c, const double* wx,
+                       const double* wy) {
+  if (!C) return STORM_E_ARG;
+  if (n < 0) return set_err(C, STORM_E_ARG, "negative wind data count");
+  try {
+    const Prep& P = C->prep;
+    HIPCHK(hipStreamSynchronize(C->stream));  // a step may still read the old values
+    drop_graph(C);
+    if (n == 0) {
+      C->o.wind = 0;
+      return STORM_OK;
+    }
+    std::vector<double> w;
+    std::string err;
+    const int rc = wind_coefs(P, n, c, wx, wy, w, err);
+    if (rc != STORM_OK) return set_err(C, rc, err);
+    if (!C->d_wind) C->d_wind = reinterpret_cast<double2*>(dalloc<double>(C->owned, 2 * (size_t)P.N));
+    HIPCHK(hipMemcpy(C->d_wind, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    C->g.wind = C->d_wind;
+    C->o.wind = 1;
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_set_culverts(storm_fvt_t* C, int32_t n, const double* pin, const double* pout,
+                           const int32_t* ntab, const double* head, const double* q) {
+  if (!C) return STORM_E_ARG;
+  if (n < 0) return set_err(C, STORM_E_ARG, "negative culvert count");
+  if (n > 0 && (!pin || !pout || !ntab || !head || !q))
+    return set_err(C, STORM_E_ARG, "null culvert array");
+  try {
+    Prep& P = C->prep;
+    if (n > 0 && P.nr > 1)
+      return set_err(C, STORM_E_ARG, "culverts need a context of one reach");
+    std::vector<int> tptr(n + 1, 0);
+    for (int m = 0; m < n; ++m) {
+      if (ntab[m] < 1) return set_err(C, STORM_E_ARG, "a culvert curve needs at least one point");
+      tptr[m + 1] = tptr[m] + ntab[m];
+    }
+    std::vector<int> cin, cout;
+    std::vector<double> rin;
+    std::string err;
+    if (n > 0) {
+      const int rc = locate_culverts(P, n, pin, pout, cin, cout, rin, err);
+      if (rc != STORM_OK) return set_err(C, rc, err);
+    }
+    HIPCHK(hipStreamSynchronize(C->stream));  // a step may still read the old tables
+    drop_graph(C);
+    for (void* b : C->cv_owned) (void)hipFree(b);
+    C->cv_owned.clear();
+    C->cv = CvTab{};
+    C->d_cvsrc = nullptr;
+    C->g.cv_io = nullptr;
+    C->g.cvsrc = nullptr;
+    C->o.ncv = 0;
+    const uint64_t flag = 1ull << CM_CULVERT;
+    for (int i = 0; i < P.N; ++i) P.cmeta[i] &= ~flag;
+    std::vector<int> io(2 * (size_t)n);
+    std::vector<double> zinv(n);
+    for (int m = 0; m < n; ++m) {
+      io[2 * m] = P.iperm[cin[m]];
+      io[2 * m + 1] = P.iperm[cout[m]];
+      P.cmeta[io[2 * m]] |= flag;
+      P.cmeta[io[2 * m + 1]] |= flag;
+      zinv[m] = pin[3 * m + 2];
+    }
+    HIPCHK(hipMemcpy(C->d_cmeta, P.cmeta.data(), sizeof(uint64_t) * P.N, hipMemcpyHostToDevice));
+    C->cv_in = cin;
+    C->cv_out = cout;
+    if (n == 0) return STORM_OK;
+    auto& own = C->cv_owned;
+    C->g.cv_io = reinterpret_cast<const int2*>(dupload(own, io));
+    C->cv.rin = reinterpret_cast<const double2*>(dupload(own, rin));
+    C->cv.zinv = dupload(own, zinv);
+    C->cv.tptr = dupload(own, tptr);
+    C->cv.head = dupload(own, std::vector<double>(head, head + tptr[n]));
+    C->cv.q = dupload(own, std::vector<double>(q, q + tptr[n]));
+    C->d_cvsrc = dalloc<double>(own, n);
+    C->g.cvsrc = C->d_cvsrc;
+    C->o.ncv = n;
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(C, STORM_E_NOMEM, "out of host memory");
+  }
+  return STORM_OK;
+}
+
+int storm_fvt_get_culverts(storm_fvt_t* C, int32_t* cell_in, int32_t* cell_out, double* src) {
+  if (!C) return STORM_E_ARG;
+  try {
+    const int n = C->o.ncv;
+    for (int m = 0; m < n; ++m) {
+      if (cell_in) cell_in[m] = C->cv_in[m];
+      if (cell_out) cell_out[m] = C->cv_out[m];
+    }
+    if (src && n > 0) {
+      HIPCHK(hipStreamSynchronize(C->stream));
+      HIPCHK(hipMemcpy(src, C->d_cvsrc, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+  } catch (const StormErr& e) {
+    return set_err(C, e.code, e.msg);
   }
   return STORM_OK;
 }

@@ -256,13 +256,143 @@ int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int
   return molecule(m->n_elems, (int)(ep.size() / 2), ee, ptr, cells, err);
 }
 
+namespace {
+
+// solverFVT.f90:244-255 sweeps the cells in input order; cell i takes the
+// new values of its t2t neighbours j < i and the old values of those j > i.
+// A cell's level is one above every cell it must follow: each j < i of its
+// row (read new) and each j < i whose row names i (it reads i's old value,
+// so it runs first).  The cells of one level read and write disjoint values,
+// so a level-by-level sweep gives the serial sweep's result bit for bit.
+// Rows never cross reaches, so every reach's levels start at 0.
+void rsavg_schedule(Prep& S, const std::vector<int>& tp, const std::vector<int>& tt,
+                    const std::vector<int>& cell_ptr) {
+  const int N = S.N, NR = (int)cell_ptr.size() - 1;
+  std::vector<int> lev(N, 0);
+  for (int i = 0; i < N; ++i) {
+    int l = lev[i];
+    for (int t = tp[i]; t < tp[i + 1]; ++t)
+      if (tt[t] < i) l = std::max(l, lev[tt[t]] + 1);
+    lev[i] = l;
+    for (int t = tp[i]; t < tp[i + 1]; ++t)
+      if (tt[t] > i) lev[tt[t]] = std::max(lev[tt[t]], l + 1);
+  }
+  S.r_lev.assign(NR + 1, 0);
+  for (int r = 0; r < NR; ++r) {
+    int top = 0;
+    for (int i = cell_ptr[r]; i < cell_ptr[r + 1]; ++i) top = std::max(top, lev[i] + 1);
+    S.r_lev[r + 1] = S.r_lev[r] + top;
+  }
+  std::vector<int> reach_of(N);
+  for (int r = 0; r < NR; ++r)
+    for (int i = cell_ptr[r]; i < cell_ptr[r + 1]; ++i) reach_of[i] = r;
+  const int L = S.r_lev[NR];
+  S.rs_lptr.assign(L + 1, 0);
+  for (int i = 0; i < N; ++i) ++S.rs_lptr[S.r_lev[reach_of[i]] + lev[i] + 1];
+  for (int l = 0; l < L; ++l) S.rs_lptr[l + 1] += S.rs_lptr[l];
+  std::vector<int> fill(S.rs_lptr.begin(), S.rs_lptr.end() - 1);
+  S.rs_cell.assign(N, 0);
+  for (int ii = 0; ii < N; ++ii) {  // ascending internal ids within a level
+    const int i = S.perm[ii];
+    S.rs_cell[fill[S.r_lev[reach_of[i]] + lev[i]]++] = ii;
+  }
+  S.rs_nptr.assign(N + 1, 0);
+  S.rs_nbr.clear();
+  for (int k = 0; k < N; ++k) {
+    const int i = S.perm[S.rs_cell[k]];
+    for (int t = tp[i]; t < tp[i + 1]; ++t) {
+      const int j = tt[t];
+      S.rs_nbr.push_back(S.iperm[j] | (j > i ? (int)0x80000000u : 0));
+    }
+    S.rs_nptr[k + 1] = (int)S.rs_nbr.size();
+  }
+}
+
+}  // namespace
+
+int wind_coefs(const Prep& S, int64_t n, const double* c, const double* wx, const double* wy,
+               std::vector<double>& w, std::string& err) {
+  const int N = S.N;
+  if (n != S.N && n != S.P) {
+    err = "wind data must hold one value per cell or one per vertex";
+    return STORM_E_ARG;
+  }
+  if (!c || !wx || !wy) { err = "null wind array"; return STORM_E_ARG; }
+  constexpr double RHO = 998.60;  // constants.f90:29
+  w.assign(2 * (size_t)N, ZERO);
+  for (int ii = 0; ii < N; ++ii) {
+    const int i = S.perm[ii];
+    double w0, w1, w2;
+    if (n == S.N) {
+      w0 = c[i]; w1 = wx[i]; w2 = wy[i];
+    } else {  // vtx2ctr.f90:32-35 on the triangle's vertices in input order
+      const int a = S.vperm[S.cvtx[ii]], b = S.vperm[S.cvtx[N + ii]],
+                d = S.vperm[S.cvtx[2 * (size_t)N + ii]];
+      w0 = (c[a] + c[b] + c[d]) * ONE_THIRD;
+      w1 = (wx[a] + wx[b] + wx[d]) * ONE_THIRD;
+      w2 = (wy[a] + wy[b] + wy[d]) * ONE_THIRD;
+    }
+    const double mag = std::sqrt(w1 * w1 + w2 * w2);
+    w[2 * (size_t)ii] = w0 * w1 * mag / RHO;
+    w[2 * (size_t)ii + 1] = w0 * w2 * mag / RHO;
+  }
+  return STORM_OK;
+}
+
+int locate_culverts(const Prep& S, int n, const double* pin, const double* pout,
+                    std::vector<int>& cin, std::vector<int>& cout, std::vector<double>& rin,
+                    std::string& err) {
+  const int N = S.N;
+  cin.assign(n, -1); cout.assign(n, -1); rin.assign(2 * (size_t)n, ZERO);
+  // t_areas.f90:40-41 and pto_in_triangle.f90:20-43, cell by cell in input order
+  auto inside = [&](int ii, double x, double y) {
+    double vx[3], vy[3];
+    for (int k = 0; k < 3; ++k) {
+      const int vp = S.cvtx[(size_t)k * N + ii];
+      vx[k] = S.vxyz[4 * (size_t)vp]; vy[k] = S.vxyz[4 * (size_t)vp + 1];
+    }
+    const double tarea = HALF * std::fabs((vx[0] - vx[1]) * (vy[0] + vy[1]) +
+                                          (vx[1] - vx[2]) * (vy[1] + vy[2]) +
+                                          (vx[2] - vx[0]) * (vy[2] + vy[0]));
+    const double x1 = vx[0], y1 = vy[0], x2 = vx[1], y2 = vy[1], x3 = vx[2], y3 = vy[2];
+    const double s = HALF / tarea * (y1 * x3 - x1 * y3 + (y3 - y1) * x + (x1 - x3) * y);
+    const double t = HALF / tarea * (x1 * y2 - y1 * x2 + (y1 - y2) * x + (x2 - x1) * y);
+    if (s < ZERO) return false;
+    if (t < ZERO) return false;
+    if (ONE - s - t < ZERO) return false;
+    return true;
+  };
+  auto find = [&](double x, double y) {
+    for (int j = 0; j < N; ++j)
+      if (inside(S.iperm[j], x, y)) return j;
+    return -1;
+  };
+  for (int m = 0; m < n; ++m) {
+    cin[m] = find(pin[3 * m], pin[3 * m + 1]);
+    if (cin[m] < 0) {
+      err = "inlet of culvert " + std::to_string(m + 1) + " not in computational domain";
+      return STORM_E_ARG;
+    }
+    const int ii = S.iperm[cin[m]];
+    rin[2 * m] = pin[3 * m] - S.xc[ii];
+    rin[2 * m + 1] = pin[3 * m + 1] - S.yc[ii];
+    cout[m] = find(pout[2 * m], pout[2 * m + 1]);
+    if (cout[m] < 0) {
+      err = "outlet of culvert " + std::to_string(m + 1) + " not in computational domain";
+      return STORM_E_ARG;
+    }
+  }
+  return STORM_OK;
+}
+
 int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const ReachSplit* split) {
   if (!m || !o) { S.err = "null mesh or options"; return STORM_E_ARG; }
   if (o->rkorder < 1 || o->rkorder > 3) { S.err = "Incorrect Runge-Kutta order"; return STORM_E_ARG; }
   if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 2 || o->limiter < 0 ||
       o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
       o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2 || o->fluxes < 0 ||
-      o->fluxes > 1) {
+      o->fluxes > 1 || o->rsavg < 0 || o->rsavg > 1 || (o->rsavg && o->rsavg_n < 0) ||
+      (o->rsavg && !(std::fabs(o->rsavg_eps) <= 1e300))) {
     S.err = "invalid option value"; return STORM_E_ARG;
   }
   // invisc_fluxes_xsf.f90 calls flux_ac only
@@ -943,6 +1073,18 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
       S.r_svtx.insert(S.r_svtx.end(), sv[r].begin(), sv[r].end());
       S.r_svptr.push_back((int)S.r_svtx.size());
     }
+  }
+  S.rs_lptr.clear(); S.rs_cell.clear(); S.rs_nptr.clear(); S.rs_nbr.clear(); S.r_lev.clear();
+  if (o->rsavg) {
+    // the smoothing reads elemnt_db's t2t whatever the gradients use
+    std::vector<int> ep2, et2;
+    if (o->gradient == STORM_GRAD_HDLS) {
+      rcode = molecule(N, E, ee, ep2, et2, S.err);
+      if (rcode) return rcode;
+    }
+    const std::vector<int>& rp = (o->gradient == STORM_GRAD_HDLS) ? ep2 : t2t_ptr;
+    const std::vector<int>& rt = (o->gradient == STORM_GRAD_HDLS) ? et2 : t2t;
+    rsavg_schedule(S, rp, rt, split->cell_ptr);
   }
   return STORM_OK;
 }

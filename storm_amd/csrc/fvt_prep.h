@@ -37,6 +37,8 @@ enum SlotKind : uint32_t {
 constexpr int CM_SLOT0 = 6, CM_SLOT_BITS = 18;
 constexpr uint64_t CMS_KIND = 0xF;
 constexpr int CMS_NL = 4, CMS_LEFT = 6, CMS_HBAR = 7, CMS_REV = 8, CMS_CHAIN = 9;
+// bit 60: the cell is the inlet or outlet of a culvert
+constexpr int CM_CULVERT = 60;
 
 constexpr int MAX_STRINGS = 64;  // boundary strings of each kind per reach
 // Most cells a tile of capacity cap (cells + halo) owns: the whole capacity,
@@ -128,6 +130,13 @@ struct Prep {
   // and its metadata word
   std::vector<int> t_hdr, t_hrec;
   std::vector<uint64_t> t_hmeta;
+  // Residual smoothing (opts rsavg): the Gauss-Seidel sweep of
+  // solverFVT.f90:244-255 as levels of independent cells.  Level l holds
+  // rs_cell[rs_lptr[l] .. rs_lptr[l+1]) (internal ids, ascending); reach r
+  // owns levels [r_lev[r], r_lev[r+1]).  Position k's t2t row (elemnt_db
+  // order) is rs_nbr[rs_nptr[k] .. rs_nptr[k+1]), internal ids with bit 31
+  // set where the neighbour comes later in input order (its old value).
+  std::vector<int> rs_lptr, rs_cell, rs_nptr, rs_nbr, r_lev;
   std::string err;
 };
 
@@ -147,6 +156,19 @@ int build_tiles(Prep& P, int cap);
 // ctr2vtx type 3 (ctr2vtx.f90:73-80 with c2v_setup.f90:106-119).
 void dataprep_fields(const Prep& P, double* z, double* zvtx, double* zeta, double* cd,
                      double* cdvtx, double* hvtx, double* uvtx, double* vvtx, double* zetavtx);
+
+// windforcing.f90:64-86 (opt_solver 2, 3): per cell (internal order) the
+// pair wcoef0 w |w| / rho (x then y), from n == N cell values (input order)
+// or n == P vertex values averaged by vtx2ctr.f90:32-35.
+int wind_coefs(const Prep& P, int64_t n, const double* c, const double* wx, const double* wy,
+               std::vector<double>& w, std::string& err);
+
+// cvprep.f90:29-58 with pto_in_triangle.f90:20-43: per culvert the first
+// cell in input order holding the inlet (and the outlet) point, as input
+// ids, and the inlet's offset from that cell's centroid (x, y).
+int locate_culverts(const Prep& P, int n, const double* pin, const double* pout,
+                    std::vector<int>& cin, std::vector<int>& cout, std::vector<double>& rin,
+                    std::string& err);
 
 // Least-squares molecule only (elemnt_db.f90:42-128), input numbering.
 int build_molecule(const storm_mesh_t* m, std::vector<int>& ptr, std::vector<int>& cells,

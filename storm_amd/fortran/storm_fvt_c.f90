@@ -24,6 +24,8 @@ MODULE storm_fvt_c
                           limiter, flux, gradient, friction, vclip
     REAL(C_DOUBLE) :: cfl, rstop, h_dry, h_wet, vceiling
     INTEGER(C_INT32_T) :: reorder, block, graph, nonfinite_stop, fluxes
+    INTEGER(C_INT32_T) :: rsavg, rsavg_n
+    REAL(C_DOUBLE) :: rsavg_eps
   END TYPE storm_fvt_opts_t
 
   INTERFACE
@@ -105,6 +107,25 @@ MODULE storm_fvt_c
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END SUBROUTINE
+
+    INTEGER(C_INT) FUNCTION storm_fvt_set_wind(ctx, n, c, wx, wy) BIND(C, name="storm_fvt_set_wind")
+      IMPORT :: C_INT, C_PTR, C_INT64_T
+      TYPE(C_PTR), VALUE :: ctx, c, wx, wy
+      INTEGER(C_INT64_T), VALUE :: n
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION storm_fvt_set_culverts(ctx, n, pin, pout, ntab, head, q) &
+        BIND(C, name="storm_fvt_set_culverts")
+      IMPORT :: C_INT, C_PTR, C_INT32_T
+      TYPE(C_PTR), VALUE :: ctx, pin, pout, ntab, head, q
+      INTEGER(C_INT32_T), VALUE :: n
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION storm_fvt_get_culverts(ctx, cell_in, cell_out, src) &
+        BIND(C, name="storm_fvt_get_culverts")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, cell_in, cell_out, src
+    END FUNCTION
   END INTERFACE
 
 CONTAINS

@@ -42,11 +42,11 @@ SUBROUTINE dataprepFVT
   INTEGER(C_INT64_T) :: nw
   INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: wptr(:), wcell(:)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: ww(:), wid(:), pc(:,:), pp(:,:)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: wc(:), wx(:), wy(:), cvh(:), cvqq(:), cvin(:), cvout(:)
+  INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: cvn(:), cvc1(:), cvc2(:)
 
   ! Options the GPU path implements (everything else stops the run).
   IF (opt_solver /= 2) CALL byebye('storm_fvt: only the FVT solver is provided.')
-  IF (activateRSAVG .OR. wind_terms .OR. culvert) &
-    CALL byebye('storm_fvt: RSAVG, wind and culvert options are not provided.')
   IF (opt_friction /= 0) CALL byebye('storm_fvt: only Manning friction is provided.')
   IF (nctr2vtx /= 3) CALL byebye('storm_fvt: only inverse-distance vertex weights.')
 
@@ -159,9 +159,49 @@ SUBROUTINE dataprepFVT
   o%gradient = MERGE(2, 0, activateHDLS)  ! STORM_GRAD_HDLS or STORM_GRAD_LSQ
   o%vclip = MERGE(1, 0, vclip)
   o%cfl = cfl; o%rstop = rstop; o%h_dry = h_dry; o%h_wet = h_wet; o%vceiling = vceiling
+  o%rsavg = MERGE(1, 0, activateRSAVG)
+  o%rsavg_n = nRSAVG
+  o%rsavg_eps = epsRSAVG
 
   cstat = storm_fvt_create(m, o, ctx)
   IF (cstat /= STORM_OK) CALL byebye('storm_fvt_create failed (mesh, options or device).')
+
+  ! Wind data (dataprepFVT.f90:237-243; the file format of windforcing.f90:
+  ! the count, then "c wx wy" per cell or per vertex).
+  IF (wind_terms) THEN
+    CALL read_wind(TRIM(wind_file), wc, wx, wy)
+    cstat = storm_fvt_set_wind(ctx, INT(SIZE(wc), C_INT64_T), C_LOC(wc), C_LOC(wx), C_LOC(wy))
+    IF (cstat /= STORM_OK) THEN
+      PRINT *, TRIM(storm_fvt_message(ctx))
+      CALL byebye('storm_fvt: invalid wind data.')
+    END IF
+  END IF
+
+  ! Culverts (dataprepFVT.f90:309): ioculvert's data, located by the library.
+  IF (culvert) THEN
+    ALLOCATE(cvin(3*nculvert), cvout(2*nculvert), cvn(nculvert), cvc1(nculvert), &
+             cvc2(nculvert), cvh(SUM(ncvtable(1:nculvert))), cvqq(SUM(ncvtable(1:nculvert))))
+    k = 0
+    DO i = 1, nculvert
+      cvin(3*i-2:3*i) = cvptoin(1:3,i)
+      cvout(2*i-1:2*i) = cvptoout(1:2,i)
+      cvn(i) = ncvtable(i)
+      DO j = 1, ncvtable(i)
+        k = k + 1
+        cvh(k) = cvhead(j,i)
+        cvqq(k) = cvq(j,i)
+      END DO
+    END DO
+    cstat = storm_fvt_set_culverts(ctx, INT(nculvert, C_INT32_T), C_LOC(cvin), C_LOC(cvout), &
+                                   C_LOC(cvn), C_LOC(cvh), C_LOC(cvqq))
+    IF (cstat == STORM_OK) cstat = storm_fvt_get_culverts(ctx, C_LOC(cvc1), C_LOC(cvc2), C_NULL_PTR)
+    IF (cstat /= STORM_OK) THEN
+      PRINT *, TRIM(storm_fvt_message(ctx))
+      CALL byebye('storm_fvt: culvert set-up failed.')
+    END IF
+    cvtrigin(1:nculvert) = cvc1 + 1
+    cvtrigout(1:nculvert) = cvc2 + 1
+  END IF
 
   ! Module copies of the cell and vertex data (dataprepFVT.f90:69-140) and
   ! the centre-to-vertex weights of c2v_setup (type 3), as the library formed
@@ -209,6 +249,42 @@ SUBROUTINE dataprepFVT
   DEALLOCATE(z_tcp, h_tcp, u_tcp, v_tcp, cd_tcp)
   datav = .FALSE.
   datav(1) = .TRUE.; datav(2) = .TRUE.; datav(3) = .TRUE.
+
+CONTAINS
+
+  ! The wind file: its count, then one "c wx wy" line per value; blank lines
+  ! and lines starting with '#' are skipped (readline.f90's rules).
+  SUBROUTINE read_wind(fname, c, a, b)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    REAL(C_DOUBLE), ALLOCATABLE, INTENT(OUT) :: c(:), a(:), b(:)
+    CHARACTER(LEN=256) :: line
+    INTEGER :: u, ios, nv, iv
+    OPEN (NEWUNIT=u, FILE=fname, STATUS='OLD', ACTION='READ', IOSTAT=ios)
+    IF (ios /= 0) CALL byebye('storm_fvt: unable to open the wind file.')
+    CALL next_line(u, line)
+    READ (line, *, IOSTAT=ios) nv
+    IF (ios /= 0 .OR. nv < 1) CALL byebye('storm_fvt: invalid wind file.')
+    ALLOCATE(c(nv), a(nv), b(nv))
+    DO iv = 1, nv
+      CALL next_line(u, line)
+      READ (line, *, IOSTAT=ios) c(iv), a(iv), b(iv)
+      IF (ios /= 0) CALL byebye('storm_fvt: invalid wind file.')
+    END DO
+    CLOSE (u)
+  END SUBROUTINE read_wind
+
+  SUBROUTINE next_line(u, line)
+    INTEGER, INTENT(IN) :: u
+    CHARACTER(LEN=*), INTENT(OUT) :: line
+    INTEGER :: ios
+    DO
+      READ (u, '(A)', IOSTAT=ios) line
+      IF (ios /= 0) CALL byebye('storm_fvt: the wind file ends early.')
+      line = ADJUSTL(line)
+      IF (line(1:1) == '#' .OR. LEN_TRIM(line) == 0) CYCLE
+      RETURN
+    END DO
+  END SUBROUTINE next_line
 END SUBROUTINE dataprepFVT
 
 
@@ -232,7 +308,7 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   REAL(C_DOUBLE) :: tt, rm
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: hh(:), uu(:), vv(:), ph(:), qh(:), qu(:), qv(:), &
                                          hv(:), zv(:), uv(:), vv2(:), cv(:), ee(:), gx(:), &
-                                         gy(:), qb(:), hb(:)
+                                         gy(:), qb(:), hb(:), cs(:)
   INTEGER(C_INT8_T), ALLOCATABLE, TARGET :: dr(:)
   INTEGER(C_INT) :: cstat
   INTEGER :: i, k
@@ -259,6 +335,11 @@ SUBROUTINE solverFVT(iter, ttime, rmax, fn, title)
   IF (cstat == STORM_OK) cstat = storm_fvt_get_output(ctx, C_LOC(cv), C_LOC(ee))
   IF (cstat == STORM_OK) cstat = storm_fvt_get_gradients(ctx, C_LOC(gx), C_LOC(gy))
   IF (cstat == STORM_OK) cstat = storm_fvt_get_bc_values(ctx, C_LOC(qb), C_LOC(hb))
+  IF (cstat == STORM_OK .AND. culvert) THEN  ! cvdisch's discharges of the last stage
+    ALLOCATE(cs(nculvert))
+    cstat = storm_fvt_get_culverts(ctx, C_NULL_PTR, C_NULL_PTR, C_LOC(cs))
+    IF (cstat == STORM_OK) cvsrc(1:nculvert) = cs
+  END IF
   IF (cstat /= STORM_OK) CALL byebye('solverFVT: reading the GPU state failed.')
   h(1:n_elems) = hh
   u = uu

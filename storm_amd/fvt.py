@@ -59,7 +59,8 @@ class Opts(C.Structure):
                 ("cfl", C.c_double), ("rstop", C.c_double), ("h_dry", C.c_double),
                 ("h_wet", C.c_double), ("vceiling", C.c_double),
                 ("reorder", C.c_int32), ("block", C.c_int32), ("graph", C.c_int32),
-                ("nonfinite_stop", C.c_int32), ("fluxes", C.c_int32)]
+                ("nonfinite_stop", C.c_int32), ("fluxes", C.c_int32),
+                ("rsavg", C.c_int32), ("rsavg_n", C.c_int32), ("rsavg_eps", C.c_double)]
 
 
 EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch",
@@ -69,6 +70,7 @@ EXPORTS = ["storm_fvt_default_opts", "storm_fvt_create", "storm_fvt_create_batch
            "storm_fvt_get_prep_fields", "storm_fvt_get_output", "storm_fvt_get_c2v_weights",
            "storm_fvt_get_gradients", "storm_fvt_get_gradients_noalloc",
            "storm_fvt_get_bc_values", "storm_fvt_drynwet", "storm_fvt_fvsources",
+           "storm_fvt_set_wind", "storm_fvt_set_culverts", "storm_fvt_get_culverts",
            "storm_fvt_device_info",
            "storm_fvt_last_run_seconds", "storm_fvt_last_error", "storm_fvt_destroy",
            "storm_fvt_inspect_molecule"]
@@ -110,6 +112,9 @@ def load(path: str = LIB_PATH):
     L.storm_fvt_drynwet.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip]
     L.storm_fvt_fvsources.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.storm_fvt_get_bc_values.argtypes = [C.c_void_p, _dp, _dp]
+    L.storm_fvt_set_wind.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp]
+    L.storm_fvt_set_culverts.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _ip, _dp, _dp]
+    L.storm_fvt_get_culverts.argtypes = [C.c_void_p, _ip, _ip, _dp]
     L.storm_fvt_device_info.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_void_p)]
     L.storm_fvt_last_run_seconds.restype = C.c_double
     L.storm_fvt_last_run_seconds.argtypes = [C.c_void_p]
@@ -203,6 +208,9 @@ def opts_struct(o: Dict[str, object]) -> Opts:
     s.graph = int(o.get("graph", 0))
     s.nonfinite_stop = int(bool(o.get("nonfinite_stop", False)))
     s.fluxes = FLUXES[str(o.get("fluxes", "std"))]
+    s.rsavg = int(bool(o.get("rsavg", False)))
+    s.rsavg_n = int(o.get("rsavg_n", 2))
+    s.rsavg_eps = float(o.get("rsavg_eps", 0.5))
     return s
 
 
@@ -230,6 +238,12 @@ class FVTSolver:
         self.h = h
         self.N, self.P = case.n_elems, case.n_pts
         self.n_in, self.n_out = len(case.inflows), len(case.outflows)
+        self.ncv = 0
+        # windforcing and ioculvert's data, read before dataprepFVT's end
+        if getattr(case, "wind", None) is not None:
+            self.set_wind(*case.wind)
+        if getattr(case, "culverts", None):
+            self.set_culverts(case.culverts)
 
     def solverFVT(self):
         """solverFVT.f90: run to MAX ITERATION (or steady convergence), then
@@ -355,6 +369,44 @@ class FVTSolver:
                                              _p(src, C.c_double)))
         return src.reshape(3, self.P)
 
+    def set_wind(self, c, wx, wy):
+        """Wind data (windforcing.f90): one value per cell or per vertex."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (c, wx, wy)]
+        if not (len(a[0]) == len(a[1]) == len(a[2])):
+            raise StormError("wind arrays differ in length")
+        self._chk(load().storm_fvt_set_wind(self.h, len(a[0]), *[_p(x, C.c_double) for x in a]))
+
+    def clear_wind(self):
+        self._chk(load().storm_fvt_set_wind(self.h, 0, None, None, None))
+
+    def set_culverts(self, culverts):
+        """Culverts (cases.Culvert records, ioculvert.f90's data)."""
+        n = len(culverts)
+        if n == 0:
+            self._chk(load().storm_fvt_set_culverts(self.h, 0, None, None, None, None, None))
+            self.ncv = 0
+            return
+        pin = np.array([cv.inlet for cv in culverts], dtype=np.float64).reshape(-1)
+        pout = np.array([cv.outlet for cv in culverts], dtype=np.float64).reshape(-1)
+        ntab = np.array([len(cv.head) for cv in culverts], dtype=np.int32)
+        head = np.concatenate([np.asarray(cv.head, dtype=np.float64) for cv in culverts])
+        q = np.concatenate([np.asarray(cv.q, dtype=np.float64) for cv in culverts])
+        self._chk(load().storm_fvt_set_culverts(self.h, n, _p(pin, C.c_double),
+                                                _p(pout, C.c_double), _p(ntab, C.c_int32),
+                                                _p(head, C.c_double), _p(q, C.c_double)))
+        self.ncv = n
+
+    def culverts(self):
+        """cvtrigin, cvtrigout (0-based input cells) and the latest stage's
+        cvsrc."""
+        n = self.ncv
+        cin = np.zeros(max(n, 1), dtype=np.int32)
+        cout = np.zeros(max(n, 1), dtype=np.int32)
+        src = np.zeros(max(n, 1))
+        self._chk(load().storm_fvt_get_culverts(self.h, _p(cin, C.c_int32), _p(cout, C.c_int32),
+                                                _p(src, C.c_double)))
+        return cin[:n], cout[:n], src[:n]
+
     def bc_values(self):
         """qin, hbc of the latest step (get_time_bcs' module values)."""
         q, hb = np.zeros(max(self.n_in, 1)), np.zeros(max(self.n_out, 1))
@@ -411,6 +463,21 @@ class BatchSolver(FVTSolver):
         self.N, self.P = int(self.cell_ptr[-1]), int(self.pts_ptr[-1])
         self.n_in = sum(len(c.inflows) for c in cases_)
         self.n_out = sum(len(c.outflows) for c in cases_)
+        self.ncv = 0
+        winds = [getattr(c, "wind", None) for c in cases_]
+        if any(w is not None for w in winds):
+            # the reaches' wind data end to end, all at cells or all at vertices
+            at_c = [w is not None and len(w[0]) == c.n_elems for w, c in zip(winds, cases_)]
+            at_v = [w is not None and len(w[0]) == c.n_pts for w, c in zip(winds, cases_)]
+            if all(at_c):
+                self.set_wind(*[np.concatenate([w[k] for w in winds]) for k in range(3)])
+            elif all(at_v):
+                self.set_wind(*[np.concatenate([w[k] for w in winds]) for k in range(3)])
+            else:
+                raise StormError("a batch takes wind data for every reach, all at cells or all "
+                                 "at vertices")
+        if any(getattr(c, "culverts", None) for c in cases_):
+            raise StormError("culverts need a context of one reach")
 
     def status(self):
         """Per reach: (iter, ttime, rmax, done) arrays."""

@@ -128,8 +128,69 @@ def xsf_cases():
     }
 
 
+def _with_wind(c, at_vtx, seed, coef=2.0):
+    rng = np.random.default_rng(seed)
+    c.wind = cases.wind_data(rng, c.n_pts if at_vtx else c.n_elems, c=coef)
+    c.meta["wind"] = "wind_data(seed %d, %s, c=%g)" % (seed, "vertices" if at_vtx else "cells", coef)
+    return c
+
+
+def _meander_culverts():
+    """Culverts on the small meander (centreline y = 20 sin(2 pi x / 200),
+    channel |n| < 20, bank-full stage): a channel-to-floodplain culvert
+    whose head moves through its curve, one whose inlet sits on the dry
+    floodplain (never runs), one whose head is above its curve (its last
+    discharge), one below its first head (its first discharge), and two
+    sharing an outlet cell with opposite flows."""
+    C = cases.Culvert
+    return [
+        C((100.0, 0.0, -1.0), (150.0, 20.0), np.array([0.2, 0.6, 1.2, 2.5]),
+          np.array([0.4, 1.5, 3.0, 5.0])),
+        C((60.0, 60.0, 3.0), (250.0, 70.0), np.array([0.1, 1.0]), np.array([1.0, 2.0])),
+        C((300.0, 0.0, -1.0), (330.0, -35.0), np.array([0.05, 0.1]), np.array([0.5, 0.8])),
+        C((200.5, 1.5, -1.0), (251.0, 40.0), np.array([5.0, 6.0]), np.array([0.3, 0.9])),
+        C((120.0, -12.0, -1.2), (251.0, 40.0), np.array([0.5, 1.3, 2.0]),
+          np.array([0.2, 0.25, 0.7])),
+    ]
+
+
+def source_cases():
+    """The optional source branches of solverFVT.f90:218-256 against the
+    reference: wind (given at the vertices on a closed wet/dry basin, and at
+    the cells), culverts on a meander with banks (wet and dry inlets, heads
+    below, inside and above the curves, a shared outlet), residual smoothing
+    (ACTIVATE RSAVG: the bare option, so epsRSAVG 0.5 and nRSAVG 2) in a
+    steady flume, with HDLS gradients (the smoothing keeps elemnt_db's
+    t2t), and all three together at RK3."""
+    mkw = dict(ns=40, nn=12, length=400.0, width=120.0, amp=20.0, wavelength=200.0,
+               chan_w=40.0, chan_d=2.0, n_bumps=5, q0=5.0, q1=20.0, t_ramp=10.0, dt_op=0,
+               cfl=0.02)
+    basin = _with_wind(cases.dambreak(nq=14, side=200.0, seed=61, x_dam=80.0, h_res=2.0,
+                                      flux="ac", dt_op=0, cfl=0.02, istop=60, h_dry=1e-3),
+                       True, 6161)
+    flume_w = _with_wind(cases.flume(nx=20, ny=5, length=100.0, width=25.0, seed=62, dt_op=0,
+                                     cfl=0.05, istop=30, q_in=20.0), False, 6262)
+    mc = cases.meander(istop=80, seed=63, **mkw)
+    mc.culverts = _meander_culverts()
+    all3 = _with_wind(cases.meander(istop=60, seed=64, rkorder=3, rsavg=True, **mkw), False,
+                      6464, coef=1.0)
+    all3.culverts = _meander_culverts()
+    return {
+        "dambreak_wind_vtx": (basin, 60),
+        "flume_wind_cells": (flume_w, 30),
+        "meander_culverts": (mc, 80),
+        "flume_rsavg_steady": (cases.flume(nx=16, ny=6, length=100.0, width=30.0, seed=65,
+                                           dt_op=0, cfl=0.3, istop=60, q_in=15.0, steady=True,
+                                           stop_cr=1, rstop=1e-12, rsavg=True), 60),
+        "meander_hdls_rsavg": (cases.meander(istop=50, seed=66, gradient="hdls", rsavg=True,
+                                             **mkw), 50),
+        "meander_sources_rk3": (all3, 60),
+    }
+
+
 def golden_cases():
     """Every case with a golden fixture from the reference."""
     out = dict(small_cases())
     out.update(xsf_cases())
+    out.update(source_cases())
     return out

@@ -1,7 +1,9 @@
 """The drop-in end to end: the reference program's own input path
 (read_mfa and the mesh set-up) with dataprepFVT/solverFVT replaced by the
 Fortran shim over libstorm_fvt (oracle/_ref/dropin_driver, linked by
-`make -C oracle dropin`), run on the .run form of each small case and
+`make -C oracle dropin`), run on the .run form of each small case (the wind, culvert and residual
+smoothing options among them: the shim reads the wind file and passes
+ioculvert's data to the library) and
 compared with the reference's golden outputs, the post-loop fields
 (vertex velocities, cdvtx, e_est) included."""
 import os
@@ -11,7 +13,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from cases_small import small_cases
+from cases_small import small_cases, source_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -21,10 +23,16 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="drop-in driver not built")
-@pytest.mark.parametrize("name", list(small_cases().keys()))
+def _cases():
+    out = dict(small_cases())
+    out.update(source_cases())
+    return out
+
+
+@pytest.mark.parametrize("name", list(_cases().keys()))
 def test_dropin_matches_reference(name):
     from storm_amd import cases
-    case, n = small_cases()[name]
+    case, n = _cases()[name]
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     print(f"[dropin] {name} N={case.n_elems}", flush=True)
     with tempfile.TemporaryDirectory() as wd:
