@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--wet", type=float, default=None,
                     help="meander: start with the whole valley under water, this many metres "
                          "above the valley floor (4 puts every cell under water)")
+    ap.add_argument("--rsavg", type=int, default=0,
+                    help="meander: residual smoothing (ACTIVATE RSAVG) with this many sweeps")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter "
                          "calibration for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
@@ -173,8 +175,12 @@ def main():
         torch.cuda.synchronize()
     case = cases.meander(wet=args.wet)
     workload = "meander-2M-rk3" if args.wet is None else "meander-2M-rk3-wet"
+    extra = {}
+    if args.rsavg > 0:
+        workload += "-rsavg%d" % args.rsavg
+        extra = dict(rsavg=True, rsavg_n=args.rsavg)
     t0 = time.perf_counter()
-    solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph, block=args.block))
+    solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph, block=args.block, **extra))
     setup_s = time.perf_counter() - t0
     solver.run(args.warmup)
     t0 = time.perf_counter()

@@ -130,6 +130,10 @@ struct State {
   uint8_t *dry_post, *vwet;
   double *sq, *sh;    // current Q (or stage) of every inflow string, stage of every outflow string
   double *phi0, *phis;  // residual smoothing: the residual before and after it
+  // the shared-memory sweep's result in level order and each cell's
+  // position in it (null: the result is phis, in cell order)
+  const double4* phis_lo;
+  const int* rs_pos;
   Ctrl* ctrl;         // one per reach
   // per reach, `done` as the first stage of the latest step saw it (that
   // stage's kernel writes it), for the next step's first vertex pass, which
@@ -1388,7 +1392,12 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     const double area = tri_area(tr);
     double p0 = ZERO, p1 = ZERO, p2 = ZERO;
     if constexpr (MODE == 2) {
-      p0 = S.phis[i]; p1 = S.phis[N + i]; p2 = S.phis[2 * N + i];
+      if (S.phis_lo) {
+        const double4 v = S.phis_lo[S.rs_pos[i]];
+        p0 = v.x; p1 = v.y; p2 = v.z;
+      } else {
+        p0 = S.phis[i]; p1 = S.phis[N + i]; p2 = S.phis[2 * N + i];
+      }
     } else {
     const double dhx = F.dhx, dhy = F.dhy;
     const double xci = si.xc, yci = si.yc;
@@ -1791,6 +1800,193 @@ k_rsavg(RsTab rs, const Ctrl* ctrl, int N, int nsweep, double fac1, double fac2,
   }
 }
 
+// The same sweep with every value a level reads kept in shared memory (the
+// tables of Prep::rs_fast): a ring of RSF_W position slots, each holding a
+// position's previous-sweep value from RSF_G + 1 levels before its own level
+// and its new value after it, plus RSF_FS far slots for values read outside
+// that window.  A level then waits only on shared memory (its barrier fences
+// shared memory alone), and the global loads of the levels ahead (records,
+// phi0, previous-sweep values) are issued RSF_D levels early.  Positions are
+// in level order; the sweep reads and writes level-ordered copies (one
+// double4 per position, w unused) and also writes the cell-ordered result
+// the update pass reads.  Records are packed so that a level issues 9 loads
+// and 2 stores: the RSF_D - 1 levels in flight stay within the 6-bit vector
+// memory counter, whose waits are then exact, and the block's one compute
+// unit moves few cache lines per level (its memory pipeline, not latency,
+// bounds a level).
+struct RsFast {
+  const int *rlev, *pptr, *psrc, *pdst;
+  const int2* cf;    // per position {cell, far slot it writes or -1}
+  const int4* code;  // per position its RSF_E entries (Prep::rs_code) as 16-bit integers
+  const int4* hdr;   // per level two int4 (Prep::rs_hdr); a zero one at index L
+  int N, NF, L;      // positions, old far values filled by levels, levels
+};
+constexpr int RSF_D = 4;
+// bytes of the ring, the far slots and one dummy slot
+constexpr int RSF_LDS = 3 * 8 * (RSF_W + RSF_FS + 1);
+
+// phi0 in level order (lo[k] = phi0 of cell[k])
+__global__ void k_rs_order(RsFast T, const double* phi0, double4* lo) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T.N) return;
+  const int i = T.cf[k].x, N = T.N;
+  lo[k] = make_double4(phi0[i], phi0[N + i], phi0[2 * N + i], ZERO);
+}
+
+// the previous sweep's values of the old far values levels fill, with their
+// far slot in w (fval[f] = src_lo[fsrc[f]], w = fdst[f])
+__global__ void k_rs_far(RsFast T, const int* fsrc, const int* fdst, const double4* src_lo,
+                         double4* fval) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= T.NF) return;
+  const double4 v = src_lo[fsrc[f]];
+  fval[f] = make_double4(v.x, v.y, v.z, (double)fdst[f]);
+}
+
+// Registers of one pipeline stage: the records of level l and the header
+// of level l + RSF_D.  Lanes without work load from valid addresses and
+// write to the dummy slot, so the loop body is straight-line code.
+struct RsSet {
+  int k, cell, farw, q;
+  bool vk, vq, vf;  // the lane has a position; a ring fill; a far fill
+  int4 c;           // the entries, 16 bits each
+  double4 p, v, g;  // phi0; previous value of the fill position; far fill (slot in w)
+  int4 h0, h1;      // header of level l + RSF_D
+};
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The header of level l (a zero one past the reach), read by vector loads:
+// a scalar load would hold up every level's barrier, which waits for the
+// scalar memory counter it shares with shared memory.
+__device__ __forceinline__ void rs_header(const RsFast& T, int l, int l1, int4& h0, int4& h1) {
+  int idx = l < l1 ? l : T.L;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(idx) : "v"(idx));
+  h0 = T.hdr[2 * idx];
+  h1 = T.hdr[2 * idx + 1];
+}
+
+// Loads of level l (whose header s.h0/s.h1 holds) and the header of level
+// l + RSF_D: position t's record and phi0, the previous value of fill
+// position t, and old far value t.
+__device__ __forceinline__ void rs_load(RsSet& s, const RsFast& T, const double4* p0lo,
+                                        const double4* src_lo, const double4* fval, int l,
+                                        int l1, int t) {
+  const int4 h0 = s.h0, h1 = s.h1;  // {position start, width, fill start, fill width}, {far start, far count}
+  s.vk = t < h0.y;
+  const int k = h0.x + (s.vk ? t : 0);
+  s.k = k;
+  const int2 cf = T.cf[k];
+  s.cell = cf.x;
+  s.farw = cf.y;
+  s.c = T.code[k];
+  s.p = p0lo[k];
+  s.vq = t < h0.w;
+  const int q = h0.z + (s.vq ? t : 0);
+  s.q = q;
+  s.v = src_lo[q];
+  s.vf = t < h1.y;
+  s.g = fval[s.vf ? h1.x + t : 0];
+  rs_header(T, l + RSF_D, l1, s.h0, s.h1);
+}
+
+// One level: position t's smoothed value from shared memory, written to its
+// ring slot (and far slot) and to global memory; then the fills.
+__device__ __forceinline__ void rs_level(const RsSet& s, double* sm, int base, double fac1,
+                                         double fac2, double4* out_lo, double* dummy) {
+  const int dum = RSF_W + RSF_FS;
+  const int code[RSF_E] = {s.c.x << 16 >> 16, s.c.x >> 16, s.c.y << 16 >> 16, s.c.y >> 16,
+                           s.c.z << 16 >> 16, s.c.z >> 16, s.c.w << 16 >> 16, s.c.w >> 16};
+  // every entry's slot read at once (the dummy slot past a row's end);
+  // past the end the term is -0.0, which leaves any sum unchanged.  Rows
+  // of more than four entries (boundary cells) sit last in their level, so
+  // most waves skip the second half.
+  double q0 = fac1 * s.p.x, q1 = fac1 * s.p.y, q2 = fac1 * s.p.z;
+  auto half = [&](int e0) {
+    double a[4][3];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double* v = sm + 3 * (code[e0 + e] < 0 ? dum : code[e0 + e]);
+      a[e][0] = v[0]; a[e][1] = v[1]; a[e][2] = v[2];
+    }
+    // (empty statements that take the values: the reads stay unconditional,
+    // issued together, rather than one branch and wait per entry)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) asm volatile("" : "+v"(a[e][0]), "+v"(a[e][1]), "+v"(a[e][2]));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool end = code[e0 + e] < 0;
+      q0 = q0 + (end ? -ZERO : fac2 * a[e][0]);
+      q1 = q1 + (end ? -ZERO : fac2 * a[e][1]);
+      q2 = q2 + (end ? -ZERO : fac2 * a[e][2]);
+    }
+  };
+  half(0);
+  if (code[4] >= 0) half(4);
+  double* w = sm + 3 * (s.vk ? ((s.k - base) & (RSF_W - 1)) : dum);
+  w[0] = q0; w[1] = q1; w[2] = q2;
+  double* f = sm + 3 * ((s.vk && s.farw >= 0) ? RSF_W + s.farw : dum);
+  f[0] = q0; f[1] = q1; f[2] = q2;
+  double4* o = s.vk ? out_lo + s.k : reinterpret_cast<double4*>(dummy);
+  *o = make_double4(q0, q1, q2, ZERO);
+  double* r = sm + 3 * (s.vq ? ((s.q - base) & (RSF_W - 1)) : dum);
+  r[0] = s.v.x; r[1] = s.v.y; r[2] = s.v.z;
+  double* g = sm + 3 * (s.vf ? RSF_W + (int)s.g.w : dum);
+  g[0] = s.g.x; g[1] = s.g.y; g[2] = s.g.z;
+}
+
+// One sweep of every reach (a block each): src_lo holds the previous
+// sweep's values in level order (phi0's in the first sweep).  dummy: four
+// doubles that lanes without a position write to.
+__global__ void __launch_bounds__(RSF_NT)
+k_rsavg_fast(RsFast T, const Ctrl* ctrl, double fac1, double fac2, const double4* p0lo,
+             const double4* src_lo, const double4* fval, double4* out_lo, double* dummy) {
+  extern __shared__ double sm[];
+  const int r = blockIdx.x;
+  if (ctrl[r].done) return;
+  const int t = threadIdx.x, N = T.N;
+  const int l0 = T.rlev[r], l1 = T.rlev[r + 1];
+  const int base = T.hdr[2 * l0].x;
+  // prologue: the ring slots of the first RSF_G + 1 levels and the far slots
+  // read before any level fills them
+  for (int m = T.pptr[r] + t; m < T.pptr[r + 1]; m += (int)blockDim.x) {
+    const double4 v = src_lo[T.psrc[m]];
+    double* w = sm + 3 * T.pdst[m];
+    w[0] = v.x; w[1] = v.y; w[2] = v.z;
+  }
+  RsSet s0, s1, s2, s3;
+  rs_header(T, l0, l1, s0.h0, s0.h1);
+  rs_header(T, l0 + 1, l1, s1.h0, s1.h1);
+  rs_header(T, l0 + 2, l1, s2.h0, s2.h1);
+  rs_header(T, l0 + 3, l1, s3.h0, s3.h1);
+  rs_load(s0, T, p0lo, src_lo, fval, l0, l1, t);
+  rs_load(s1, T, p0lo, src_lo, fval, l0 + 1, l1, t);
+  rs_load(s2, T, p0lo, src_lo, fval, l0 + 2, l1, t);
+  rs_load(s3, T, p0lo, src_lo, fval, l0 + 3, l1, t);
+  __syncthreads();
+  for (int l = l0; l < l1; l += RSF_D) {
+    rs_level(s0, sm, base, fac1, fac2, out_lo, dummy);
+    rs_load(s0, T, p0lo, src_lo, fval, l + RSF_D, l1, t);
+    lds_barrier();
+    if (l + 1 >= l1) break;
+    rs_level(s1, sm, base, fac1, fac2, out_lo, dummy);
+    rs_load(s1, T, p0lo, src_lo, fval, l + 1 + RSF_D, l1, t);
+    lds_barrier();
+    if (l + 2 >= l1) break;
+    rs_level(s2, sm, base, fac1, fac2, out_lo, dummy);
+    rs_load(s2, T, p0lo, src_lo, fval, l + 2 + RSF_D, l1, t);
+    lds_barrier();
+    if (l + 3 >= l1) break;
+    rs_level(s3, sm, base, fac1, fac2, out_lo, dummy);
+    rs_load(s3, T, p0lo, src_lo, fval, l + 3 + RSF_D, l1, t);
+    lds_barrier();
+  }
+}
+
 // ---------------------------------------------------------------- output fields
 // Tables of the final to_vertices' velocity part (enforce_bcs.f90): per
 // vertex the outflow-string normals its velocity is checked against and its
@@ -2157,6 +2353,15 @@ struct storm_fvt {
   RsTab rs{};
   double rs_fac1 = 0.0, rs_fac2 = 0.0;
   int rs_n = 0;
+  // the shared-memory sweep (Prep::rs_fast): tables, the far values' sources,
+  // phi0 in level order, the level-ordered results of alternate sweeps and
+  // the old far values of the sweep being run
+  bool rsf = false;
+  int rsf_threads = 0;  // the widest level (and fill list) in whole waves
+  RsFast rsf_tab{};
+  const int *d_rsf_fsrc = nullptr, *d_rsf_fdst = nullptr;
+  double4 *d_p0lo = nullptr, *d_lo_a = nullptr, *d_lo_b = nullptr, *d_fval = nullptr;
+  double* d_rs_dummy = nullptr;
   Tiles tl{};
   int cap = 0;
   std::vector<Ctrl> hctrl;  // host copy of the reaches' control words
@@ -2372,6 +2577,48 @@ void setup_device(storm_fvt* C) {
     rs.rlev = dupload(own, P.r_lev);
     S.phi0 = dalloc<double>(own, 3 * N);
     S.phis = (C->rs_n > 0) ? dalloc<double>(own, 3 * N) : S.phi0;
+    if (P.rs_fast && C->rs_n > 0) {
+      RsFast& T = C->rsf_tab;
+      T.rlev = rs.rlev;
+      {
+        std::vector<int> cf(2 * N);
+        for (size_t k = 0; k < N; ++k) { cf[2 * k] = P.rs_cell[k]; cf[2 * k + 1] = P.rs_farw[k]; }
+        T.cf = reinterpret_cast<const int2*>(dupload(own, cf));
+        std::vector<int16_t> code(RSF_E * N);
+        for (size_t k = 0; k < N; ++k)
+          for (int e = 0; e < RSF_E; ++e) code[RSF_E * k + e] = (int16_t)P.rs_code[e * N + k];
+        T.code = reinterpret_cast<const int4*>(dupload(own, code));
+        std::vector<int> pos(N);
+        for (size_t k = 0; k < N; ++k) pos[P.rs_cell[k]] = (int)k;
+        S.rs_pos = dupload(own, pos);
+      }
+      T.hdr = reinterpret_cast<const int4*>(dupload(own, P.rs_hdr));
+      T.pptr = dupload(own, P.rs_pptr);
+      T.psrc = dupload(own, P.rs_psrc);
+      T.pdst = dupload(own, P.rs_pdst);
+      T.N = (int)N;
+      T.NF = (int)P.rs_fsrc.size();
+      T.L = (int)P.rs_lptr.size() - 1;
+      C->d_rsf_fsrc = dupload(own, P.rs_fsrc);
+      C->d_rsf_fdst = dupload(own, P.rs_fdst);
+      C->d_p0lo = reinterpret_cast<double4*>(dalloc<double>(own, 4 * N));
+      C->d_lo_a = reinterpret_cast<double4*>(dalloc<double>(own, 4 * N));
+      C->d_lo_b = reinterpret_cast<double4*>(dalloc<double>(own, 4 * N));
+      C->d_fval = reinterpret_cast<double4*>(dalloc<double>(own, 4 * (size_t)std::max(T.NF, 1)));
+      C->d_rs_dummy = dalloc<double>(own, 4);
+      S.phis_lo = ((C->rs_n - 1) & 1) ? C->d_lo_b : C->d_lo_a;  // the last sweep's
+      {
+        // idle waves would still issue every load of a level, so the block
+        // has only as many waves as the widest level or fill list needs
+        int w = 1;
+        for (size_t l = 0; l + 1 < P.rs_lptr.size(); ++l)
+          w = std::max({w, P.rs_lptr[l + 1] - P.rs_lptr[l], P.rs_fptr[l + 1] - P.rs_fptr[l]});
+        C->rsf_threads = std::min(RSF_NT, (w + 63) / 64 * 64);
+      }
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rsavg_fast),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, RSF_LDS));
+      C->rsf = true;
+    }
   }
   {
     if (C->device < 0 || C->device >= MAX_DEVICES)
@@ -2417,10 +2664,27 @@ void launch_stage(storm_fvt* C, const Stage& st) {
 void launch_smoothed_stage(storm_fvt* C, const Stage& st) {
   if (C->o.xsf) launch_stage<256, true, 1>(C, st);
   else launch_stage<256, false, 1>(C, st);
-  if (C->rs_n > 0)
+  if (C->rs_n > 0 && C->rsf) {
+    const RsFast& T = C->rsf_tab;
+    hipLaunchKernelGGL(k_rs_order, dim3((T.N + 255) / 256), dim3(256), 0, C->stream, T,
+                       (const double*)C->S.phi0, C->d_p0lo);
+    const double4* src = C->d_p0lo;
+    for (int sw = 0; sw < C->rs_n; ++sw) {
+      double4* out_lo = (sw & 1) ? C->d_lo_b : C->d_lo_a;
+      if (T.NF > 0)
+        hipLaunchKernelGGL(k_rs_far, dim3((T.NF + 255) / 256), dim3(256), 0, C->stream, T,
+                           C->d_rsf_fsrc, C->d_rsf_fdst, src, C->d_fval);
+      hipLaunchKernelGGL(k_rsavg_fast, dim3(C->prep.nr), dim3(C->rsf_threads), RSF_LDS, C->stream, T,
+                         (const Ctrl*)C->S.ctrl, C->rs_fac1, C->rs_fac2,
+                         (const double4*)C->d_p0lo, src, (const double4*)C->d_fval, out_lo,
+                         C->d_rs_dummy);
+      src = out_lo;
+    }
+  } else if (C->rs_n > 0) {
     hipLaunchKernelGGL(k_rsavg, dim3(C->prep.nr), dim3(RS_THREADS), 0, C->stream, C->rs,
                        C->S.ctrl, C->prep.N, C->rs_n, C->rs_fac1, C->rs_fac2, C->S.phi0,
                        C->S.phis);
+  }
   launch_stage<256, false, 2>(C, st);
 }
 

@@ -10,7 +10,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
+#include <queue>
 #include <unordered_map>
 
 namespace storm {
@@ -292,10 +294,15 @@ void rsavg_schedule(Prep& S, const std::vector<int>& tp, const std::vector<int>&
   for (int l = 0; l < L; ++l) S.rs_lptr[l + 1] += S.rs_lptr[l];
   std::vector<int> fill(S.rs_lptr.begin(), S.rs_lptr.end() - 1);
   S.rs_cell.assign(N, 0);
-  for (int ii = 0; ii < N; ++ii) {  // ascending internal ids within a level
-    const int i = S.perm[ii];
-    S.rs_cell[fill[S.r_lev[reach_of[i]] + lev[i]]++] = ii;
-  }
+  // within a level: rows of at most four entries first (the device sweep
+  // gives longer rows a second pass that whole waves of short rows skip),
+  // each group by ascending internal id
+  for (int pass = 0; pass < 2; ++pass)
+    for (int ii = 0; ii < N; ++ii) {
+      const int i = S.perm[ii];
+      if ((tp[i + 1] - tp[i] > 4) != (pass == 1)) continue;
+      S.rs_cell[fill[S.r_lev[reach_of[i]] + lev[i]]++] = ii;
+    }
   S.rs_nptr.assign(N + 1, 0);
   S.rs_nbr.clear();
   for (int k = 0; k < N; ++k) {
@@ -306,6 +313,149 @@ void rsavg_schedule(Prep& S, const std::vector<int>& tp, const std::vector<int>&
     }
     S.rs_nptr[k + 1] = (int)S.rs_nbr.size();
   }
+}
+
+// The tables of the shared-memory sweep (see Prep::rs_fast); false when the
+// mesh exceeds its limits (the sweep then reads global memory per level).
+bool rsavg_fast_tables(Prep& S) {
+  const int N = S.N, NR = (int)S.r_lev.size() - 1, L = S.r_lev[NR];
+  const int W = RSF_W, G = RSF_G;
+  S.rs_fast = false;
+  S.rs_code.clear(); S.rs_farw.clear(); S.rs_fptr.clear(); S.rs_fsrc.clear();
+  S.rs_fdst.clear(); S.rs_pptr.clear(); S.rs_psrc.clear(); S.rs_pdst.clear(); S.rs_hdr.clear();
+  for (int l = 0; l < L; ++l)
+    if (S.rs_lptr[l + 1] - S.rs_lptr[l] > RSF_NT) return false;
+  for (int k = 0; k < N; ++k)
+    if (S.rs_nptr[k + 1] - S.rs_nptr[k] > RSF_E) return false;
+  std::vector<int> pos(N), levp(N), lreach(L);
+  for (int k = 0; k < N; ++k) pos[S.rs_cell[k]] = k;
+  for (int r = 0; r < NR; ++r)
+    for (int l = S.r_lev[r]; l < S.r_lev[r + 1]; ++l) {
+      lreach[l] = r;
+      for (int k = S.rs_lptr[l]; k < S.rs_lptr[l + 1]; ++k) levp[k] = l;
+    }
+  auto base = [&](int k) { return S.rs_lptr[S.r_lev[lreach[levp[k]]]]; };
+  auto rend = [&](int k) { return S.rs_lptr[S.r_lev[lreach[levp[k]] + 1]]; };
+  // the level whose fill overwrites position p's ring slot (p + W's fill)
+  const int NEVER = std::numeric_limits<int>::max();
+  auto evict = [&](int p) {
+    const int q = p + W;
+    return q < rend(p) ? levp[q] - G - 1 : NEVER;
+  };
+  S.rs_code.assign((size_t)RSF_E * N, -1);
+  S.rs_farw.assign(N, -1);
+  // far values: new ones (written by their producer), old ones (filled from
+  // the previous sweep before their first reader)
+  std::vector<int> fn_end(N, -1), fo_first(N, NEVER), fo_last(N, -1);
+  std::vector<std::pair<int, int>> far_entries;  // (code index, position), kind by sign
+  for (int k = 0; k < N; ++k) {
+    const int lc = levp[k];
+    for (int m = S.rs_nptr[k], e = 0; m < S.rs_nptr[k + 1]; ++m, ++e) {
+      const int code = S.rs_nbr[m];
+      const bool old = code < 0;
+      const int p = pos[code & 0x7fffffff];
+      const int lj = levp[p];
+      const bool ring = (!old || lj <= lc + G) && evict(p) > lc;
+      const size_t ci = (size_t)e * N + k;
+      if (ring) {
+        S.rs_code[ci] = (p - base(p)) % W;
+      } else if (!old) {
+        fn_end[p] = std::max(fn_end[p], lc);
+        far_entries.push_back({(int)ci, p});
+      } else {
+        fo_first[p] = std::min(fo_first[p], lc);
+        fo_last[p] = std::max(fo_last[p], lc);
+        far_entries.push_back({(int)ci, -1 - p});
+      }
+    }
+  }
+  // old far values: filled at the latest level before their first reader
+  // with room in its fill list (else by the reach's prologue, level -1)
+  std::vector<int> fcount(L, 0);  // (the ring fills take their own lanes)
+  std::vector<int> fo_level(N, -2);
+  for (int p = 0; p < N; ++p) {
+    if (fo_last[p] < 0) continue;
+    const int l0 = S.r_lev[lreach[levp[p]]];
+    int lf = fo_first[p] - 1;
+    while (lf >= l0 && fcount[lf] >= RSF_NT) --lf;
+    if (lf >= l0) { ++fcount[lf]; fo_level[p] = lf; }
+    else fo_level[p] = -1;
+  }
+  // far slots: intervals [start, end] of levels, greedy in start order; a
+  // slot is free again for an interval starting after its last reader
+  struct Iv { int start, end, p; bool old; };
+  std::vector<Iv> iv;
+  for (int p = 0; p < N; ++p) {
+    if (fn_end[p] >= 0) iv.push_back({levp[p], fn_end[p], p, false});
+    if (fo_last[p] >= 0) {
+      const int l0 = S.r_lev[lreach[levp[p]]];
+      iv.push_back({fo_level[p] >= 0 ? fo_level[p] : l0 - 1, fo_last[p], p, true});
+    }
+  }
+  std::sort(iv.begin(), iv.end(), [](const Iv& a, const Iv& b) {
+    return a.start != b.start ? a.start < b.start : a.p < b.p;
+  });
+  std::vector<int> fn_slot(N, -1), fo_slot(N, -1);
+  std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                      std::greater<std::pair<int, int>>> busy;  // (end, slot)
+  std::vector<int> free_slots;
+  int nslots = 0;
+  for (const Iv& v : iv) {
+    while (!busy.empty() && busy.top().first < v.start) {
+      free_slots.push_back(busy.top().second);
+      busy.pop();
+    }
+    int slot;
+    if (!free_slots.empty()) { slot = free_slots.back(); free_slots.pop_back(); }
+    else slot = nslots++;
+    if (nslots > RSF_FS) return false;
+    busy.push({v.end, slot});
+    (v.old ? fo_slot : fn_slot)[v.p] = slot;
+  }
+  for (const auto& fe : far_entries) {
+    const int p = fe.second >= 0 ? fe.second : -1 - fe.second;
+    S.rs_code[fe.first] = W + (fe.second >= 0 ? fn_slot[p] : fo_slot[p]);
+  }
+  for (int p = 0; p < N; ++p)
+    if (fn_slot[p] >= 0) S.rs_farw[p] = fn_slot[p];
+  // per level, the old far values it fills; per reach, its prologue: the
+  // ring slots of its first G + 1 levels and the old far values of level -1
+  std::vector<std::vector<int>> fl(L);
+  std::vector<std::vector<int>> pro(NR);
+  for (int p = 0; p < N; ++p) {
+    if (fo_last[p] < 0) continue;
+    if (fo_level[p] >= 0) fl[fo_level[p]].push_back(p);
+    else pro[lreach[levp[p]]].push_back(p);
+  }
+  S.rs_fptr.assign(L + 1, 0);
+  for (int l = 0; l < L; ++l) {
+    for (int p : fl[l]) { S.rs_fsrc.push_back(p); S.rs_fdst.push_back(fo_slot[p]); }
+    S.rs_fptr[l + 1] = (int)S.rs_fsrc.size();
+  }
+  S.rs_pptr.assign(NR + 1, 0);
+  for (int r = 0; r < NR; ++r) {
+    const int l0 = S.r_lev[r], l1 = S.r_lev[r + 1];
+    for (int l = l0; l < std::min(l1, l0 + G + 1); ++l)
+      for (int k = S.rs_lptr[l]; k < S.rs_lptr[l + 1]; ++k) {
+        S.rs_psrc.push_back(k);
+        S.rs_pdst.push_back((k - S.rs_lptr[l0]) % W);
+      }
+    for (int p : pro[r]) { S.rs_psrc.push_back(p); S.rs_pdst.push_back(W + fo_slot[p]); }
+    S.rs_pptr[r + 1] = (int)S.rs_psrc.size();
+  }
+  S.rs_hdr.assign(8 * ((size_t)L + 1), 0);
+  for (int l = 0; l < L; ++l) {
+    int* h = &S.rs_hdr[8 * (size_t)l];
+    h[0] = S.rs_lptr[l];
+    h[1] = S.rs_lptr[l + 1] - S.rs_lptr[l];
+    const int lf = l + G + 1;
+    if (lf < S.r_lev[lreach[l] + 1]) { h[2] = S.rs_lptr[lf]; h[3] = S.rs_lptr[lf + 1] - S.rs_lptr[lf]; }
+    h[4] = S.rs_fptr[l];
+    h[5] = S.rs_fptr[l + 1] - S.rs_fptr[l];
+  }
+  S.rs_nfar = nslots;
+  S.rs_fast = true;
+  return true;
 }
 
 }  // namespace
@@ -1085,6 +1235,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
     const std::vector<int>& rp = (o->gradient == STORM_GRAD_HDLS) ? ep2 : t2t_ptr;
     const std::vector<int>& rt = (o->gradient == STORM_GRAD_HDLS) ? et2 : t2t;
     rsavg_schedule(S, rp, rt, split->cell_ptr);
+    rsavg_fast_tables(S);
   }
   return STORM_OK;
 }

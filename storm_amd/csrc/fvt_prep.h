@@ -137,8 +137,32 @@ struct Prep {
   // order) is rs_nbr[rs_nptr[k] .. rs_nptr[k+1]), internal ids with bit 31
   // set where the neighbour comes later in input order (its old value).
   std::vector<int> rs_lptr, rs_cell, rs_nptr, rs_nbr, r_lev;
+  // The sweep through shared memory (rs_fast; when the mesh fits its
+  // limits): every value a level reads lives in a block-local window, so a
+  // level waits on no global memory.  Position p's value sits in ring slot
+  // (p - first position of its reach) % RSF_W from RSF_G + 1 levels before
+  // p's own level (filled with the previous sweep's value) until the ring
+  // wraps; a value read outside that window goes through one of RSF_FS far
+  // slots (shared-memory index RSF_W + slot).  rs_code[e * N + k]: the
+  // shared-memory index of entry e of position k's t2t row, -1 past its end;
+  // rs_farw[k]: the far slot position k also writes, or -1.  Level l fills
+  // the ring with level l + RSF_G + 1's positions and the far slots listed
+  // in rs_fsrc/rs_fdst[rs_fptr[l] .. rs_fptr[l+1]) (source position, far
+  // slot); each reach first fills rs_psrc/rs_pdst[rs_pptr[r] ..
+  // rs_pptr[r+1]) (source position, shared-memory index).
+  // rs_hdr: per level l two int4, {first position, width, first position of
+  // level l + RSF_G + 1 (the ring fill), its width or 0 past the reach},
+  // {rs_fptr[l], its count, 0, 0}; then one zero entry (index L).
+  bool rs_fast = false;
+  int rs_nfar = 0;
+  std::vector<int> rs_code, rs_farw, rs_fptr, rs_fsrc, rs_fdst, rs_pptr, rs_psrc, rs_pdst, rs_hdr;
   std::string err;
 };
+
+// Limits of the shared-memory sweep: threads (positions per level), ring
+// slots, lookahead levels of the ring fill, far slots, entries per row.
+constexpr int RSF_NT = 512, RSF_W = 4096, RSF_G = 2, RSF_FS = 1536, RSF_E = 8;
+static_assert(RSF_W + RSF_FS < 32768, "entry codes are stored in 16 bits");
 
 // Builds everything from the C-ABI mesh; returns STORM_OK or a STORM_E_* code
 // with P.err set.

@@ -5,6 +5,21 @@ import numpy as np
 from storm_amd import cases
 
 
+def shuffled(case, seed):
+    """The same mesh with its triangles in a seeded random order (edges
+    rebuilt in the reference's order for the new numbering)."""
+    import copy
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(case.n_elems)
+    c = copy.deepcopy(case)
+    c.tri = np.ascontiguousarray(case.tri[perm])
+    c.edge_p, c.edge_e = cases.build_edges(c.tri, c.n_pts)
+    for k in ("cd", "h", "u", "v"):
+        setattr(c, k, np.ascontiguousarray(getattr(case, k)[perm]))
+    c.meta["shuffled"] = seed
+    return c
+
+
 def vtx_data_case():
     """A flume whose initial data are given where the reference's other
     branch of dataprepFVT.f90:73-140 reads them: the bed at the cells, depth,

@@ -7,7 +7,7 @@ with every branch on."""
 import numpy as np
 import pytest
 
-from cases_small import _meander_culverts, _with_wind, source_cases
+from cases_small import _meander_culverts, _with_wind, shuffled, source_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -149,4 +149,19 @@ def test_larger_meander_with_every_branch():
     print(f"[gpu sources] meander N={case.n_elems}", flush=True)
     o, s = _run_both(case, 40)
     _check(o.state(), s.state(), "large")
+    s.close()
+
+
+def test_rsavg_on_a_shuffled_numbering():
+    """Cells in random order: a few levels of thousands of cells each, wider
+    than the shared-memory sweep's block, so the global-memory level sweep
+    runs."""
+    from storm_amd import cases
+    case = shuffled(cases.meander(ns=100, nn=40, length=1000.0, width=400.0, amp=20.0,
+                                  wavelength=200.0, chan_w=40.0, chan_d=2.0, n_bumps=10,
+                                  q0=10.0, q1=40.0, t_ramp=20.0, dt_op=0, cfl=0.02, istop=30,
+                                  seed=95, rkorder=2, rsavg=True), 5)
+    print(f"[gpu sources] shuffled meander N={case.n_elems}", flush=True)
+    o, s = _run_both(case, 30)
+    _check(o.state(), s.state(), "shuffled")
     s.close()
