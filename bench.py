@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--wet", type=float, default=None,
                     help="meander: start with the whole valley under water, this many metres "
                          "above the valley floor (4 puts every cell under water)")
+    ap.add_argument("--reorder", type=int, default=1,
+                    help="meander: device cell order (1 Hilbert, 2 reverse Cuthill-McKee, 0 input)")
     ap.add_argument("--rsavg", type=int, default=0,
                     help="meander: residual smoothing (ACTIVATE RSAVG) with this many sweeps")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -180,6 +182,9 @@ def main():
         workload += "-rsavg%d" % args.rsavg
         extra = dict(rsavg=True, rsavg_n=args.rsavg)
     t0 = time.perf_counter()
+    if args.reorder != 1:
+        workload += "-order%d" % args.reorder
+        extra["reorder"] = args.reorder
     solver = fvt.FVTSolver(case, dict(istop=10 ** 9, graph=args.graph, block=args.block, **extra))
     setup_s = time.perf_counter() - t0
     solver.run(args.warmup)

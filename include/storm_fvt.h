@@ -113,7 +113,8 @@ typedef struct storm_fvt_opts {
   int32_t vclip;     /* MAX VELOCITY given */
   double cfl, rstop, h_dry, h_wet, vceiling;
   /* MI355X execution options (no reference counterpart) */
-  int32_t reorder;   /* 1 = Hilbert-curve cell order (default), 0 = input order */
+  int32_t reorder;   /* cell order on the device: 1 = Hilbert curve (default), 2 = reverse
+                        Cuthill-McKee, 0 = input order (results are the same) */
   int32_t block;     /* stage-kernel tile capacity (threads per block): 256, 384, 512; 0 = default */
   int32_t graph;     /* 1 = replay captured HIP graphs of whole steps */
   int32_t nonfinite_stop; /* 1 = a steady run (or reach of a batch) whose residual is

@@ -207,6 +207,64 @@ uint64_t hilbert_index(uint32_t x, uint32_t y) {
   return d;
 }
 
+// Reverse Cuthill-McKee order of cells [a, a + n) over their edge neighbours
+// (t2t3): from a pseudo-peripheral cell (the last one reached by a
+// breadth-first search from the first, repeated while the depth grows),
+// breadth-first with each cell's unvisited neighbours by ascending degree,
+// then reversed; every connected part in turn.  Returns local ids.
+std::vector<int> rcm_order(int a, int n, const std::vector<int>& t2t3) {
+  auto nbrs = [&](int i, int* out) {
+    int c = 0;
+    for (int j = 0; j < 3; ++j) {
+      const int k = t2t3[3 * (size_t)(a + i) + j];
+      if (k >= a && k < a + n) out[c++] = k - a;
+    }
+    return c;
+  };
+  std::vector<int> deg(n);
+  int tmp[3];
+  for (int i = 0; i < n; ++i) deg[i] = nbrs(i, tmp);
+  std::vector<int> order, lvl(n, -1);
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  auto bfs_last = [&](int s, int& depth) {  // the deepest cell of s's part, least degree
+    std::vector<int> q{s};
+    std::fill(lvl.begin(), lvl.end(), -1);
+    lvl[s] = 0;
+    int last = s;
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int i = q[h];
+      const int c = nbrs(i, tmp);
+      for (int j = 0; j < c; ++j)
+        if (lvl[tmp[j]] < 0) { lvl[tmp[j]] = lvl[i] + 1; q.push_back(tmp[j]); }
+      if (lvl[i] > lvl[last] || (lvl[i] == lvl[last] && deg[i] < deg[last])) last = i;
+    }
+    depth = lvl[last];
+    return last;
+  };
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (seen[s0]) continue;
+    int depth = -1, d2 = 0, s = s0;
+    for (int it = 0; it < 8; ++it) {
+      const int t = bfs_last(s, d2);
+      if (d2 <= depth) break;
+      depth = d2;
+      s = t;
+    }
+    const size_t b = order.size();
+    order.push_back(s);
+    seen[s] = 1;
+    for (size_t h = b; h < order.size(); ++h) {
+      const int c = nbrs(order[h], tmp);
+      std::sort(tmp, tmp + c, [&](int x, int y) { return deg[x] != deg[y] ? deg[x] < deg[y] : x < y; });
+      for (int j = 0; j < c; ++j)
+        if (!seen[tmp[j]]) { seen[tmp[j]] = 1; order.push_back(tmp[j]); }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
 std::vector<int> hilbert_order(int N, const std::vector<double>& xc, const std::vector<double>& yc) {
   double x0 = xc[0], x1 = xc[0], y0 = yc[0], y1 = yc[0];
   for (int i = 1; i < N; ++i) {
@@ -541,7 +599,7 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
   if (o->flux < 0 || o->flux > 3 || o->gradient < 0 || o->gradient > 2 || o->limiter < 0 ||
       o->limiter > 2 || o->btype < 0 || o->btype > 1 || o->friction < 0 || o->friction > 1 ||
       o->dt_op < 0 || o->dt_op > 1 || o->stop_cr < 0 || o->stop_cr > 2 || o->fluxes < 0 ||
-      o->fluxes > 1 || o->rsavg < 0 || o->rsavg > 1 || (o->rsavg && o->rsavg_n < 0) ||
+      o->fluxes > 1 || o->reorder < 0 || o->reorder > 2 || o->rsavg < 0 || o->rsavg > 1 || (o->rsavg && o->rsavg_n < 0) ||
       (o->rsavg && !(std::fabs(o->rsavg_eps) <= 1e300))) {
     S.err = "invalid option value"; return STORM_E_ARG;
   }
@@ -1006,11 +1064,17 @@ int build_prep(const storm_mesh_t* m, const storm_fvt_opts_t* o, Prep& S, const 
   std::vector<int> order;
   order.resize(N);
   std::iota(order.begin(), order.end(), 0);
-  if (o->reorder)
+  if (o->reorder == 1)
     for (int r = 0; r < NR; ++r) {
       const int a = split->cell_ptr[r], n = split->cell_ptr[r + 1] - a;
       std::vector<double> rx(xc.begin() + a, xc.begin() + a + n), ry(yc.begin() + a, yc.begin() + a + n);
       const std::vector<int> ro = hilbert_order(n, rx, ry);
+      for (int k = 0; k < n; ++k) order[a + k] = a + ro[k];
+    }
+  else if (o->reorder == 2)
+    for (int r = 0; r < NR; ++r) {
+      const int a = split->cell_ptr[r], n = split->cell_ptr[r + 1] - a;
+      const std::vector<int> ro = rcm_order(a, n, t2t3);
       for (int k = 0; k < n; ++k) order[a + k] = a + ro[k];
     }
   S.perm = order;

@@ -245,3 +245,27 @@ extern "C" int probe_rsavg(const storm_mesh_t* m, const storm_fvt_opts_t* o, int
     if (a[q] != outc[q]) return fail("shared-memory sweep differs from the serial sweep");
   return 0;
 }
+
+// The source-term set-up helpers on a mesh: wind coefficients from n values
+// (cells or vertices) and culvert location; returns their status codes
+// (wind in rc[0], culverts in rc[1]) and the located cells.
+extern "C" int probe_sources(const storm_mesh_t* m, const storm_fvt_opts_t* o, int64_t nw,
+                             const double* c, const double* wx, const double* wy, int ncv,
+                             const double* pin, const double* pout, int* rc, int* cells) {
+  Prep P;
+  int r = build_prep(m, o, P);
+  if (r != STORM_OK) return r;
+  std::vector<double> w;
+  std::string err;
+  rc[0] = wind_coefs(P, nw, c, wx, wy, w, err);
+  std::vector<int> cin, cout;
+  std::vector<double> rin;
+  rc[1] = ncv > 0 ? locate_culverts(P, ncv, pin, pout, cin, cout, rin, err) : 0;
+  if (rc[1] == STORM_OK)
+    for (int k = 0; k < ncv; ++k) { cells[2 * k] = cin[k]; cells[2 * k + 1] = cout[k]; }
+  std::vector<double> z(P.N), zv(P.P), ze(P.N), cd(P.N), cdv(P.P), hv(P.P), uv(P.P), vv(P.P),
+      zev(P.P);
+  dataprep_fields(P, z.data(), zv.data(), ze.data(), cd.data(), cdv.data(), hv.data(), uv.data(),
+                  vv.data(), zev.data());
+  return 0;
+}
