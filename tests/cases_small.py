@@ -203,6 +203,21 @@ def source_cases():
     }
 
 
+def large_cases():
+    """Full-length runs against the reference (fixtures of the final state
+    only): the config-1 flume (20,000 triangles, 2,000 SSP-RK2 steps) with
+    the reference's own Roe flux (flux_ac) at a fixed dt of 0.025 s (CFL
+    about 0.7-0.9; the reference's CFL time step reads out of bounds), and a
+    200,000-triangle cut of the config-2 meander (2 km, 20 bumps) whose
+    inflow ramps 500 -> 3,000 m3/s in 60 s so that the bank-full channel
+    spills and wet/dry fronts move over 1,500 SSP-RK3 steps."""
+    return {
+        "flume20k_ac": (cases.flume(flux="ac", dt_op=0, cfl=0.025, istop=2000), 2000),
+        "meander200k": (cases.meander(ns=400, length=2000.0, n_bumps=20, t_ramp=60.0, dt_op=0,
+                                      cfl=0.025, istop=1500), 1500),
+    }
+
+
 def golden_cases():
     """Every case with a golden fixture from the reference."""
     out = dict(small_cases())

@@ -8,6 +8,9 @@ the post-loop fields (uvtx, vvtx, cdvtx, zvtx, delh, e_est) and the module
 copies dataprepFVT leaves (prep_*) as .npz fixtures.  The fixtures are data: inputs are regenerated from the case
 generator and seed recorded with them.
 
+The full-length runs of tests/cases_small.py:large_cases() (final h, hu,
+hv, dry mask and vertex depths only) are written when named.
+
 Usage (from the repository root):  python tests/golden/make_golden.py [name ...]
 """
 import os
@@ -23,8 +26,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle as O  # noqa: E402
 from storm_amd import cases  # noqa: E402
-from cases_small import golden_cases  # noqa: E402
+from cases_small import golden_cases, large_cases  # noqa: E402
 
+# the full-length runs keep their final state only
+LARGE_KEYS = ("h", "qh", "qu", "qv", "dry", "hvtx")
 KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", "zetavtx",
         "uvtx", "vvtx", "cdvtx", "zvtx", "delhx", "delhy", "e_est")
 
@@ -32,14 +37,21 @@ KEYS = ("h", "u", "v", "qh", "qu", "qv", "phi1", "phi2", "phi3", "dry", "hvtx", 
 def main():
     out_dir = os.path.dirname(os.path.abspath(__file__))
     only = set(sys.argv[1:])
-    for name, (case, n) in golden_cases().items():
+    large = large_cases()
+    todo = dict(golden_cases())
+    if only:
+        todo.update(large)
+    for name, (case, n) in todo.items():
         if only and name not in only:
             continue
         assert int(case.opts["istop"]) == n
         with tempfile.TemporaryDirectory() as wd:
-            ref = O.run_reference(case, wd)
-        data = {k: ref[k] for k in KEYS}
-        data.update({"prep_" + k: ref["prep_" + k] for k, _ in cases.PREP_KEYS})
+            ref = O.run_reference(case, wd, timeout=3600.0)
+        if name in large:
+            data = {k: ref[k] for k in LARGE_KEYS}
+        else:
+            data = {k: ref[k] for k in KEYS}
+            data.update({"prep_" + k: ref["prep_" + k] for k, _ in cases.PREP_KEYS})
         data["iter"] = np.int64(ref["iter"])
         data["ttime"] = np.float64(ref["ttime"])
         data["meta"] = np.array(repr(dict(case.meta)))

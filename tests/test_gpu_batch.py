@@ -104,3 +104,41 @@ def test_nonfinite_stop_retires_a_diverging_run():
     it, _, rm = s.run(200)
     assert it == first and np.isnan(rm)
     s.close()
+
+
+def test_ensemble_reaches_against_the_oracle():
+    """Seven reaches of the config-4 generator, 5,344 to 153,648 triangles
+    (about 0.5M together), advanced as one batch on the ensemble's stated
+    scheme (HLL, least squares with Barth-Jespersen, SSP-RK2, steady local
+    time steps at CFL 0.9) for 100 steps: each reach equals the oracle's
+    run of it alone, non-finite values included (reach 3 diverges from this
+    cold start on the oracle exactly as on the device)."""
+    import oracle as O
+    ks = (7, 13, 3, 29, 21, 15, 18)
+    rs = [cases.ensemble_reach(k, steady=True) for k in ks]
+    opts = dict(flux="hll", limiter=2, rkorder=2, cfl=0.9, steady=True, stop_cr=1,
+                rstop=1e-12, dt_op=0, istop=100)
+    print(f"[gpu batch] ensemble reaches {[c.n_elems for c in rs]}", flush=True)
+    b = fvt.BatchSolver(rs, opts)
+    b.run(100)
+    parts = b.split(b.state())
+    it, tt, rm, dn = b.status()
+    b.close()
+    for k, c, p, i, t in zip(ks, rs, parts, it, tt):
+        o = O.Oracle(c, opts)
+        r = o.run(100)
+        so = o.state()
+        assert i == r["iter"] == 100
+        assert t == pytest.approx(r["ttime"], rel=1e-13)
+        bad = 0
+        for key in ("h", "qh", "qu", "qv"):
+            a, g = so[key], p[key]
+            fin = np.isfinite(a)
+            assert np.array_equal(fin, np.isfinite(g)), (k, key)
+            assert np.array_equal(a[~fin], g[~fin], equal_nan=True), (k, key)
+            bad += int((~fin).sum())
+            if fin.any():
+                err = np.abs(a[fin] - g[fin]).max() / max(np.abs(a[fin]).max(), 1e-300)
+                assert err <= 1e-10, (k, key, err)
+        assert np.array_equal(so["dry"], p["dry"]), k
+        print(f"  reach {k} ({c.n_elems} cells): equal, {bad} non-finite values", flush=True)

@@ -12,10 +12,15 @@ computes that power correctly rounded, which the host libm's pow also is
 on all but rare inputs; the device libm's pow differs from it on about a
 fifth of them.
 """
+import os
+
 import numpy as np
 import pytest
 
+from cases_small import large_cases
 from storm_amd import cases, fvt
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -116,3 +121,31 @@ def test_meander_2m_10000_steps():
     assert (st["qh"] >= 0).all()
     assert np.array_equal(st["dry"] != 0, st["h"] < h_dry)
     assert np.mean(st["h"] > h_dry) >= 0.99 * wet0
+
+
+@pytest.mark.parametrize("name", list(large_cases().keys()))
+def test_full_length_runs_against_the_reference(name):
+    """The flang reference's own final state (tests/golden/<name>.npz): the
+    config-1 flume with flux_ac through 2,000 steps, and the 200k meander
+    cut through 1,500 steps while its fronts move onto the floodplain."""
+    case, n = large_cases()[name]
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    print(f"[gpu fullsize] {name} N={case.n_elems}, {n} steps", flush=True)
+    s = fvt.FVTSolver(case)
+    it, tt, rm = s.run(n)
+    s.finalize()
+    st = s.state()
+    s.close()
+    assert it == int(g["iter"]) == n
+    assert tt == pytest.approx(float(g["ttime"]), rel=1e-13)
+    for k in ("h", "qh", "qu", "qv"):
+        err = np.abs(g[k] - st[k]).max() / max(np.abs(g[k]).max(), 1e-300)
+        print("  %s rel L-inf %.3e" % (k, err), flush=True)
+        assert err <= RTOL, k
+    assert np.abs(g["hvtx"] - st["hvtx"]).max() <= 1e-9 * np.abs(g["hvtx"]).max()
+    assert np.array_equal(g["dry"], st["dry"])
+    h_dry = case.opts["h_dry"]
+    wet0, wet1 = int((case.h > h_dry).sum()), int((st["h"] > h_dry).sum())
+    print("  wet cells %d -> %d" % (wet0, wet1), flush=True)
+    if name.startswith("meander"):
+        assert wet1 > wet0 + 100  # the fronts moved
