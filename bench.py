@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--wet", type=float, default=None,
                     help="meander: start with the whole valley under water, this many metres "
                          "above the valley floor (4 puts every cell under water)")
+    ap.add_argument("--limiter", type=int, default=2,
+                    help="ensemble: LIMITER (2 Barth-Jespersen, the stated scheme; 0 first order)")
     ap.add_argument("--reorder", type=int, default=1,
                     help="meander: device cell order (1 Hilbert, 2 reverse Cuthill-McKee, 0 input)")
     ap.add_argument("--rsavg", type=int, default=0,
@@ -246,16 +248,19 @@ def main():
 
 
 def ensemble(args):
-    """Time to steady state of a batched ensemble from the generator's cold
-    start, or (--start) from the plane guess or the learned warm start."""
+    """Time to steady state of a batched ensemble from the cold start
+    (warmstart.cold_state: still water covering every cell), or (--start)
+    from the plane guess or the learned warm start."""
     from storm_amd import cases, fvt, tune
     t0 = time.perf_counter()
     reaches = [cases.ensemble_reach(k, steady=True) for k in range(args.reaches)]
     gen_s = time.perf_counter() - t0
-    # first-order HLL: the Barth-Jespersen runs of this generator stall at an
-    # RMS dh of about 1e-5, above the 1e-6 target
-    opts = dict(flux="hll", limiter=0, rkorder=2, cfl=0.9, steady=True, stop_cr=1, rstop=1e-6,
-                dt_op=0, istop=args.max_steps, nonfinite_stop=True)
+    # the stated scheme is least squares + Barth-Jespersen (--limiter 2);
+    # --limiter 0 runs first-order HLL
+    opts = dict(flux="hll", limiter=args.limiter, rkorder=2, cfl=0.9, steady=True, stop_cr=1,
+                rstop=1e-6, dt_op=0, istop=args.max_steps, nonfinite_stop=True)
+    if args.rsavg > 0:
+        opts.update(rsavg=True, rsavg_n=args.rsavg)
     best, times = tune.tune(reaches, opts, steps=4)
     opts.update(best)
     t0 = time.perf_counter()
@@ -270,6 +275,11 @@ def ensemble(args):
             ws = W.WarmStart.load(args.checkpoint or W.DEFAULT_CHECKPOINT)
             init = ws.predict(reaches)
             ws.close()
+        b.set_state(np.concatenate([x[0] for x in init]), np.concatenate([x[1] for x in init]),
+                    np.concatenate([x[2] for x in init]))
+    else:
+        from storm_amd import warmstart as W
+        init = [W.cold_state(c) for c in reaches]
         b.set_state(np.concatenate([x[0] for x in init]), np.concatenate([x[1] for x in init]),
                     np.concatenate([x[2] for x in init]))
     init_s = time.perf_counter() - t0
@@ -292,10 +302,15 @@ def ensemble(args):
         "cell_updates_per_s": updates / wall, "gpu_seconds": gpu_s,
         "reaches": len(reaches), "triangles": int(n.sum()),
         "converged": int((dn & np.isfinite(rm)).sum()), "diverged": int((~np.isfinite(rm)).sum()),
+        "diverged_reaches": [int(k) for k in np.where(~np.isfinite(rm))[0]],
+        "capped_reaches": [int(k) for k in np.where(~dn)[0]],
         "steps_max": int(it.max()), "steps_median": float(np.median(it)),
         "dtype": "f64", "data": "synthetic (storm_amd/cases.py ensemble_reach, seeds 4004+k)",
-        "config": {"workload": "ensemble-%d" % len(reaches), "scheme": "HLL first order, SSP-RK2, "
-                   "steady local dt CFL 0.9, stop RMS dh <= 1e-6", "launch": best,
+        "config": {"workload": "ensemble-%d" % len(reaches),
+                   "scheme": "HLL %s, SSP-RK2, steady local dt CFL 0.9%s, stop RMS dh <= 1e-6"
+                   % ("first order" if args.limiter == 0 else
+                      "LSQ + Barth-Jespersen" if args.limiter == 2 else "LSQ unlimited",
+                      ", RSAVG %d sweeps" % args.rsavg if args.rsavg else ""), "launch": best,
                    "tune_s_per_step": times, "generate_s": round(gen_s, 1),
                    "setup_s": round(setup_s, 1)},
     }
@@ -428,7 +443,7 @@ def warmstart_accuracy(args):
     res = {}
     for mode in args.modes.split(","):
         if mode == "cold":
-            init = None
+            init = [W.cold_state(c) for c in reaches]
         elif mode == "plane":
             init = [W.baseline_state(c) for c in reaches]
         else:
@@ -483,8 +498,8 @@ def warmstart_accuracy(args):
 
 def warmstart(args):
     """Wall time and steps to steady state (RMS dh <= rstop) of held-out
-    reaches from three starts, each run as one batched context: cold (the
-    generator's flat water surface at the outlet stage), the plane guess
+    reaches from three starts, each run as one batched context: cold (still
+    water covering every cell, warmstart.cold_state), the plane guess
     (warmstart.baseline_state, no network), and the learned warm start
     (features + network inference + upload inside the timed region)."""
     from storm_amd import warmstart as W
@@ -499,7 +514,9 @@ def warmstart(args):
     for mode in modes:
         t0 = time.perf_counter()
         init, pred_s = None, 0.0
-        if mode == "plane":
+        if mode == "cold":
+            init = [W.cold_state(c) for c in reaches]
+        elif mode == "plane":
             init = [W.baseline_state(c) for c in reaches]
         elif mode in ("learned", "learned_surface"):
             # learned_surface: the network's water surface with the plane's

@@ -39,9 +39,10 @@ def signature(cases_: Sequence, opts: Dict[str, object]) -> str:
     """Workload shape: reach count, total cells (to a factor of 2) and the
     options that change the kernels' work."""
     n = sum(c.n_elems for c in cases_)
-    return "r%d-n2^%d-rk%s-%s-%s-f%s" % (len(cases_), int(round(math.log2(max(n, 1)))),
-                                         opts.get("rkorder"), opts.get("flux"),
-                                         opts.get("gradient"), opts.get("friction"))
+    return "r%d-n2^%d-rk%s-%s-%s-f%s-l%s-s%s" % (
+        len(cases_), int(round(math.log2(max(n, 1)))), opts.get("rkorder"), opts.get("flux"),
+        opts.get("gradient"), opts.get("friction"), opts.get("limiter"),
+        int(opts.get("rsavg_n", 0)) if opts.get("rsavg") else 0)
 
 
 def _make(cases_, opts):
@@ -70,7 +71,11 @@ def tune(case_or_cases, opts: Optional[Dict[str, object]] = None, steps: int = 6
         return dict(cache[key]["best"]), dict(cache[key]["times"])
     times = {}
     best, best_t = None, float("inf")
-    for cand in candidates or CANDIDATES:
+    if candidates is None:
+        candidates = CANDIDATES
+        if base.get("rsavg") or str(base.get("fluxes", "std")) == "xsf":
+            candidates = [c for c in CANDIDATES if c["block"] == 256]  # their only tile size
+    for cand in candidates:
         o = dict(base)
         o.update(cand)
         o["istop"] = 10 ** 9

@@ -228,6 +228,23 @@ def decode(fr: Frame, y: np.ndarray, h_dry: float = 1e-3, discharge: str = "lear
     return h, u, v
 
 
+def cold_state(case):
+    """The uninformed start: still water at one stage, the larger of the
+    outlet stage and the highest cell bed plus the outlet depth, so that
+    every cell (the inflow string's too) starts under at least the outlet's
+    depth.  Flat water at the outlet stage alone leaves the upper part of a
+    reach whose bed drops more than its outlet depth shallow or dry, and 30
+    of the 256 ensemble reaches then blow up within ten steps (on the oracle
+    as on the device); from this start every one converges."""
+    zc = case.zvtx[case.tri].sum(axis=1) / 3.0
+    bo = case.outflows[0]
+    H = float(bo.v[0])
+    d_out = H - float(case.zvtx[bo.nodes].mean())
+    stage = max(H, float(zc.max()) + d_out)
+    h = np.maximum(stage - zc, 0.0)
+    return h, np.zeros(case.n_elems), np.zeros(case.n_elems)
+
+
 def baseline_state(case, h_dry: float = 1e-3):
     """The decode of zero outputs: the plane through the outlet stage
     parallel to the fitted bed, with uniform discharge along the axis."""

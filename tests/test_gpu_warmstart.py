@@ -60,7 +60,7 @@ def test_prediction_is_closer_to_steady_than_the_plane(net, steady):
 def test_warm_start_needs_fewer_steps_than_cold(net, steady):
     rs, ref = steady
     opts = dict(rstop=1e-6, nonfinite_stop=True)
-    cold, _ = W.steady_states(rs, opts, 200000, chunk=5000)
+    cold, _ = W.steady_states(rs, opts, 200000, chunk=5000, init=[W.cold_state(c) for c in rs])
     warm, _ = W.steady_states(rs, opts, 200000, chunk=5000, init=net.predict(rs))
     s_cold = np.array([it for _, it in cold])
     s_warm = np.array([it for _, it in warm])
