@@ -150,6 +150,12 @@ struct Tiles {
   const int4* hrec;             // per halo entry, two int4: {cell, nbr0..2}, {vtx0..2, 0}
   const uint64_t* hmeta;        // per halo entry: the cell's metadata word
   const uint64_t* cln;          // per tile cell: local neighbour ids (0xFFFF: none)
+  // per block {first, interior count | boundary count << 16} of its side
+  // records (the sides whose fluxes are formed once each: interior edges with
+  // an owned cell, then owned boundary sides); record bits 0-15 owned local id a, 16-17 side ja, 18-35 the slot
+  // bits of (a, ja), 36-51 the neighbour's local id across an interior side
+  const int2* ehdr;
+  const uint64_t* erec;
   int nt;
 };
 
@@ -653,7 +659,10 @@ __global__ void k_gauss1(Geo g, const CellD* d, double* gwork) {
   gwork[5 * N + i] = cy0 * d0.v + cy1 * d1.v + cy2 * d2.v;
 }
 
-// limiterBJ.f90:42-81, beta = 1 (cell_grads.f90:59)
+// limiterBJ.f90:42-81, beta = 1 (cell_grads.f90:59).  The side's ratio only
+// enters through min(ratio, 1): where the quotient is at least 1 (its
+// numerator is no smaller in size than dq, same sign) that is 1 whatever the
+// quotient's bits, so the division is formed only where it can limit.
 __device__ __forceinline__ void bj_limit(double q0, const double qn[3], const double rcx[3],
                                          const double rcy[3], double& gx, double& gy) {
   double r[3];
@@ -662,10 +671,11 @@ __device__ __forceinline__ void bj_limit(double q0, const double qn[3], const do
     const double qmin = dmin(q0, qn[j]), qmax = dmax(q0, qn[j]);
     const double qj = q0 + gx * rcx[j] + gy * rcy[j];
     const double dq = qj - q0;
-    double rr;
-    if (equals(dq, ZERO)) rr = ONE;
-    else if (dq < 0) rr = (qmin - q0) / dq;
-    else rr = (qmax - q0) / dq;
+    double rr = ONE;
+    if (!equals(dq, ZERO)) {
+      const double num = dq < 0 ? qmin - q0 : qmax - q0;
+      if (dq < 0 ? !(num <= dq) : !(num >= dq)) rr = num / dq;
+    }
     const double a = dmin(ONE * rr, ONE), b = dmin(rr, ONE);
     r[j] = dmax(a, b);
   }
@@ -1041,13 +1051,15 @@ struct Nbr {
 
 // invisc_fluxes.f90:39-166 for one interior side of cell i: the flux of the
 // edge computed exactly as the reference does for it (left state = its e(1)
-// cell, normal out of e(1)), returned as cell i's change of phi.
-__device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, const Side& sd,
+// cell, normal out of e(1)), returned as cell i's change of phi.  Returns
+// false where the edge carries no flux (zero change, +0 for both cells);
+// otherwise the neighbour's change is the exact negation of cell i's.
+__device__ __forceinline__ bool interior_flux(const Opts& o, uint32_t bits, const Side& sd,
                                               bool dry_i, double hi, double zi, double fho,
                                               double fuo, double fvo, const Nbr& nb, double& d0,
                                               double& d1, double& d2) {
   d0 = ZERO; d1 = ZERO; d2 = ZERO;
-  if (dry_i && nb.dry) return;
+  if (dry_i && nb.dry) return false;
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   const bool left = (bits >> CMS_LEFT) & 1;
   double hl = left ? fho : nb.fh, ul = left ? fuo : nb.fu, vl = left ? fvo : nb.fv;
@@ -1056,7 +1068,7 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
   const double zL = left ? zi : nb.z, zR = left ? nb.z : zi;
   const double wL = left ? hi : nb.h, wR = left ? nb.h : hi;
   const double nx = left ? sd.nx : -sd.nx, ny = left ? sd.ny : -sd.ny;
-  if (hl < h_dry && hr < h_dry) return;
+  if (hl < h_dry && hr < h_dry) return false;
   bool set_wall = false;
   if (dL || dR) {
     const double h0 = dmax(hr, hl) + sd.zm;
@@ -1087,6 +1099,7 @@ __device__ __forceinline__ void interior_flux(const Opts& o, uint32_t bits, cons
   const double L = sd.len;
   if (left) { d0 = -(f[0] * L); d1 = -(f[1] * L); d2 = -(f[2] * L); }
   else { d0 = f[0] * L; d1 = f[1] * L; d2 = f[2] * L; }
+  return true;
 }
 
 // ---------------------------------------------------------------- stage kernel
@@ -1257,6 +1270,9 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 template <int CAP, bool XSF = false, int MODE = 0>
 __global__ void __launch_bounds__(CAP)
 k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
+  // fluxes once per edge from the side records (the xsf loop forms them per
+  // cell side, as before)
+  constexpr bool EDGES = !XSF && MODE != 2;
   extern __shared__ double lds_dyn[];
   __shared__ double s_at[MAX_STRINGS];
   __shared__ int s_null[MAX_STRINGS], s_skip[MAX_STRINGS];
@@ -1311,6 +1327,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   }
 
   Faces F;
+  int tile_wet = 1;  // some tile or halo cell is wet (phase 3b)
   if constexpr (MODE == 2) {
     // what phase 4 reads of the faces: the stage input's depth, dry flag
     // and |u| (hgradfix2's zeroed slot values come from the slots themselves)
@@ -1363,7 +1380,56 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     L.h[t] = F.h;
     L.dry[t] = F.dry;
   }
-  __syncthreads();
+  if constexpr (EDGES) {
+    // 3b. the flux of every side of the tile's owned cells, once per edge,
+    // one record per thread.  The two face states of an edge are read only
+    // by its record, which overwrites them with the two cells' changes of
+    // phi (the neighbour's is the exact negation, +0 where the edge carries
+    // no flux), so phase 4 reads its sides' changes where its faces were.
+    // A tile without a wet cell has no flux across its interior sides and
+    // walls: only its other boundary sides are formed, and phase 4 takes
+    // zero for the rest.
+    tile_wet = __syncthreads_or(t < T + H && !F.dry);
+    const int2 eh = tl.ehdr[blockIdx.x];
+    const int n_int = eh.y & 0xFFFF, n_all = n_int + (eh.y >> 16);
+    if (tile_wet || n_all > n_int) {
+    for (int e = (tile_wet ? 0 : n_int) + t; e < n_all; e += CAP) {
+      const uint64_t r = tl.erec[eh.x + e];
+      const int a = (int)(r & 0xFFFF), ja = (int)(r >> 16) & 3;
+      const uint32_t bits = (uint32_t)(r >> 18) & ((1u << CM_SLOT_BITS) - 1);
+      const uint32_t kind = bits & CMS_KIND;
+      const int sa = ja * CAP + a;
+      const bool dry_a = L.dry[a];
+      double d0 = ZERO, d1 = ZERO, d2 = ZERO;
+      if (kind == SK_INTERIOR) {
+        const int k = (int)(r >> 36) & 0xFFFF, sk = ((bits >> CMS_NL) & 3) * CAP + k;
+        Nbr nb;
+        nb.dry = L.dry[k];
+        bool formed = false;
+        if (!(dry_a && nb.dry)) {
+          nb.fh = L.fh[sk]; nb.fu = L.fu[sk]; nb.fv = L.fv[sk]; nb.z = L.z[k]; nb.h = L.h[k];
+          formed = interior_flux(o, bits, side_dyn<CAP>(L, a, ja), dry_a, L.h[a], L.z[a], L.fh[sa],
+                                 L.fu[sa], L.fv[sa], nb, d0, d1, d2);
+        }
+        if (k < T) {
+          L.fh[sk] = formed ? -d0 : ZERO; L.fu[sk] = formed ? -d1 : ZERO; L.fv[sk] = formed ? -d2 : ZERO;
+        }
+      } else if (!(dry_a && kind == SK_WALL)) {
+        const int i = c0 + a;
+        const int va = g.cvtx[ja * N + i], vb = g.cvtx[(ja == 2 ? 0 : ja + 1) * N + i];
+        const double ha = S.hvtx[va], hb2 = S.hvtx[vb];
+        const bool rev = (bits >> CMS_REV) & 1;
+        boundary_flux(o, ri, S.sq, S.sh, s_at, s_null, s_skip, bits, side_dyn<CAP>(L, a, ja),
+                      L.fh[sa], L.fu[sa], L.fv[sa], L.h[a], dry_a, rev ? hb2 : ha, rev ? ha : hb2,
+                      d0, d1, d2);
+      }
+      L.fh[sa] = d0; L.fu[sa] = d1; L.fv[sa] = d2;
+    }
+    __syncthreads();
+    }
+  } else {
+    __syncthreads();
+  }
   }  // MODE != 2
 
   // 4. update of the tile cells
@@ -1436,6 +1502,18 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     // the sides in the reference's accumulation order (invisc_fluxes:
     // interior edges by number, walls, boundary routines), each added to phi
     // as it is formed
+    if constexpr (EDGES) {
+#pragma unroll
+      for (int tt = 0; tt < 3; ++tt) {
+        const int j = (int)(meta >> (2 * tt)) & 3;
+        const uint32_t kind = slot_bits(meta, j) & CMS_KIND;
+        if (tile_wet || (kind != SK_INTERIOR && kind != SK_WALL)) {
+          p0 = p0 + L.fh[j * CAP + t]; p1 = p1 + L.fu[j * CAP + t]; p2 = p2 + L.fv[j * CAP + t];
+        } else {
+          p0 = p0 + ZERO; p1 = p1 + ZERO; p2 = p2 + ZERO;
+        }
+      }
+    } else {
 #pragma unroll 1  // one side at a time: fewer live registers
     for (int tt = 0; tt < 3; ++tt) {
       const int j = (int)(meta >> (2 * tt)) & 3;
@@ -1448,6 +1526,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       if (!none) side_flux(bits, side_dyn<CAP>(L, t, j), j, d0, d1, d2);
       p0 = p0 + d0; p1 = p1 + d1; p2 = p2 + d2;
     }
+    }  // EDGES
     if (o.friction == 1) {
       // fric_terms.f90:24-98 (explicit Manning), after the edge fluxes
       double r = dry_i ? ZERO : G * si.cd * si.cd / pow_one_third(hi);
@@ -2529,6 +2608,8 @@ void setup_device(storm_fvt* C) {
   tl.hrec = reinterpret_cast<const int4*>(dupload(own, P.t_hrec));
   tl.hmeta = dupload(own, P.t_hmeta);
   tl.cln = dupload(own, P.t_cln);
+  tl.ehdr = reinterpret_cast<const int2*>(dupload(own, P.t_ehdr));
+  tl.erec = dupload(own, P.t_erec);
   tl.nt = P.nt;
   {
     std::vector<RInfo> ri(P.nr);

@@ -22,22 +22,33 @@ constexpr double SMALL = (double)1.0e-9f;
 // h**one_third of the Manning friction factor (fric_terms_impl.f90:36,
 // fric_terms.f90:37), correctly rounded.  The exponent is fl(1/3) =
 // 1/3 - 2^-54/3, so h**fl(1/3) = cbrt(h) * (1 - (2^-54/3) ln h) to far below
-// an ulp.  cbrt(h) is refined by one Newton step whose residual c^3 - h is
-// formed with exact products (fma), which leaves the cube root good to about
-// 2^-100; the exponent's correction and the Newton step are added to it in one
+// an ulp.  The cube root starts from single precision: h = m 2^(3q) with m in
+// [0.5, 4), cbrt(m) = 2^(log2(m) / 3) from the hardware's single-precision
+// log2 and exp2 (relative error near 1e-7), one plain Newton step in double
+// (error near 1e-14), then one Newton step whose residual c^3 - h is formed
+// with exact products (fma), which leaves the root good to about 2^-90; the
+// exponent's correction (its ln h from the same log2, good to 1e-7, which is
+// ample for a term of 2^-54 relative size) and that step are added in one
 // final rounding.  The result is the correctly rounded value on all but
-// inputs within about 2^-48 ulp of a rounding midpoint.  A libm pow is not
+// inputs within about 2^-20 ulp of a rounding midpoint.  A libm pow is not
 // correctly rounded: glibc's (0.52 ulp bound) differs on about 8 inputs in
 // 10,000, the device's own more often, and the step amplifies such one-ulp
 // differences over thousands of steps.  h <= 0 or not finite: plain pow.
 SD double pow_one_third(double h) {
   if (!(h > 0.0 && h <= DBL_MAX)) return pow(h, 1.0 / 3.0);
-  const double c = cbrt(h);
+  int e;
+  const double m = frexp(h, &e);               // h = m 2^e, m in [0.5, 1)
+  const int q = (e >= 0 ? e : e - 2) / 3;      // floor(e / 3)
+  const double mr = ldexp(m, e - 3 * q);       // h = mr 2^(3q), mr in [0.5, 4)
+  const float lf = __builtin_amdgcn_logf((float)mr);  // log2(mr)
+  double c = ldexp((double)__builtin_amdgcn_exp2f(lf * (1.0f / 3.0f)), q);
+  c = c - (c * c * c - h) / (3.0 * (c * c));
   const double p = c * c, pe = fma(c, c, -p);   // c^2 = p + pe exactly
-  const double q = p * c, qe = fma(p, c, -q);   // p c = q + qe exactly
-  const double r = (q - h) + (qe + pe * c);     // c^3 - h
+  const double pc = p * c, qe = fma(p, c, -pc); // p c = pc + qe exactly
+  const double r = (pc - h) + (qe + pe * c);    // c^3 - h
   const double lo = -r / (3.0 * p);             // Newton correction
-  const double t = -1.8503717077085943e-17 * log(h);  // -(1/3 - fl(1/3)) ln h
+  const double lnh = ((double)lf + 3.0 * q) * 0.6931471805599453;
+  const double t = -1.8503717077085943e-17 * lnh;  // -(1/3 - fl(1/3)) ln h
   return c + (lo + c * t);
 }
 

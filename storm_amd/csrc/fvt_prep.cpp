@@ -1467,6 +1467,36 @@ int build_tiles(Prep& S, int cap) {
     h[6] = b;
     h[7] = tile_reach[b];
   }
+  // per block, the sides of its tile whose fluxes are formed once each:
+  // interior edges first (owned-owned under the lower local id), then the
+  // boundary sides, so that a wave mostly runs one kind
+  S.t_ehdr.assign(2 * (size_t)nt, 0);
+  S.t_erec.clear();
+  for (int blk = 0; blk < nt; ++blk) {
+    const int b = S.t_order[blk];
+    const int c0 = S.t_ptr[b], T = S.t_ptr[b + 1] - c0;
+    S.t_ehdr[2 * (size_t)blk] = (int)S.t_erec.size();
+    int n_int = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 1) n_int = (int)S.t_erec.size() - S.t_ehdr[2 * (size_t)blk];
+      for (int a = 0; a < T; ++a) {
+        const int c = c0 + a;
+        for (int j = 0; j < 3; ++j) {
+          const uint64_t bits = (S.cmeta[c] >> (CM_SLOT0 + CM_SLOT_BITS * j)) & ((1u << CM_SLOT_BITS) - 1);
+          const bool interior = (bits & CMS_KIND) == SK_INTERIOR;
+          if (interior != (pass == 0)) continue;
+          uint64_t k = 0;
+          if (interior) {
+            k = (S.t_cln[c] >> (16 * j)) & 0xFFFFu;
+            if (k < (uint64_t)T && k < (uint64_t)a) continue;  // listed under the lower id
+          }
+          S.t_erec.push_back((uint64_t)a | ((uint64_t)j << 16) | (bits << 18) | (k << 36));
+        }
+      }
+    }
+    const int n_bnd = (int)S.t_erec.size() - S.t_ehdr[2 * (size_t)blk] - n_int;
+    S.t_ehdr[2 * (size_t)blk + 1] = n_int | (n_bnd << 16);  // both below 3 * cap
+  }
   const size_t nh = S.t_hcell.size();
   S.t_hrec.assign(8 * nh, 0);
   S.t_hmeta.assign(nh, 0);

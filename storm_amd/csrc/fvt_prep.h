@@ -130,6 +130,16 @@ struct Prep {
   // and its metadata word
   std::vector<int> t_hdr, t_hrec;
   std::vector<uint64_t> t_hmeta;
+  // Per block, the sides whose fluxes the stage kernel forms once each
+  // (2 ints per block {first record, interior count | boundary count << 16};
+  // one uint64 per record): every
+  // interior edge of the tile with at least one owned cell, then every
+  // boundary side of an owned cell.  Record: bits 0-15 owned local id a,
+  // 16-17 its side ja, 18-35 the slot bits of (a, ja), 36-51 the local id of
+  // the neighbour across an interior side (an interior edge between two owned
+  // cells appears once, under the lower id).
+  std::vector<int> t_ehdr;
+  std::vector<uint64_t> t_erec;
   // Residual smoothing (opts rsavg): the Gauss-Seidel sweep of
   // solverFVT.f90:244-255 as levels of independent cells.  Level l holds
   // rs_cell[rs_lptr[l] .. rs_lptr[l+1]) (internal ids, ascending); reach r

@@ -1,6 +1,7 @@
 // Test infrastructure only: exposes the shared-denominator quotient of
-// storm_amd/csrc/fvt_math.h (recip/qdiv) next to the plain division, for
-// tests/test_gpu_qdiv.py.  Built by __graft_entry__.build() into
+// storm_amd/csrc/fvt_math.h (recip/qdiv) next to the plain division, and the
+// friction factor's pow_one_third, for tests/test_gpu_qdiv.py.  Built by
+// __graft_entry__.build() into
 // tests/libqdiv_probe.so.
 #include <hip/hip_runtime.h>
 
@@ -45,5 +46,27 @@ extern "C" int qdiv_probe(int n, const double* num, const double* den, double* o
               hipMemcpy(ref, dref, b3, hipMemcpyDeviceToHost) != hipSuccess))
     rc = -4;
   (void)hipFree(dn); (void)hipFree(dd); (void)hipFree(dout); (void)hipFree(dref);
+  return rc;
+}
+
+__global__ void k_pow13(int n, const double* h, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pow_one_third(h[i]);
+}
+
+// out[i] = pow_one_third(h[i]) on the device
+extern "C" int pow13_probe(int n, const double* h, double* out) {
+  double *dh = nullptr, *dout = nullptr;
+  if (n <= 0) return 0;
+  const size_t b1 = sizeof(double) * (size_t)n;
+  if (hipMalloc(&dh, b1) != hipSuccess || hipMalloc(&dout, b1) != hipSuccess) return -1;
+  int rc = 0;
+  if (hipMemcpy(dh, h, b1, hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+  if (!rc) {
+    hipLaunchKernelGGL(k_pow13, dim3((n + 255) / 256), dim3(256), 0, 0, n, dh, dout);
+    if (hipDeviceSynchronize() != hipSuccess) rc = -3;
+  }
+  if (!rc && hipMemcpy(out, dout, b1, hipMemcpyDeviceToHost) != hipSuccess) rc = -4;
+  (void)hipFree(dh); (void)hipFree(dout);
   return rc;
 }

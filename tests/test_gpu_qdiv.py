@@ -2,7 +2,10 @@
 against the plain IEEE division on the device, bit for bit: random operands
 over the whole exponent range, operands whose quotient is near overflow or
 underflow (where v_div_scale scales the denominator and qdiv must fall back),
-subnormals, zeros, infinities and NaNs."""
+subnormals, zeros, infinities and NaNs.  Also the device's pow_one_third
+(the implicit friction's h**(1/3)): correctly rounded against 60-digit
+arithmetic over the depths the step meets, and equal to the Python
+restatement of its arithmetic in tests/test_pow_one_third.py."""
 import ctypes as C
 import os
 
@@ -66,3 +69,23 @@ def test_qdiv_is_bitwise_the_plain_division():
     hb = host.view(np.uint64)
     agree = (b == hb) | (np.isnan(ref) & np.isnan(host))
     assert agree.all(), "%d device quotients differ from the host's" % (~agree).sum()
+
+
+def test_device_pow_one_third_is_correctly_rounded():
+    import math
+    from decimal import Decimal, getcontext
+    from test_pow_one_third import pow_one_third as restated
+    lib = C.CDLL(LIB)
+    lib.pow13_probe.restype = C.c_int
+    rng = np.random.default_rng(9)
+    h = np.exp(rng.uniform(math.log(1e-4), math.log(200.0), 4000))
+    h = np.concatenate([h, [1e-30, 1e30, 1.0, 8.0, 27.0, 0.001, 1e-3 * (1 + 2 ** -52)]])
+    out = np.zeros_like(h)
+    dp = C.POINTER(C.c_double)
+    assert lib.pow13_probe(h.shape[0], h.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+    getcontext().prec = 60
+    third = Decimal(1.0 / 3.0)
+    for x, y in zip(h, out):
+        want = float((Decimal(float(x)).ln() * third).exp())
+        assert y == want, (x, y, want)
+        assert y == restated(float(x)), x

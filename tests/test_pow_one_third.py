@@ -1,9 +1,9 @@
 """The correctly rounded h**(1/3) of the device friction factor
 (fvt_math.h pow_one_third), restated in Python with exact products (Dekker's
 two-product in place of fma) and checked against 60-digit arithmetic: every
-sample is correctly rounded, whatever the starting cube root's last bits,
-and it differs from the host libm's pow only where that pow is off by more
-than half an ulp."""
+sample is correctly rounded, from the device's single-precision seed and
+whatever the starting cube root's last bits, and it differs from the host
+libm's pow only where that pow is off by more than half an ulp."""
 import math
 from decimal import Decimal, getcontext
 
@@ -25,15 +25,29 @@ def _two_prod_err(a, b, p):
     return ((ah * bh - p) + ah * bl + al * bh) + al * bl
 
 
+def _seed(h):
+    """The device's start: h = mr 2^(3q), mr in [0.5, 4); single-precision
+    log2(mr), cbrt(mr) = 2^(log2(mr)/3) in single precision, one plain
+    Newton step in double."""
+    m, e = math.frexp(h)
+    q = e // 3
+    mr = math.ldexp(m, e - 3 * q)
+    lf = np.log2(np.float32(mr))
+    c = math.ldexp(float(np.exp2(np.float32(lf * np.float32(1.0 / 3.0)))), q)
+    c = c - (c * c * c - h) / (3.0 * (c * c))
+    return c, float(lf) + 3.0 * q
+
+
 def pow_one_third(h, c=None):
-    c = float(np.cbrt(h)) if c is None else c
+    c0, log2h = _seed(h)
+    c = c0 if c is None else c
     p = c * c
     pe = _two_prod_err(c, c, p)
     q = p * c
     qe = _two_prod_err(p, c, q)
     r = (q - h) + (qe + pe * c)
     lo = -r / (3.0 * p)
-    t = -1.8503717077085943e-17 * math.log(h)
+    t = -1.8503717077085943e-17 * (log2h * 0.6931471805599453)
     return c + (lo + c * t)
 
 
@@ -49,6 +63,7 @@ def test_correctly_rounded_from_any_nearby_cube_root():
         h = float(h)
         ex = _exact(h)
         want = float(ex)  # Decimal -> float rounds to nearest
+        assert pow_one_third(h) == want, h  # from the device's own seed
         c0 = float(np.cbrt(h))
         for k in (-2, -1, 0, 1, 2):  # start a few ulps away
             c = c0
