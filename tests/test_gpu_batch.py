@@ -62,6 +62,10 @@ def test_hdls_batch_equals_separate_runs():
 
 
 def test_batch_steady_reaches_retire_independently():
+    """Steady reaches of one batch stop at their own steps while the others
+    go on (the stage buffers keep swapping roles around a stopped reach):
+    each reach's step count, time, residual, finalized state, vertex fields,
+    phi and last-stage gradients equal its run alone, bit for bit."""
     # reach 3 of this generator blows up in steady mode at CFL 0.3 (NaN
     # residual, in the oracle too), so it is left out here
     rs = _reaches(ks=(0, 1, 2, 4), steady=True)
@@ -71,11 +75,21 @@ def test_batch_steady_reaches_retire_independently():
     b.run(3000)
     it, tt, rm, dn = b.status()
     assert dn.all()
+    gx, gy = b.gradients()
+    b.finalize()
+    parts = b.split(b.state())
     for r, c in enumerate(rs):
         s = fvt.FVTSolver(c, opts)
         i1, t1, r1 = s.run(3000)
         assert i1 == it[r] and t1 == tt[r] and r1 == rm[r]
+        sx, sy = s.gradients()
+        s.finalize()
+        solo = s.state()
         s.close()
+        a, e = b.cell_ptr[r], b.cell_ptr[r + 1]
+        assert np.array_equal(sx, gx[a:e]) and np.array_equal(sy, gy[a:e]), r
+        for k, v in solo.items():
+            assert np.array_equal(v, parts[r][k]), (r, k)
     assert len(set(it.tolist())) > 1  # they converged at different steps
     b.close()
 
