@@ -46,6 +46,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 B_STEP_RK3 = 1248.0          # algorithmic bytes per cell-update, SSP-RK3
+B_STAGE_RK3 = 384.0          # algorithmic bytes per cell-stage, SSP-RK3 average
 PEAK_HBM = 8.0e12            # B/s, MI355X
 METRIC = "FP64 cell-updates/s + HBM % roofline; time-to-steady warm vs cold"
 
@@ -92,15 +93,34 @@ def cpu_baseline(case, steps):
     return case.n_elems * steps / t, t
 
 
+KERNEL_SOURCES = ("storm_amd/csrc/fvt_device.hip", "storm_amd/csrc/fvt_math.h",
+                  "storm_amd/csrc/fvt_prep.cpp", "storm_amd/csrc/fvt_prep.h")
+
+
+def kernel_source_hash():
+    """SHA-1 (first 16 hex digits) of the sources the step's kernels are
+    built from: a traffic figure measured on other sources is not reported."""
+    import hashlib
+    h = hashlib.sha1()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profiles(workload):
+    """HBM bytes of one step as rocprofv3's FETCH_SIZE/WRITE_SIZE counted
+    them (profiles/traffic_meander2m.json, one entry per workload), when that
+    measurement was taken on the current kernel sources; else None."""
     path = os.path.join(ROOT, "profiles", "traffic_meander2m.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("workload") == workload:
-            return d.get("bytes_per_step")
+        e = d.get("workloads", {}).get(workload)
+        if e and e.get("kernel_sources") == kernel_source_hash():
+            return e.get("bytes_per_step")
     except Exception:
         return None
     return None
@@ -226,7 +246,14 @@ def main():
                      "traffic": traffic,
                      "kernel": "one SSP-RK3 time step (all its launches)",
                      "algorithmic_bytes_per_launch": B_STEP_RK3 * n,
-                     "avg_launch_ms": 1e3 * step_s},
+                     "avg_launch_ms": 1e3 * step_s,
+                     # the three stages' 384 B per cell-stage over the whole
+                     # step's time (friction, dt and residual counted as
+                     # overhead of the stages)
+                     "frac_stage": B_STAGE_RK3 * 3 * n / step_s / PEAK_HBM,
+                     # measured HBM bytes over the step time
+                     "frac_traffic": (traffic / step_s / PEAK_HBM) if traffic else None,
+                     "kernel_sources": kernel_source_hash()},
     }
     if not args.no_cpu:
         cpu_rate, cpu_t = cpu_baseline(case, args.cpu_steps)
@@ -518,6 +545,8 @@ def warmstart(args):
             init = [W.cold_state(c) for c in reaches]
         elif mode == "plane":
             init = [W.baseline_state(c) for c in reaches]
+        elif mode == "gvf":
+            init = [W.gvf_state(c) for c in reaches]
         elif mode in ("learned", "learned_surface"):
             # learned_surface: the network's water surface with the plane's
             # uniform discharge (warmstart.decode)
@@ -564,7 +593,8 @@ def warmstart(args):
         "config": {"workload": "warmstart-%dx%d" % (n_r, args.size), "rstop": args.rstop,
                    "scheme": "HLL first order, SSP-RK2, steady local dt CFL 0.9, stop RMS dh",
                    "checkpoint": os.path.relpath(ckpt, ROOT), "generate_s": round(gen_s, 1)},
-        "cold": res.get("cold"), "plane": res.get("plane"), "learned": res.get("learned"),
+        "cold": res.get("cold"), "plane": res.get("plane"), "gvf": res.get("gvf"),
+        "learned": res.get("learned"),
         "learned_surface": res.get("learned_surface"),
         "plane_over_cold": wall("cold") / wall("plane") if wall("cold") and wall("plane") else None,
         "h_rms_learned_vs_cold_final_m": float(np.median(dd)) if dd else None,
