@@ -146,9 +146,9 @@ def main():
     ap.add_argument("--size", type=int, default=200000, help="warmstart: triangles per reach")
     ap.add_argument("--rstop", type=float, default=1e-6, help="warmstart: RMS dh tolerance")
     ap.add_argument("--checkpoint", default="", help="warmstart: trained network (.npz)")
-    ap.add_argument("--start", choices=["cold", "plane", "learned"], default="cold",
-                    help="ensemble: initial state (plane and learned: storm_amd/warmstart.py, "
-                         "their set-up inside the timed region)")
+    ap.add_argument("--start", choices=["cold", "plane", "gvf", "learned"], default="cold",
+                    help="ensemble: initial state (plane, gvf (the 1-D backwater profile) and "
+                         "learned: storm_amd/warmstart.py, their set-up inside the timed region)")
     ap.add_argument("--accuracy", action="store_true",
                     help="warmstart: time to a fixed accuracy against tightly converged "
                          "states (on --reaches reaches, default 8) instead of the RMS dh test")
@@ -277,7 +277,7 @@ def main():
 def ensemble(args):
     """Time to steady state of a batched ensemble from the cold start
     (warmstart.cold_state: still water covering every cell), or (--start)
-    from the plane guess or the learned warm start."""
+    from the plane guess, the backwater profile or the learned warm start."""
     from storm_amd import cases, fvt, tune
     t0 = time.perf_counter()
     reaches = [cases.ensemble_reach(k, steady=True) for k in range(args.reaches)]
@@ -298,6 +298,8 @@ def ensemble(args):
         from storm_amd import warmstart as W
         if args.start == "plane":
             init = [W.baseline_state(c) for c in reaches]
+        elif args.start == "gvf":
+            init = [W.gvf_state(c) for c in reaches]
         else:
             ws = W.WarmStart.load(args.checkpoint or W.DEFAULT_CHECKPOINT)
             init = ws.predict(reaches)
@@ -547,12 +549,13 @@ def warmstart(args):
             init = [W.baseline_state(c) for c in reaches]
         elif mode == "gvf":
             init = [W.gvf_state(c) for c in reaches]
-        elif mode in ("learned", "learned_surface"):
-            # learned_surface: the network's water surface with the plane's
-            # uniform discharge (warmstart.decode)
+        elif mode in ("learned", "learned_discharge"):
+            # learned: the network's water surface with uniform discharge
+            # along the axis (WarmStart.predict's default); learned_discharge:
+            # its discharge outputs too (warmstart.decode)
             ws = W.WarmStart.load(ckpt)
             init = ws.predict(reaches,
-                              discharge="uniform" if mode == "learned_surface" else "learned")
+                              discharge="learned" if mode == "learned_discharge" else "uniform")
             ws.close()
         pred_s = time.perf_counter() - t0
         out, _ = W.steady_states(reaches, opts, args.max_steps, chunk=1000, init=init,
@@ -566,8 +569,8 @@ def warmstart(args):
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
                          steps_sum=int(steps.sum()), steps=steps.tolist(),
                          cell_updates=float(sum(c.n_elems * it for c, it in zip(reaches, steps))))
-        if mode in ("learned", "learned_surface"):
-            # prediction error against the state the solver converged to
+        if mode != "cold":
+            # the start's error against the state the solver converged to
             eh, eq = [], []
             for (st, _), (h, u, v) in zip(out, init):
                 if st is None:
@@ -595,7 +598,7 @@ def warmstart(args):
                    "checkpoint": os.path.relpath(ckpt, ROOT), "generate_s": round(gen_s, 1)},
         "cold": res.get("cold"), "plane": res.get("plane"), "gvf": res.get("gvf"),
         "learned": res.get("learned"),
-        "learned_surface": res.get("learned_surface"),
+        "learned_discharge": res.get("learned_discharge"),
         "plane_over_cold": wall("cold") / wall("plane") if wall("cold") and wall("plane") else None,
         "h_rms_learned_vs_cold_final_m": float(np.median(dd)) if dd else None,
     }

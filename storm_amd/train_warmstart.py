@@ -5,7 +5,8 @@ generates.
 
 1. Training reaches k = 0..R-1 come from the ensemble generator at 20k-100k
    triangles (seeds 5005+k).  The batched solver runs them together, from
-   the plane guess of storm_amd.warmstart.baseline_state, to RMS dh <= rstop
+   the 1-D backwater profile of storm_amd.warmstart.gvf_state (or, with
+   --start plane, the plane guess of baseline_state), to RMS dh <= rstop
    (default 1e-8: at 1e-6 the RMS-change test still leaves centimetre errors
    in h).  A reach that diverges or does not converge is left out.
 2. Features and targets (storm_amd.warmstart.frame / targets); a held-out
@@ -34,7 +35,7 @@ def log(msg):
 
 
 def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int,
-             gen_time: float = 0.0, first: int = 0):
+             gen_time: float = 0.0, first: int = 0, start: str = "gvf"):
     """Steady states of training reaches first..first+n-1 (None where not
     converged).  gen_time > 0: no further batch is started once that many
     seconds have passed, so fewer than n reaches may come back."""
@@ -46,7 +47,7 @@ def generate(n_reaches: int, gen_batch: int, rstop: float, max_steps: int,
             break
         ks = list(range(s, min(first + n_reaches, s + gen_batch)))
         rs = [W.training_reach(k) for k in ks]
-        init = [W.baseline_state(c) for c in rs]
+        init = [W.gvf_state(c) if start == "gvf" else W.baseline_state(c) for c in rs]
         out, wall = W.steady_states(rs, dict(rstop=rstop, nonfinite_stop=True), max_steps,
                                     chunk=5000, init=init, log=log)
         walls.append(wall)
@@ -82,12 +83,15 @@ def main(argv=None):
                     help="first training reach (a later phase takes reaches no earlier one saw)")
     ap.add_argument("--init", default="",
                     help="continue training this checkpoint (a later phase on new reaches)")
+    ap.add_argument("--start", choices=["gvf", "plane"], default="gvf",
+                    help="the solver's start in generation: the 1-D backwater profile "
+                         "(warmstart.gvf_state) or the plane (baseline_state)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
     t0 = time.perf_counter()
     states, steps, walls = generate(a.reaches, a.gen_batch, a.rstop, a.max_steps, a.gen_time,
-                                    a.first)
+                                    a.first, a.start)
     gen_s = time.perf_counter() - t0
     # keys: reach numbers of the converged steady states
     ks = [a.first + j for j, st in enumerate(states) if st is not None]
@@ -132,7 +136,8 @@ def main(argv=None):
             reaches_per_step=a.reaches_per_step, window=a.window, lr=a.lr,
             lr_final=a.lr_final, val_loss_initial=val0, val_loss=val1,
             val_loss_of_zero_output=zero,
-            generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)")
+            generator="storm_amd.warmstart.training_reach (seeds 5005+k, 20k-100k triangles)",
+            start=a.start)
         # a later phase keeps the record of the earlier ones
         phases = []
         if a.init:

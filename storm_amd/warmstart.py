@@ -22,7 +22,7 @@ against ``dist`` (the fitted bed slope S).  The frame assumes a reach whose
 axis is straight (the flumes of the ensemble generator); a curved reach
 would need a curvilinear coordinate in its place.
 
-Node features (16), set 2 (the default): bed relative to the outlet stage,
+Node features (16), set 2: bed relative to the outlet stage,
 bed relative to the fitted line, logs of Manning n, q, the outlet depth h_out
 (outlet stage over the fitted bed) and the fitted slope, distance to the
 outlet and to the inlet, inflow / outflow / wall flags, cell size, the
@@ -36,10 +36,21 @@ depends on as sums.  Edge features (16): the centroid offset in the reach
 frame and its length, the bed step and the step of the bed relative to the
 fitted line, a constant, ten zeros.
 
+Set 3 (the default) adds a non-learned base: the steady 1-D gradually
+varied flow over the fitted bed from the outlet stage (``gvf_depth``), with
+the Manning coefficient scaled by the steady mode's friction factor
+(``friction_multiplier``: in steady runs every cell's implicit friction acts
+with the local time step of the reach's last cell).  Its surface over the
+plane is the zero output and node feature 15, and the log of each cell's
+friction factor dt_last/dt_i at the edge's two cells are edge features 6
+and 7.
+
 Outputs, each scaled to order one:
-  y0 = (zeta - H - S*dist) / 0.05 m  water surface relative to the plane
-                                     through the outlet stage parallel to
-                                     the fitted bed
+  y0 = (zeta - H - S*dist - base) / 0.05 m
+                                     water surface relative to the base: the
+                                     plane through the outlet stage parallel
+                                     to the fitted bed (sets 1, 2), or the
+                                     backwater surface (set 3)
   y1 = (hu_e / q - 1) / 0.04         discharge along the axis
   y2 = (hv_e / q) / 0.04             discharge across the axis
 The scales weight a relative error of the depth (about 4 m) twice as much as
@@ -47,8 +58,9 @@ the same relative error of the discharge, whose steady values carry the
 inflow fixed point's tolerance of 0.5% (inflowQbyH.f90:120-121).
 The other 13 outputs are trained to zero.  ``decode`` turns them back into
 h = max(zeta - zc, 0), hu, hv and the u, v that storm_fvt_set_state takes.
-With all outputs zero the decode is the plane itself with uniform discharge
-(``baseline_state``), the non-learned guess the network improves on.
+With all outputs zero the decode is the base with uniform discharge: the
+plane (``baseline_state``) or the backwater start (``gvf_state``), the
+non-learned guesses the network improves on.
 """
 from __future__ import annotations
 
@@ -86,6 +98,9 @@ class Frame:
     a: float
     order: Optional[np.ndarray] = None   # cells along the axis (set by window)
     rank: Optional[np.ndarray] = None
+    # water surface of the zero output relative to the plane (feature set 3:
+    # the 1-D backwater profile's; 0 otherwise)
+    base: Optional[np.ndarray] = None
 
 
 def _string_geometry(case, b):
@@ -109,7 +124,7 @@ def _string_cells(case, nodes) -> np.ndarray:
     return np.unique(np.array(out, dtype=np.int64))
 
 
-FEATURES = 2  # node feature set of new networks
+FEATURES = 3  # node feature set of new networks
 
 
 def frame(case, features: int = FEATURES) -> Frame:
@@ -165,7 +180,7 @@ def frame(case, features: int = FEATURES) -> Frame:
         X[:, 5] = S * 1000.0
         X[:, 12] = is_in * q / D_SCALE
         X[:, 13] = is_out * (H - zc) / D_SCALE
-    elif features == 2:
+    elif features in (2, 3):
         h_out = H - a  # outlet stage over the fitted bed at the outlet (dist 0)
         if not (h_out > 0 and S > 0):
             raise ValueError("the warm start needs a positive outlet depth and bed slope")
@@ -193,8 +208,23 @@ def frame(case, features: int = FEATURES) -> Frame:
     EF[:, 3] = (zc[dst] - zc[src]) / Y0_SCALE
     EF[:, 4] = (za[dst] - za[src]) / Y0_SCALE
     EF[:, 5] = 1.0
-    return Frame(src, dst, X, EF, zc, dist, e, H, q,
-                 float(S), float(a))
+    fr = Frame(src, dst, X, EF, zc, dist, e, H, q, float(S), float(a))
+    fr.base = np.zeros(N)
+    if features == 3:
+        # log of each cell's steady friction factor dt_last / dt_i
+        # (friction_multiplier) at the edge's two cells
+        per = sum(np.hypot(case.x[tri[:, j]] - case.x[tri[:, (j + 1) % 3]],
+                           case.y[tri[:, j]] - case.y[tri[:, (j + 1) % 3]]) for j in range(3))
+        ap = area / per
+        lr = np.log(ap[-1] / ap)
+        EF[:, 6] = lr[src]
+        EF[:, 7] = lr[dst]
+        # the gradually varied flow's surface over the plane, as the zero
+        # output and as a node feature
+        n_eff = float(np.mean(case.cd)) * np.sqrt(friction_multiplier(case))
+        fr.base = (a + gvf_depth(fr, n_eff)) - H
+        X[:, 15] = fr.base / Y0_SCALE
+    return fr
 
 
 def targets(fr: Frame, h, qu, qv) -> np.ndarray:
@@ -202,7 +232,7 @@ def targets(fr: Frame, h, qu, qv) -> np.ndarray:
     N = h.shape[0]
     T = np.zeros((N, F_OUT), dtype=np.float64)
     zeta = h + fr.zc
-    T[:, 0] = (zeta - fr.H - fr.S * fr.dist) / Y0_SCALE
+    T[:, 0] = (zeta - fr.H - fr.S * fr.dist - fr.base) / Y0_SCALE
     T[:, 1] = ((qu * fr.e[0] + qv * fr.e[1]) / fr.q - 1.0) / Q_SCALE
     T[:, 2] = ((qv * fr.e[0] - qu * fr.e[1]) / fr.q) / Q_SCALE
     return T.astype(np.float32)
@@ -213,7 +243,7 @@ def decode(fr: Frame, y: np.ndarray, h_dry: float = 1e-3, discharge: str = "lear
     the learned water surface but takes the plane's uniform discharge along
     the axis in place of the learned one (outputs 1 and 2 read as zero)."""
     y = np.asarray(y, dtype=np.float64)
-    zeta = fr.H + fr.S * fr.dist + Y0_SCALE * y[:, 0]
+    zeta = fr.H + fr.S * fr.dist + fr.base + Y0_SCALE * y[:, 0]
     h = np.maximum(zeta - fr.zc, 0.0)
     if discharge not in ("learned", "uniform"):
         raise ValueError("discharge must be 'learned' or 'uniform'")
@@ -246,9 +276,81 @@ def cold_state(case):
 
 
 def baseline_state(case, h_dry: float = 1e-3):
-    """The decode of zero outputs: the plane through the outlet stage
-    parallel to the fitted bed, with uniform discharge along the axis."""
-    fr = frame(case)
+    """The decode of zero outputs of feature sets 1 and 2: the plane
+    through the outlet stage parallel to the fitted bed, with uniform
+    discharge along the axis."""
+    fr = frame(case, 2)
+    return decode(fr, np.zeros((case.n_elems, 3)), h_dry)
+
+
+G = 9.81
+
+
+def friction_multiplier(case) -> float:
+    """The steady mode's friction scale over the reach.  In steady runs the
+    implicit friction of every cell is applied with the local time step of
+    the reach's last cell (solverFVT.f90:260-265,336) while its fluxes take
+    the cell's own, so at the fixed point cell i's Manning term carries the
+    factor r_i = dt_last / dt_i.  With the flow's speed about the same in
+    every cell, r_i is the ratio of area-over-perimeter of the two cells.
+    The reach responds to its harmonic mean, 1 / mean(1 / r_i): the last
+    cell's area-over-perimeter over the reach's mean (on converged training
+    reaches the multiplier that best fits the 2-D surface was within 1.2 %
+    of it, where the plain mean of r_i overshoots by up to 3.5 %).  The
+    reference's last cell is the last one in input order."""
+    tri = case.tri
+    x, y = case.x[tri], case.y[tri]
+    area = 0.5 * np.abs((x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0])
+                        - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0]))
+    per = sum(np.hypot(x[:, j] - x[:, (j + 1) % 3], y[:, j] - y[:, (j + 1) % 3])
+              for j in range(3))
+    ap = area / per
+    return float(ap[-1] / np.mean(ap))
+
+
+def gvf_depth(fr: Frame, manning: float, n_steps: int = 4000) -> np.ndarray:
+    """Depth over the fitted bed along the reach axis of the steady 1-D
+    gradually varied flow with the frame's unit discharge q:
+    (1 - Fr^2) dh/dx = S - Sf, Fr^2 = q^2 / (g h^3), Sf = n^2 q^2 / h^(10/3)
+    (the depth-based Manning slope of the 2-D model's friction term), x
+    downstream.  Integrated upstream from the outlet depth H - a by the
+    classical fourth-order Runge-Kutta method on n_steps equal steps of the
+    axis; returns h at each cell's distance from the outlet.  Where the
+    profile would reach 1.1 times the critical depth it is held there (the
+    frame's subcritical assumption)."""
+    q, S = fr.q, fr.S
+    span = float(fr.dist.max()) if fr.dist.size else 0.0
+    hc = (q * q / G) ** (1.0 / 3.0)
+    h_floor = 1.1 * hc
+
+    def dh_ddist(h):
+        h = max(h, h_floor)
+        fr2 = q * q / (G * h ** 3)
+        sf = manning * manning * q * q / h ** (10.0 / 3.0)
+        return -(S - sf) / (1.0 - fr2)
+
+    s = np.linspace(0.0, max(span, 1e-9), n_steps + 1)
+    ds = s[1] - s[0]
+    h = np.empty_like(s)
+    h[0] = max(fr.H - fr.a, h_floor)
+    for k in range(n_steps):
+        y = h[k]
+        k1 = dh_ddist(y)
+        k2 = dh_ddist(y + 0.5 * ds * k1)
+        k3 = dh_ddist(y + 0.5 * ds * k2)
+        k4 = dh_ddist(y + ds * k3)
+        h[k + 1] = max(y + ds * (k1 + 2 * k2 + 2 * k3 + k4) / 6.0, h_floor)
+    return np.interp(fr.dist, s, h)
+
+
+def gvf_state(case, h_dry: float = 1e-3):
+    """A non-learned start from 1-D hydraulics: the water surface of the
+    gradually varied flow (gvf_depth with the reach's mean Manning n scaled
+    by the steady mode's friction factor, friction_multiplier) over the
+    fitted bed, with uniform discharge along the axis.  Where the plane
+    guess assumes normal flow throughout, this follows the backwater curve
+    the outlet stage sets."""
+    fr = frame(case, 3)  # zero outputs of set 3 decode to this start
     return decode(fr, np.zeros((case.n_elems, 3)), h_dry)
 
 
@@ -469,9 +571,14 @@ class WarmStart:
         return out
 
     def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3,
-                discharge: str = "learned"):
-        """Per case: (h, u, v) for storm_fvt_set_state (``decode``'s
-        discharge choice)."""
+                discharge: str = "uniform"):
+        """Per case: (h, u, v) for storm_fvt_set_state.  By default the
+        network's water surface with uniform discharge along the axis: a
+        steady flow carries the same discharge through every section, and
+        the network's cell-scale discharge errors would start transients
+        that the surface alone does not (64 held-out 200k reaches reached
+        the steady test in 18.4 s this way, 21.5 s with the network's
+        discharge outputs).  discharge="learned" decodes those too."""
         frs = [frame(c, self.features) for c in cases_]
         ys = self.predict_outputs(frs, batch)
         return [decode(fr, y, h_dry, discharge) for fr, y in zip(frs, ys)]
