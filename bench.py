@@ -297,9 +297,9 @@ def ensemble(args):
     if args.start != "cold":
         from storm_amd import warmstart as W
         if args.start == "plane":
-            init = [W.baseline_state(c) for c in reaches]
+            init = W.starts(reaches, "plane")
         elif args.start == "gvf":
-            init = [W.gvf_state(c) for c in reaches]
+            init = W.starts(reaches, "gvf")
         else:
             ws = W.WarmStart.load(args.checkpoint or W.DEFAULT_CHECKPOINT)
             init = ws.predict(reaches)
@@ -308,7 +308,7 @@ def ensemble(args):
                     np.concatenate([x[2] for x in init]))
     else:
         from storm_amd import warmstart as W
-        init = [W.cold_state(c) for c in reaches]
+        init = W.starts(reaches, "cold")
         b.set_state(np.concatenate([x[0] for x in init]), np.concatenate([x[1] for x in init]),
                     np.concatenate([x[2] for x in init]))
     init_s = time.perf_counter() - t0
@@ -449,7 +449,7 @@ def dambreak(args):
 
 def warmstart_accuracy(args):
     """Time to a fixed accuracy from each start.  The reference is each
-    reach's state converged to RMS dh <= 1e-9 (from the plane guess).  Each
+    reach's state converged to RMS dh <= 1e-9 (from the backwater start).  Each
     start then runs, with the RMS dh test off, until every reach's RMS error
     in h against its reference is at most 1 cm and then 1 mm; the state is
     read back every 500 steps, so step counts are multiples of 500.  The GPU
@@ -462,7 +462,7 @@ def warmstart_accuracy(args):
     reaches = [W.heldout_reach(k, args.size) for k in range(n_r)]
     t0 = time.perf_counter()
     ref, _ = W.steady_states(reaches, dict(rstop=1e-9, nonfinite_stop=True), 400000,
-                             chunk=5000, init=[W.baseline_state(c) for c in reaches],
+                             chunk=5000, init=W.starts(reaches, "gvf"),
                              log=lambda m: print("reference: %s" % m, file=sys.stderr, flush=True))
     ref_s = time.perf_counter() - t0
     keep = [r for r, st in enumerate(ref) if st[0] is not None]
@@ -472,9 +472,11 @@ def warmstart_accuracy(args):
     res = {}
     for mode in args.modes.split(","):
         if mode == "cold":
-            init = [W.cold_state(c) for c in reaches]
+            init = W.starts(reaches, "cold")
         elif mode == "plane":
-            init = [W.baseline_state(c) for c in reaches]
+            init = W.starts(reaches, "plane")
+        elif mode == "gvf":
+            init = W.starts(reaches, "gvf")
         else:
             ws = W.WarmStart.load(ckpt)
             init = ws.predict(reaches)
@@ -518,7 +520,7 @@ def warmstart_accuracy(args):
         "triangles_per_reach": args.size, "reference_s": ref_s,
         "data": "synthetic (storm_amd/warmstart.py heldout_reach, seeds 9009+k)",
         "config": {"workload": "warmstart-accuracy-%dx%d" % (len(reaches), args.size),
-                   "reference": "RMS dh <= 1e-9 from the plane guess",
+                   "reference": "RMS dh <= 1e-9 from the backwater start",
                    "checkpoint": os.path.relpath(ckpt, ROOT)},
         "results": res,
     }
@@ -544,11 +546,11 @@ def warmstart(args):
         t0 = time.perf_counter()
         init, pred_s = None, 0.0
         if mode == "cold":
-            init = [W.cold_state(c) for c in reaches]
+            init = W.starts(reaches, "cold")
         elif mode == "plane":
-            init = [W.baseline_state(c) for c in reaches]
+            init = W.starts(reaches, "plane")
         elif mode == "gvf":
-            init = [W.gvf_state(c) for c in reaches]
+            init = W.starts(reaches, "gvf")
         elif mode in ("learned", "learned_discharge"):
             # learned: the network's water surface with uniform discharge
             # along the axis (WarmStart.predict's default); learned_discharge:

@@ -275,6 +275,29 @@ def cold_state(case):
     return h, np.zeros(case.n_elems), np.zeros(case.n_elems)
 
 
+def _pool_map(fn, items):
+    """fn over items on a thread pool (the array work releases the GIL),
+    results in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    items = list(items)
+    if len(items) < 2:
+        return [fn(x) for x in items]
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1, len(items))) as ex:
+        return list(ex.map(fn, items))
+
+
+def frames(cases_: Sequence, features: int = FEATURES) -> List[Frame]:
+    """frame() of every case, built in parallel."""
+    return _pool_map(lambda c: frame(c, features), cases_)
+
+
+def starts(cases_: Sequence, kind: str) -> List:
+    """The non-learned starts of every case, built in parallel: "plane"
+    (baseline_state), "gvf" (gvf_state) or "cold" (cold_state)."""
+    fn = {"plane": baseline_state, "gvf": gvf_state, "cold": cold_state}[kind]
+    return _pool_map(fn, cases_)
+
+
 def baseline_state(case, h_dry: float = 1e-3):
     """The decode of zero outputs of feature sets 1 and 2: the plane
     through the outlet stage parallel to the fitted bed, with uniform
@@ -579,7 +602,7 @@ class WarmStart:
         that the surface alone does not (64 held-out 200k reaches reached
         the steady test in 18.4 s this way, 21.5 s with the network's
         discharge outputs).  discharge="learned" decodes those too."""
-        frs = [frame(c, self.features) for c in cases_]
+        frs = frames(cases_, self.features)
         ys = self.predict_outputs(frs, batch)
         return [decode(fr, y, h_dry, discharge) for fr, y in zip(frs, ys)]
 
