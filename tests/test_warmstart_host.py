@@ -108,3 +108,55 @@ def test_window_cut_and_loss_weights():
     # a window that takes the whole reach has no cut
     *_, w_all = W.window(fr, y, 0, c.n_elems, hops)
     assert w_all.all()
+
+
+def test_friction_multiplier_of_identical_cells_is_one():
+    """Every cell of an unjittered grid has the last cell's area over
+    perimeter, so the steady friction factor is exactly 1; jitter moves it."""
+    c = cases.flume(nx=20, ny=6, length=100.0, width=30.0, seed=5)
+    c.x = np.round(c.x / 5.0) * 5.0  # the grid before jitter (5 m spacing)
+    c.y = np.round(c.y / 5.0) * 5.0
+    assert abs(W.friction_multiplier(c) - 1.0) < 1e-12
+    assert abs(W.friction_multiplier(_reach()) - 1.0) > 1e-3
+
+
+def _gvf_frame(h_out, n=0.03, q=2.0, S=1e-3, span=1000.0):
+    dist = np.linspace(0.0, span, 501)
+    fr = W.Frame(src=None, dst=None, node=None, edge=None, zc=None, dist=dist,
+                 e=np.array([1.0, 0.0]), H=10.0 + h_out, q=q, S=S, a=10.0)
+    return fr
+
+
+def test_backwater_profile_closed_forms():
+    """gvf_depth against what fixes it: at the normal depth the flow stays
+    uniform; above it the M1 curve falls monotonically upstream towards the
+    normal depth; below it the M2 curve rises towards it; the profile obeys
+    (1 - Fr^2) dh/dx = S - Sf (x downstream) wherever it is sampled."""
+    n, q, S = 0.03, 2.0, 1e-3
+    hn = (n * q / np.sqrt(S)) ** 0.6  # n^2 q^2 / hn^(10/3) = S
+    h = W.gvf_depth(_gvf_frame(hn, n, q, S), n)
+    assert np.abs(h - hn).max() < 1e-9 * hn
+    for h_out in (hn + 0.3, hn - 0.2):
+        fr = _gvf_frame(h_out, n, q, S)
+        h = W.gvf_depth(fr, n)
+        d = np.diff(h)
+        if h_out > hn:
+            assert (d <= 0).all() and h[-1] > hn and abs(h[0] - h_out) < 1e-12
+        else:
+            assert (d >= 0).all() and h[-1] < hn and abs(h[0] - h_out) < 1e-12
+        # the ODE at interior samples (dist points upstream: dh/ddist = -dh/dx)
+        s = fr.dist
+        dh = (h[2:] - h[:-2]) / (s[2:] - s[:-2])
+        hm = h[1:-1]
+        rhs = -(S - n * n * q * q / hm ** (10.0 / 3.0)) / (1.0 - q * q / (9.81 * hm ** 3))
+        assert np.abs(dh - rhs).max() < 1e-4 * np.abs(rhs).max() + 1e-12
+
+
+def test_parallel_builders_equal_the_serial_ones():
+    cs = [cases.ensemble_reach(k, n_min=2000, n_max=4000, base_seed=91) for k in range(3)]
+    for kind, fn in (("plane", W.baseline_state), ("gvf", W.gvf_state), ("cold", W.cold_state)):
+        for a, b in zip(W.starts(cs, kind), [fn(c) for c in cs]):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), kind
+    for a, b in zip(W.frames(cs), [W.frame(c) for c in cs]):
+        assert np.array_equal(a.node, b.node) and np.array_equal(a.edge, b.edge)
+        assert np.array_equal(a.base, b.base)
