@@ -11,6 +11,9 @@ wet/dry fronts.  A step is one full SSP-RK3 time step of the whole reach
 resident in HBM when timing starts.
 
 Prints one JSON line.  `value` is cell-updates/s (triangles x steps / s).
+`wet_valley` times the same reach, the same way, with the whole valley under
+water (every cell wet: each forms its gradients, limiter, faces and fluxes,
+where a dry cell skips them), the stage kernel's harder case.
 
 `--workload ensemble` instead runs the batched ensemble (storm_amd/cases.py
 `ensemble_reach`: reaches of log-uniform size in [5k, 200k] triangles, own
@@ -20,9 +23,9 @@ apart) in one context, with the autotuned launch configuration, and reports
 the wall time to steady and the cell-updates/s.
 
 `--workload warmstart` times held-out reaches (200k triangles each) to steady
-state from a cold start, from the plane guess, and from the learned warm
-start (storm_amd/warmstart.py), and reports the cold / learned wall-time
-ratio.
+state from a cold start, from the plane guess, from the 1-D backwater start
+and from the learned warm start (storm_amd/warmstart.py), and reports the
+cold / learned wall-time ratio.
 `roofline` prices the step with the algorithmic bytes of one cell-update of
 SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
 residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
@@ -133,6 +136,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-wet", action="store_true",
+                    help="meander: skip the wet-valley line (the same reach with every cell wet)")
     ap.add_argument("--graph", type=int, default=0,
                     help="1: replay whole steps from a captured HIP graph")
     ap.add_argument("--block", type=int, default=0,
@@ -152,7 +157,7 @@ def main():
     ap.add_argument("--accuracy", action="store_true",
                     help="warmstart: time to a fixed accuracy against tightly converged "
                          "states (on --reaches reaches, default 8) instead of the RMS dh test")
-    ap.add_argument("--modes", default="cold,plane,learned",
+    ap.add_argument("--modes", default="cold,plane,gvf,learned",
                     help="warmstart: starts to time (the ratio needs cold and learned)")
     ap.add_argument("--wet", type=float, default=None,
                     help="meander: start with the whole valley under water, this many metres "
@@ -255,6 +260,26 @@ def main():
                      "frac_traffic": (traffic / step_s / PEAK_HBM) if traffic else None,
                      "kernel_sources": kernel_source_hash()},
     }
+    if args.wet is None and not args.no_wet and not extra:
+        # the same reach with every cell wet (whole valley under water, 4 m
+        # above the floor), timed the same way: the stage kernel's harder case
+        solver.close()
+        wcase = cases.meander(wet=4.0)
+        ws = fvt.FVTSolver(wcase, dict(istop=10 ** 9, graph=args.graph, block=args.block))
+        ws.run(args.warmup)
+        t0 = time.perf_counter()
+        ws.run(args.steps)
+        wwall = time.perf_counter() - t0
+        wstep = ws.last_run_seconds() / args.steps
+        wtraffic = traffic_from_profiles("meander-2M-rk3-wet")
+        out["wet_valley"] = {
+            "workload": "meander-2M-rk3-wet", "ms_per_step": 1e3 * wwall / args.steps,
+            "value": n * args.steps / wwall, "unit": "cell-updates/s",
+            "wet_fraction": round(float((ws.state()["h"] > case.opts["h_dry"]).mean()), 4),
+            "roofline_frac": B_STEP_RK3 * n / wstep / PEAK_HBM,
+            "roofline_frac_stage": B_STAGE_RK3 * 3 * n / wstep / PEAK_HBM,
+            "traffic": wtraffic, "avg_launch_ms": 1e3 * wstep}
+        ws.close()
     if not args.no_cpu:
         cpu_rate, cpu_t = cpu_baseline(case, args.cpu_steps)
         port = {"value": cpu_rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
