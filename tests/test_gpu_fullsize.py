@@ -60,6 +60,16 @@ def test_meander_2m():
     _compare(c, 8)
 
 
+def test_meander_2m_whole_valley_wet():
+    """The same reach with every cell wet (the bench's wet-valley line): every
+    cell forms its gradients, limiter, faces and fluxes, the path a dry cell
+    skips."""
+    c = cases.meander(wet=4.0)
+    assert (c.h > c.opts["h_dry"]).all()
+    print("[gpu fullsize] meander, whole valley wet, N=%d, 6 steps" % c.n_elems, flush=True)
+    _compare(c, 6)
+
+
 def test_dambreak_1m():
     c = cases.dambreak()
     print("[gpu fullsize] dam break N=%d, 30 steps" % c.n_elems, flush=True)
