@@ -26,6 +26,9 @@ the wall time to steady and the cell-updates/s.
 state from a cold start, from the plane guess, from the 1-D backwater start
 and from the learned warm start (storm_amd/warmstart.py), and reports the
 cold / learned wall-time ratio.
+`--workload warmstart-train` times AdamW updates of the warm-start network
+(BF16 MFMA, FP32 accumulation) on batches of 32 windows of 4,096 cells cut
+from 64 training reaches, against synthetic targets.
 `roofline` prices the step with the algorithmic bytes of one cell-update of
 SSP-RK3 (1,248 B: three stages of 384 B average plus 96 B of friction, dt and
 residual) against 8 TB/s; one "launch" is one time step.  `cpu_baseline`
@@ -142,7 +145,8 @@ def main():
                     help="1: replay whole steps from a captured HIP graph")
     ap.add_argument("--block", type=int, default=0,
                     help="stage-kernel tile capacity (256, 384 or 512; 0 = library default)")
-    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart", "dambreak", "flume"],
+    ap.add_argument("--workload", choices=["meander", "ensemble", "warmstart", "warmstart-train",
+                                           "dambreak", "flume"],
                     default="meander")
     ap.add_argument("--t-end", type=float, default=300.0, help="dambreak: simulated seconds")
     ap.add_argument("--reaches", type=int, default=256,
@@ -188,6 +192,9 @@ def main():
             warmstart_accuracy(args)
         else:
             warmstart(args)
+        return
+    if args.workload == "warmstart-train":
+        warmstart_train(args)
         return
     if args.workload == "dambreak":
         dambreak(args)
@@ -630,6 +637,39 @@ def warmstart(args):
         "h_rms_learned_vs_cold_final_m": float(np.median(dd)) if dd else None,
     }
     print(json.dumps(out))
+
+
+def warmstart_train(args):
+    """Training rate of the warm-start network: `--steps` AdamW updates (after
+    `--warmup` untimed ones) on batches joining 32 windows of 4,096 cells cut
+    from 64 training reaches (131,072 nodes, about 390,000 directed edges per
+    batch), with random targets; each update runs the forward pass, the
+    backward pass and the optimizer on the device, its batch assembled on a
+    host thread meanwhile."""
+    from storm_amd import warmstart as W
+    import torch
+    reaches = [W.training_reach(k) for k in range(64)]
+    frs = W.frames(reaches)
+    rng = np.random.default_rng(0)
+    ys = [(0.1 * rng.standard_normal((fr.node.shape[0], W.F_OUT))).astype(np.float32)
+          for fr in frs]
+    ws = W.WarmStart(features=W.FEATURES)
+    if args.warmup:
+        ws.train_windows(frs, ys, args.warmup, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ws.train_windows(frs, ys, args.steps, seed=2)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ws.close()
+    print(json.dumps({
+        "metric": "warm-start network AdamW updates/s", "value": args.steps / wall,
+        "unit": "updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (storm_amd/warmstart.py training_reach 0-63, random targets)",
+        "config": {"workload": "warmstart-train-32x4096", "layers": ws.layers,
+                   "width": 64, "features": W.FEATURES}}))
 
 
 if __name__ == "__main__":

@@ -24,7 +24,9 @@ typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int HID = 64;      // hidden width
-constexpr int ROWS_PER_PART = 1024;  // rows per partial of a weight gradient
+constexpr int ROWS_PER_PART = 1024;  // rows per partial of a weight gradient (general K)
+constexpr int DW_ROWS = 128;         // rows per partial of k_gemm_dw_full (one wave each)
+constexpr int RED_CHUNK = 32;        // partials summed per thread in the first reduction level
 
 __device__ __forceinline__ short to_bf16(float f) {  // round to nearest even
   uint32_t u = __float_as_uint(f);
@@ -151,6 +153,83 @@ k_gemm_dw(const float* X, const float* dY, const float* act, int M, int K, float
 #pragma unroll
     for (int j = 0; j < 4; ++j) out[(size_t)(kt * 16 + kq + j) * NOUT + nt * 16 + cn] = acc[j];
   }
+}
+
+// Partial weight and bias gradients of rows [blockIdx.x*DW_ROWS, +DW_ROWS)
+// for K = 16*KT, one wave per block: part[blk][K][NOUT] = X^T dZ and
+// partb[blk][n] = the sum of dZ[m][n] (FP32, unrounded).  The wave holds every
+// (k, n) tile of the product, so X and dY are read once.  Rows beyond M count
+// as zero.
+template <int NOUT, int KT>
+__global__ void __launch_bounds__(64)
+k_gemm_dw_full(const float* X, const float* dY, const float* act, int M, float* part,
+               float* partb) {
+  constexpr int K = 16 * KT, NT = NOUT / 16;
+  __shared__ float s_b[4][NOUT];
+  const int l = threadIdx.x;
+  const int kq = 4 * (l >> 4), cn = l & 15;
+  v4f acc[KT][NT];
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[kt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+  float cs[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) cs[nt] = 0.f;
+  const int w_begin = blockIdx.x * DW_ROWS;
+  const int w_end = min(M, w_begin + DW_ROWS);
+  for (int m = w_begin; m < w_end; m += 16) {
+    v4s b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = m + kq + j;
+        float d = 0.f;
+        if (row < w_end) {
+          const size_t iy = (size_t)row * NOUT + nt * 16 + cn;
+          d = dY[iy];
+          if (act && !(act[iy] > 0.f)) d = 0.f;
+        }
+        cs[nt] += d;
+        b[nt][j] = to_bf16(d);  // B[m][n] = dZ[m][n]
+      }
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+      v4s a;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = m + kq + j;
+        a[j] = to_bf16(row < w_end ? X[(size_t)row * K + kt * 16 + cn] : 0.f);  // A[k][m] = X[m][k]
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[kt][nt] = mfma16(a, b[nt], acc[kt][nt]);
+    }
+  }
+  float* out = part + (size_t)blockIdx.x * K * NOUT;
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[(size_t)(kt * 16 + kq + j) * NOUT + nt * 16 + cn] = acc[kt][nt][j];
+  // column sums: the four row groups of each column, in group order
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) s_b[kq / 4][nt * 16 + cn] = cs[nt];
+  __syncthreads();
+  for (int n = l; n < NOUT; n += 64)
+    partb[(size_t)blockIdx.x * NOUT + n] = ((s_b[0][n] + s_b[1][n]) + s_b[2][n]) + s_b[3][n];
+}
+
+// First reduction level: part2[c][i] = sum over blocks c*RED_CHUNK ..
+// min(nblk, (c+1)*RED_CHUNK) - 1, in block order, of part[blk][i].
+__global__ void k_reduce_chunk(const float* part, int nblk, int size, float* part2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= size) return;
+  const int c = blockIdx.y, b0 = c * RED_CHUNK, b1 = min(nblk, b0 + RED_CHUNK);
+  float s = 0.f;
+  for (int b = b0; b < b1; ++b) s += part[(size_t)b * size + i];
+  part2[(size_t)c * size + i] = s;
 }
 
 // Partial bias gradient: partb[blk][n] = sum of dZ[m][n] over the block's rows.
@@ -334,9 +413,44 @@ namespace {
 
 // Weight gradient of Y = act(X W + b) (rows M, inputs K, NOUT outputs): the
 // partials of each row block, reduced in block order into gw and gb.
+// Sum of the nblk rows of part[nblk][size] into out[size], in a fixed tree:
+// chunks of RED_CHUNK rows in order, then the chunks in order.
+void reduce_rows(storm_mpnn* S, const float* part, int nblk, int size, float* scratch, float* out) {
+  const int nch = (nblk + RED_CHUNK - 1) / RED_CHUNK;
+  if (nch <= 1) {
+    hipLaunchKernelGGL(k_reduce, dim3(blocks(size)), dim3(256), 0, S->stream, part, nblk, size, out);
+    return;
+  }
+  hipLaunchKernelGGL(k_reduce_chunk, dim3(blocks(size), nch), dim3(256), 0, S->stream, part, nblk,
+                     size, scratch);
+  hipLaunchKernelGGL(k_reduce, dim3(blocks(size)), dim3(256), 0, S->stream, scratch, nch, size, out);
+}
+
+template <int NOUT, int KT>
+void weight_grad_full(storm_mpnn* S, const float* X, const float* dY, const float* act, int M,
+                      float* gw, float* gb) {
+  constexpr int K = 16 * KT;
+  const int nblk = (M + DW_ROWS - 1) / DW_ROWS;
+  const int nch = (nblk + RED_CHUNK - 1) / RED_CHUNK;
+  S->part.need((size_t)(nblk + nch) * (K * NOUT + NOUT));
+  float* pw = S->part.p;
+  float* pb = pw + (size_t)nblk * K * NOUT;
+  float* sw = pb + (size_t)nblk * NOUT;
+  float* sb = sw + (size_t)nch * K * NOUT;
+  hipLaunchKernelGGL((k_gemm_dw_full<NOUT, KT>), dim3(nblk), dim3(64), 0, S->stream, X, dY, act, M,
+                     pw, pb);
+  reduce_rows(S, pw, nblk, K * NOUT, sw, gw);
+  reduce_rows(S, pb, nblk, NOUT, sb, gb);
+}
+
 template <int NOUT>
 void weight_grad(storm_mpnn* S, const float* X, const float* dY, const float* act, int M, int K,
                  float* gw, float* gb) {
+  // the widths of this network: every (k, n) tile held by one wave
+  if (K == 16) return weight_grad_full<NOUT, 1>(S, X, dY, act, M, gw, gb);
+  if (K == 64) return weight_grad_full<NOUT, 4>(S, X, dY, act, M, gw, gb);
+  if (K == 128) return weight_grad_full<NOUT, 8>(S, X, dY, act, M, gw, gb);
+  if (K == 144) return weight_grad_full<NOUT, 9>(S, X, dY, act, M, gw, gb);
   const int nblk = (M + ROWS_PER_PART - 1) / ROWS_PER_PART;
   S->part.need((size_t)nblk * (K * NOUT + NOUT));
   float* pw = S->part.p;

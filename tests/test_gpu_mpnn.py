@@ -72,3 +72,30 @@ def test_adamw_lowers_the_loss():
         net.step()
     assert loss < 0.2 * first, (first, loss)
     net.close()
+
+
+def test_gradients_of_a_larger_graph():
+    """Thousands of nodes and edges: the weight gradients go through many
+    row-block partials and both levels of their fixed-order reduction."""
+    from storm_amd import mpnn
+    print("[gpu mpnn] gradients, 6,000 nodes", flush=True)
+    src, dst, X, EF, T = toy_graph(n=6000, seed=3)
+    assert src.shape[0] > 32 * 128  # more than one reduction chunk of edge partials
+    net = mpnn.Net(layers=2, seed=9)
+    ref = Ref(layers=2, bf16=True)
+    p = net.params()
+    o, n = ref.off["dec_w"]
+    p[o:o + n] = np.random.default_rng(6).uniform(-0.3, 0.3, n)
+    net.set_params(p)
+    net.set_graph(src, dst, X, EF)
+    net.forward()
+    net.backward(T)
+    g = net.grads()
+    gr = ref.grads(p, src, dst, X, EF, T)
+    for name, (o, n) in ref.off.items():
+        err = _rel(g[o:o + n], gr[o:o + n])
+        assert err < 1e-2, (name, err)
+    net.forward()
+    net.backward(T)
+    assert np.array_equal(net.grads(), g)
+    net.close()
