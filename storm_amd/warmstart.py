@@ -98,6 +98,8 @@ class Frame:
     a: float
     order: Optional[np.ndarray] = None   # cells along the axis (set by window)
     rank: Optional[np.ndarray] = None
+    eperm: Optional[np.ndarray] = None   # edges by their source's rank (set by window)
+    ers: Optional[np.ndarray] = None     # the source ranks in that order
     # water surface of the zero output relative to the plane (feature set 3:
     # the 1-D backwater profile's; 0 otherwise)
     base: Optional[np.ndarray] = None
@@ -432,14 +434,21 @@ def window(fr: Frame, y: np.ndarray, start: int, size: int, hops: int):
         fr.order = order.astype(np.int64)
         fr.rank = np.empty_like(fr.order)
         fr.rank[fr.order] = np.arange(fr.order.shape[0])
+        rs = fr.rank[fr.src]
+        fr.eperm = np.argsort(rs, kind="stable")
+        fr.ers = rs[fr.eperm]
     a, b = start, start + size
-    rs, rd = fr.rank[fr.src], fr.rank[fr.dst]
-    ins, ind = (rs >= a) & (rs < b), (rd >= a) & (rd < b)
-    keep = ins & ind
-    src = (rs[keep] - a).astype(np.int32)
-    dst = (rd[keep] - a).astype(np.int32)
+    # the edges whose source lies in the window, then those whose
+    # destination does too, in edge order
+    j0, j1 = np.searchsorted(fr.ers, [a, b])
+    cand = fr.eperm[j0:j1]
+    rd = fr.rank[fr.dst[cand]]
+    ind = (rd >= a) & (rd < b)
+    keep = np.sort(cand[ind])
+    src = (fr.rank[fr.src[keep]] - a).astype(np.int32)
+    dst = (fr.rank[fr.dst[keep]] - a).astype(np.int32)
     near = np.zeros(size, dtype=bool)
-    near[rs[ins & ~ind] - a] = True  # cells with a neighbour outside the window
+    near[fr.ers[j0:j1][~ind] - a] = True  # cells with a neighbour outside the window
     for _ in range(hops - 1):
         near[dst[near[src]]] = True
     cells = fr.order[a:b]

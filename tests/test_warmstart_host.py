@@ -90,6 +90,10 @@ def test_window_cut_and_loss_weights():
     inside = set(cells.tolist())
     assert pairs == {(i, j) for i, j in full if i in inside and j in inside}
     assert EF.shape[0] == src.shape[0]
+    # in the reach's edge order, each with its own features
+    k = np.nonzero(np.isin(fr.src, cells) & np.isin(fr.dst, cells))[0]
+    assert np.array_equal(cells[src], fr.src[k]) and np.array_equal(cells[dst], fr.dst[k])
+    assert np.array_equal(EF, fr.edge[k])
     # weight zero exactly within `hops` edges of a cell that has an outside
     # neighbour (brute-force breadth-first distance)
     nb = {}
