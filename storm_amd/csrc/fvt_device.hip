@@ -47,6 +47,10 @@ namespace {
 
 constexpr int CTRL_THREADS = 1024;
 constexpr int DEFAULT_CAP = 256;  // cells + halo cells per tile (threads per stage block)
+// Waves per SIMD the stage kernel is compiled for: four at capacity 256 (four
+// blocks per CU), the compiler's choice otherwise (the xsf kernel's shared
+// memory and the larger capacities allow fewer blocks per CU anyway).
+#define STAGE_WAVES(cap, xsf) ((cap) == 256 && !(xsf) ? 4 : 1)
 
 // largest power of two not above n (the fixed tree shape of block reductions)
 __host__ __device__ constexpr int pow2_floor(int n) {
@@ -1261,14 +1265,17 @@ __device__ void bcq_string(const Geo& g, const Opts& o, const CellD* din, const 
 //      last stage also implicit friction, hvalve, the residual partial sums
 //      and the next step's CFL candidate (per-tile partials, reduced in a
 //      fixed order by k_control).
-// Three waves per SIMD (about 150 VGPRs, no spills).  Forcing four (128
-// VGPRs) spills the prefetched phase-4 inputs and runs 30 % slower.
+// Four waves per SIMD at capacity 256 (at most 128 VGPRs, under 40 KB of
+// shared memory): the inflow strings run before the other loads are issued
+// and each cell's rk slots and record are loaded in phase 4, so no value is
+// held in registers through the strings or phases 2-3.  The few registers the
+// strings' fixed point spills are reloaded on that path alone.
 // MODE 0 runs it whole.  Residual smoothing splits it around k_rsavg:
 // MODE 1 stops after the residual (fluxes and sources) of every tile cell
 // and writes it to S.phi0; MODE 2 skips phases 2-3 and the residual, reads
 // the smoothed residual from S.phis and runs the rest of phase 4.
 template <int CAP, bool XSF = false, int MODE = 0>
-__global__ void __launch_bounds__(CAP)
+__global__ void __launch_bounds__(CAP) __attribute__((amdgpu_waves_per_eu(STAGE_WAVES(CAP, XSF))))
 k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   // fluxes once per edge from the side records (the xsf loop forms them per
   // cell side, as before)
@@ -1283,27 +1290,31 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   const int t = threadIdx.x;
   const int N = g.N;
 
-  // Loads that need only the tile header are issued first, so their latency
-  // overlaps the wait for the reach's control block and the inflow strings:
-  // an owned cell's phase-4 inputs (its rk slots 0 and s-1, the only ones
-  // with a nonzero alpha in the SSP tables, metadata, local neighbour ids
-  // and cell record) and, for every tile and halo cell, the neighbour and
-  // vertex ids phase 3 gathers through.
-  double pf_h0 = ZERO, pf_u0 = ZERO, pf_v0 = ZERO, pf_h1 = ZERO, pf_u1 = ZERO, pf_v1 = ZERO;
+  const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
+  const Lds L = lds_layout(lds_dyn, CAP, XSF);
+
+  // 2. inflow discharge strings (their buffers overlay the cell arrays), in
+  // the few tiles that touch one, before any other load is issued
+  if (MODE != 2 && qmask) {
+    const RInfo rq = g.reach[hd1.w];
+    const int dq = c->done;
+    if (st.s == 1 && t == 0 && b == rq.tile0) S.vdone[hd1.w] = dq;
+    if (dq) return;
+    for (uint64_t qm = qmask; qm; qm &= qm - 1)
+      bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, rq, __builtin_ctzll(qm), L, s_at,
+                      s_null, s_skip);
+  }
+
+  // Loads that need only the tile header are issued next, so their latency
+  // overlaps the wait for the reach's control block: an owned cell's
+  // metadata and local neighbour ids and, for every tile and halo cell, the
+  // neighbour and vertex ids phase 3 gathers through.
   uint64_t pf_meta = 0, pf_ln = 0;
-  CellS pf_cs{};
   int ci = 0, nb0 = -1, nb1 = -1, nb2 = -1, v0 = 0, v1 = 0, v2 = 0;
   if (t < T) {
     const int i = c0 + t;
-    const double* s0 = st.slot[0];
-    pf_h0 = s0[i]; pf_u0 = s0[N + i]; pf_v0 = s0[2 * N + i];
-    if (st.s >= 2) {
-      const double* s1 = st.slot[st.s - 1];
-      pf_h1 = s1[i]; pf_u1 = s1[N + i]; pf_v1 = s1[2 * N + i];
-    }
     pf_meta = g.cmeta[i];
     pf_ln = tl.cln[i];
-    pf_cs = ld(&g.cs[i]);
     ci = i;
     nb0 = g.cnbr[i]; nb1 = g.cnbr[N + i]; nb2 = g.cnbr[2 * N + i];
     v0 = g.cvtx[i]; v1 = g.cvtx[N + i]; v2 = g.cvtx[2 * N + i];
@@ -1317,8 +1328,6 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
   const int done = c->done;
   if (st.s == 1 && t == 0 && b == ri.tile0) S.vdone[hd1.w] = done;
   if (done) return;
-  const uint64_t qmask = (uint64_t)(uint32_t)hd1.x | ((uint64_t)(uint32_t)hd1.y << 32);
-  const Lds L = lds_layout(lds_dyn, CAP, XSF);
   const double h_dry = o.h_dry, h_wet = o.h_wet;
   if (MODE != 1 && t == 0 && o.steady && st.last) {
     s_dlast = last_cell_dt(g, o, S.din, ri.last_cell);
@@ -1338,11 +1347,6 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       F.umag = F.dry ? ZERO : sqrt(di.u * di.u + di.v * di.v);
     }
   } else {
-  // 2. inflow discharge strings (their buffers overlay the cell arrays)
-  for (uint64_t qm = qmask; qm; qm &= qm - 1)
-    bcq_string<CAP>(g, o, S.din, S.gwork, S.vwet, S.sq, ri, __builtin_ctzll(qm), L, s_at, s_null,
-                    s_skip);
-
   // 3. face states of the tile and halo cells
   if (t < T + H) {
     const bool own = t < T;
@@ -1367,8 +1371,6 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     if (own && F.fix) {
       st.hin[N + i] = ZERO;  // rku, rkv(i, rkstep-1) (hgradfix2.f90:80-81)
       st.hin[2 * N + i] = ZERO;
-      if (st.s >= 2) { pf_u1 = ZERO; pf_v1 = ZERO; }  // the same values, prefetched
-      else { pf_u0 = ZERO; pf_v0 = ZERO; }
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -1376,7 +1378,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       if (XSF) L.rcy[j * CAP + t] = F.rcy[j];
     }
     if (XSF) { L.gy[t] = F.guy; L.gy[CAP + t] = F.gvy; }
-    L.z[t] = own ? pf_cs.z : g.cs[i].z;
+    L.z[t] = g.cs[i].z;
     L.h[t] = F.h;
     L.dry[t] = F.dry;
   }
@@ -1438,7 +1440,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     const int i = c0 + t;
     const uint64_t meta = pf_meta;
     const uint64_t ln = pf_ln;
-    const CellS si = pf_cs;
+    const CellS si = ld(&g.cs[i]);
     Tri tr;
     if constexpr (MODE == 2) {
       const double4 q0 = g.vxyz[v0], q1 = g.vxyz[v1], q2 = g.vxyz[v2];
@@ -1599,15 +1601,28 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
     // the SSP combination: slots 0 and s-1 in ascending order (the slots in
     // between have alpha 0 in every table: rk_table)
     double qh = ZERO, qu = ZERO, qv = ZERO;
+    // the cell's rk slots 0 and s-1, the only ones with a nonzero alpha in
+    // the SSP tables; hgradfix2 zeroed the hu, hv of slot s-1 above
+    const double* s0 = st.slot[0];
+    double rh0 = s0[i], ru0 = s0[N + i], rv0 = s0[2 * N + i];
+    double rh1 = ZERO, ru1 = ZERO, rv1 = ZERO;
+    if (st.s >= 2) {
+      const double* s1 = st.slot[st.s - 1];
+      rh1 = s1[i]; ru1 = s1[N + i]; rv1 = s1[2 * N + i];
+    }
+    if (MODE != 2 && F.fix) {
+      if (st.s >= 2) { ru1 = ZERO; rv1 = ZERO; }
+      else { ru0 = ZERO; rv0 = ZERO; }
+    }
     if (st.alpha[0] != ZERO) {
       const double a = st.alpha[0];
-      qh = qh + a * pf_h0; qu = qu + a * pf_u0; qv = qv + a * pf_v0;
+      qh = qh + a * rh0; qu = qu + a * ru0; qv = qv + a * rv0;
     }
     if (st.s >= 2 && st.alpha[st.s - 1] != ZERO) {
       const double a = st.alpha[st.s - 1];
-      qh = qh + a * pf_h1; qu = qu + a * pf_u1; qv = qv + a * pf_v1;
+      qh = qh + a * rh1; qu = qu + a * ru1; qv = qv + a * rv1;
     }
-    const double h0n = pf_h0;  // h at the start of the step (residual)
+    const double h0n = rh0;  // h at the start of the step (residual)
     qh = qh + st.beta * dt * p0;
     qu = (qu + st.beta * dt * p1);
     qv = (qv + st.beta * dt * p2);
@@ -2027,7 +2042,7 @@ k_rsavg_fast(RsFast T, const Ctrl* ctrl, double fac1, double fac2, const double4
   extern __shared__ double sm[];
   const int r = blockIdx.x;
   if (ctrl[r].done) return;
-  const int t = threadIdx.x, N = T.N;
+  const int t = threadIdx.x;
   const int l0 = T.rlev[r], l1 = T.rlev[r + 1];
   const int base = T.hdr[2 * l0].x;
   // prologue: the ring slots of the first RSF_G + 1 levels and the far slots

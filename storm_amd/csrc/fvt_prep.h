@@ -41,13 +41,13 @@ constexpr int CMS_NL = 4, CMS_LEFT = 6, CMS_HBAR = 7, CMS_REV = 8, CMS_CHAIN = 9
 constexpr int CM_CULVERT = 60;
 
 constexpr int MAX_STRINGS = 64;  // boundary strings of each kind per reach
-// Most cells a tile of capacity cap (cells + halo) owns: the whole capacity,
-// so a tile's halo cells share its last wave with owned cells.  (Capping the
-// owned cells at whole waves, 192 of 256, left the last wave idle in the
-// per-cell update and needed 4 % more tiles; at three waves per SIMD that was
-// the slower choice.)  The stage kernel sizes the owned cells' vertex arrays
-// in shared memory by it.
-constexpr int tile_tmax(int cap) { return cap; }
+// Most cells a tile of capacity cap (cells + halo) owns.  The stage kernel
+// sizes the owned cells' vertex arrays in shared memory by it.  At capacity
+// 256 it is 224, which keeps a block's shared memory under 40 KB, so four
+// blocks (four waves per SIMD) share a CU; a tile rarely owns more, since its
+// halo is about a quarter of its owned cells.  The larger capacities run one
+// or two blocks per CU and keep the whole capacity.
+constexpr int tile_tmax(int cap) { return cap == 256 ? 224 : cap; }
 
 // A batch of independent reaches laid end to end in one mesh: reach r owns
 // input cells [cell_ptr[r], cell_ptr[r+1]), inflow strings [in_ptr[r],
