@@ -64,6 +64,19 @@ int storm_mpnn_adamw_step(storm_mpnn_t *net);
 int storm_mpnn_set_lr(storm_mpnn_t *net, float lr);
 const char *storm_mpnn_last_error(storm_mpnn_t *net);
 
+/* The depth profile of the non-learned start the network refines (host
+ * code, no device): the steady 1-D gradually varied flow with unit
+ * discharge q over a bed of slope S,
+ *   dh/ds = -(S - Sf) / (1 - Fr^2),  Fr^2 = q^2 / (g h^3),
+ *   Sf = manning^2 q^2 / h^(10/3),  g = 9.81,
+ * integrated upstream from h[0] = h0 by the classical fourth-order
+ * Runge-Kutta method on n_steps steps of ds, each stage's depth and each
+ * result held at h_floor or above.  h: n_steps + 1 values.  Every operation
+ * is the IEEE double one in the order written, with the C library's pow.
+ * Returns 0, or -1 for a null h or a negative n_steps. */
+int storm_mpnn_gvf_profile(double q, double S, double manning, double h0, double h_floor,
+                           double ds, int32_t n_steps, double *h);
+
 #ifdef __cplusplus
 }
 #endif

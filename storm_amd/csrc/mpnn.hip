@@ -748,4 +748,28 @@ int storm_mpnn_set_lr(storm_mpnn_t* S, float lr) {
 
 const char* storm_mpnn_last_error(storm_mpnn_t* S) { return S ? S->err.c_str() : "null network"; }
 
+int storm_mpnn_gvf_profile(double q, double S, double manning, double h0, double h_floor,
+                           double ds, int32_t n_steps, double* h) {
+  if (!h || n_steps < 0) return -1;
+  constexpr double G = 9.81;
+  const double third10 = 10.0 / 3.0;
+  const auto rate = [&](double y) {
+    if (h_floor > y) y = h_floor;
+    const double fr2 = q * q / (G * std::pow(y, 3.0));
+    const double sf = manning * manning * q * q / std::pow(y, third10);
+    return -(S - sf) / (1.0 - fr2);
+  };
+  h[0] = h0;
+  for (int32_t k = 0; k < n_steps; ++k) {
+    const double y = h[k];
+    const double k1 = rate(y);
+    const double k2 = rate(y + 0.5 * ds * k1);
+    const double k3 = rate(y + 0.5 * ds * k2);
+    const double k4 = rate(y + ds * k3);
+    const double y1 = y + ds * (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
+    h[k + 1] = (h_floor > y1) ? h_floor : y1;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libstorm_mpnn.so")
 EXPORTS = ["storm_mpnn_create", "storm_mpnn_destroy", "storm_mpnn_num_params",
            "storm_mpnn_get_params", "storm_mpnn_set_params", "storm_mpnn_set_graph",
            "storm_mpnn_forward", "storm_mpnn_backward", "storm_mpnn_get_grads",
-           "storm_mpnn_adamw_step", "storm_mpnn_set_lr", "storm_mpnn_last_error"]
+           "storm_mpnn_adamw_step", "storm_mpnn_set_lr", "storm_mpnn_last_error",
+           "storm_mpnn_gvf_profile"]
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -56,12 +57,25 @@ def load(path: str = LIB_PATH):
     L.storm_mpnn_set_lr.argtypes = [C.c_void_p, C.c_float]
     L.storm_mpnn_last_error.restype = C.c_char_p
     L.storm_mpnn_last_error.argtypes = [C.c_void_p]
+    L.storm_mpnn_gvf_profile.argtypes = [C.c_double] * 6 + [C.c_int32, C.POINTER(C.c_double)]
     _lib = L
     return L
 
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def gvf_profile(q: float, S: float, manning: float, h0: float, h_floor: float, ds: float,
+                n_steps: int) -> np.ndarray:
+    """storm_mpnn_gvf_profile: the RK4 depths of the steady 1-D gradually
+    varied flow at n_steps + 1 points ds apart, upstream from h0 (host
+    code)."""
+    h = np.empty(n_steps + 1, dtype=np.float64)
+    if load().storm_mpnn_gvf_profile(q, S, manning, h0, h_floor, ds, n_steps,
+                                     _p(h, C.c_double)) != 0:
+        raise MpnnError("storm_mpnn_gvf_profile: bad arguments")
+    return h
 
 
 class Net:

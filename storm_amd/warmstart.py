@@ -114,16 +114,23 @@ def _string_geometry(case, b):
 
 def _string_cells(case, nodes) -> np.ndarray:
     """Cells with a side on the string (consecutive nodes of the chain)."""
-    key = {}
-    for k in np.nonzero(case.edge_e[:, 1] < 0)[0]:  # string sides are boundary sides
-        p0, p1 = case.edge_p[k]
-        key[(min(p0, p1), max(p0, p1))] = k
-    out = []
-    for a, b in zip(nodes[:-1], nodes[1:]):
-        k = key.get((min(a, b), max(a, b)))
-        if k is not None:
-            out.append(case.edge_e[k, 0])
-    return np.unique(np.array(out, dtype=np.int64))
+    bk = np.nonzero(case.edge_e[:, 1] < 0)[0]  # string sides are boundary sides
+    nodes = np.asarray(nodes, dtype=np.int64)
+    if bk.size == 0 or nodes.size < 2:
+        return np.zeros(0, dtype=np.int64)
+    ep = case.edge_p[bk].astype(np.int64)
+    P = int(max(ep.max(), nodes.max())) + 1
+
+    def key(a, b):  # an undirected side as one integer
+        return np.minimum(a, b) * P + np.maximum(a, b)
+
+    kb = key(ep[:, 0], ep[:, 1])
+    order = np.argsort(kb, kind="stable")
+    kb = kb[order]
+    ks = key(nodes[:-1], nodes[1:])
+    pos = np.minimum(np.searchsorted(kb, ks), kb.size - 1)
+    hit = kb[pos] == ks
+    return np.unique(case.edge_e[bk[order[pos[hit]]], 0].astype(np.int64))
 
 
 FEATURES = 3  # node feature set of new networks
@@ -343,28 +350,18 @@ def gvf_depth(fr: Frame, manning: float, n_steps: int = 4000) -> np.ndarray:
     axis; returns h at each cell's distance from the outlet.  Where the
     profile would reach 1.1 times the critical depth it is held there (the
     frame's subcritical assumption)."""
+    from . import mpnn
     q, S = fr.q, fr.S
     span = float(fr.dist.max()) if fr.dist.size else 0.0
     hc = (q * q / G) ** (1.0 / 3.0)
     h_floor = 1.1 * hc
-
-    def dh_ddist(h):
-        h = max(h, h_floor)
-        fr2 = q * q / (G * h ** 3)
-        sf = manning * manning * q * q / h ** (10.0 / 3.0)
-        return -(S - sf) / (1.0 - fr2)
-
     s = np.linspace(0.0, max(span, 1e-9), n_steps + 1)
-    ds = s[1] - s[0]
-    h = np.empty_like(s)
-    h[0] = max(fr.H - fr.a, h_floor)
-    for k in range(n_steps):
-        y = h[k]
-        k1 = dh_ddist(y)
-        k2 = dh_ddist(y + 0.5 * ds * k1)
-        k3 = dh_ddist(y + 0.5 * ds * k2)
-        k4 = dh_ddist(y + ds * k3)
-        h[k + 1] = max(y + ds * (k1 + 2 * k2 + 2 * k3 + k4) / 6.0, h_floor)
+    ds = float(s[1] - s[0])
+    h0 = max(float(fr.H - fr.a), h_floor)
+    # the integration itself is host C++ in the network's library
+    # (storm_mpnn_gvf_profile): the same operations as in Python, without
+    # holding the interpreter while the starts of many reaches are built
+    h = mpnn.gvf_profile(float(q), float(S), float(manning), h0, h_floor, ds, n_steps)
     return np.interp(fr.dist, s, h)
 
 

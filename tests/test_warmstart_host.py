@@ -156,6 +156,45 @@ def test_backwater_profile_closed_forms():
         assert np.abs(dh - rhs).max() < 1e-4 * np.abs(rhs).max() + 1e-12
 
 
+def _gvf_steps_in_python(q, S, n, h0, h_floor, ds, n_steps):
+    """The integration as the network's training data was made with it:
+    RK4 in Python floats (libm pow)."""
+    def rate(h):
+        h = max(h, h_floor)
+        fr2 = q * q / (9.81 * h ** 3)
+        sf = n * n * q * q / h ** (10.0 / 3.0)
+        return -(S - sf) / (1.0 - fr2)
+    h = [h0]
+    for _ in range(n_steps):
+        y = h[-1]
+        k1 = rate(y)
+        k2 = rate(y + 0.5 * ds * k1)
+        k3 = rate(y + 0.5 * ds * k2)
+        k4 = rate(y + ds * k3)
+        h.append(max(y + ds * (k1 + 2 * k2 + 2 * k3 + k4) / 6.0, h_floor))
+    return np.array(h)
+
+
+def test_gvf_profile_is_bit_identical_to_the_python_steps():
+    """storm_mpnn_gvf_profile (host C++) gives the same bits as the Python
+    RK4 steps, M1 and M2 curves and the critical-depth floor included, so
+    the starts and the network's features are unchanged by it."""
+    from storm_amd import mpnn
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        q = float(rng.uniform(0.3, 8.0))
+        S = float(10 ** rng.uniform(-4.5, -2.5))
+        n = float(rng.uniform(0.02, 0.08))
+        hc = (q * q / 9.81) ** (1.0 / 3.0)
+        hn = (n * q / np.sqrt(S)) ** 0.6
+        h0 = float(max(hn * rng.uniform(0.3, 2.5), 1.1 * hc))
+        ds = float(rng.uniform(0.05, 5.0))
+        want = _gvf_steps_in_python(q, S, n, h0, 1.1 * hc, ds, 300)
+        got = mpnn.gvf_profile(q, S, n, h0, 1.1 * hc, ds, 300)
+        assert np.array_equal(got, want)
+    assert mpnn.gvf_profile(1.0, 1e-3, 0.03, 1.0, 0.5, 1.0, 0).tolist() == [1.0]
+
+
 def test_parallel_builders_equal_the_serial_ones():
     cs = [cases.ensemble_reach(k, n_min=2000, n_max=4000, base_seed=91) for k in range(3)]
     for kind, fn in (("plane", W.baseline_state), ("gvf", W.gvf_state), ("cold", W.cold_state)):
