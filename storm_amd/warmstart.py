@@ -585,7 +585,10 @@ class WarmStart:
         return tot / len(frames)
 
     # -- inference ------------------------------------------------------
-    def predict_outputs(self, frames: Sequence[Frame], batch: int = 4) -> List[np.ndarray]:
+    def predict_outputs(self, frames: Sequence[Frame], batch: int = 16) -> List[np.ndarray]:
+        """Network outputs of every frame, `batch` reaches per forward pass
+        (the reaches' graphs are disjoint, so a reach's outputs do not
+        depend on the others in its batch)."""
         out = []
         for s in range(0, len(frames), batch):
             fs = frames[s:s + batch]
@@ -599,7 +602,7 @@ class WarmStart:
                 o += n
         return out
 
-    def predict(self, cases_: Sequence, batch: int = 4, h_dry: float = 1e-3,
+    def predict(self, cases_: Sequence, batch: int = 16, h_dry: float = 1e-3,
                 discharge: str = "uniform"):
         """Per case: (h, u, v) for storm_fvt_set_state.  By default the
         network's water surface with uniform discharge along the axis: a
@@ -610,7 +613,7 @@ class WarmStart:
         discharge outputs).  discharge="learned" decodes those too."""
         frs = frames(cases_, self.features)
         ys = self.predict_outputs(frs, batch)
-        return [decode(fr, y, h_dry, discharge) for fr, y in zip(frs, ys)]
+        return _pool_map(lambda a: decode(a[0], a[1], h_dry, discharge), list(zip(frs, ys)))
 
     # -- checkpoint ---------------------------------------------------------
     def save(self, path: str, meta: Optional[Dict] = None):

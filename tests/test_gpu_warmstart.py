@@ -76,3 +76,15 @@ def test_warm_start_needs_fewer_steps_than_cold(net, steady):
     for (a, _), (b, _), st in zip(warm, cold, ref):
         if b is not None:
             assert _rms(a["h"], st["h"]) <= 1.5 * _rms(b["h"], st["h"]) + 1e-3
+
+
+def test_outputs_do_not_depend_on_the_batch(net):
+    """A forward pass over several reaches (their disjoint union) gives each
+    reach the bits it gets alone."""
+    rs = [W.heldout_reach(k, 20000) for k in range(3)]
+    frs = W.frames(rs, net.features)
+    print("[gpu warmstart] batch independence, 3 reaches", flush=True)
+    alone = net.predict_outputs(frs, batch=1)
+    together = net.predict_outputs(frs, batch=3)
+    for a, b in zip(alone, together):
+        assert np.array_equal(a, b)
