@@ -1123,8 +1123,8 @@ struct Lds {
 };
 
 // Face states, bed and depth for every cell of the tile and halo; vertex
-// coordinates and bed for the owned cells only: 40.3 KB at capacity 256
-// (three blocks per CU, as many as the registers allow).
+// coordinates and bed for the owned cells only: 38.0 KB at capacity 256
+// (four blocks per CU, as many as the registers allow).
 __host__ __device__ constexpr size_t lds_bytes(int cap, bool xsf = false) {
   return sizeof(double) * ((xsf ? 16 : 11) * (size_t)cap + 9 * (size_t)tile_tmax(cap)) +
          (size_t)cap;
