@@ -97,6 +97,9 @@ struct Geo {
   const int* vell;          // [vW][P] incident cells per vertex, n2t order
   const double* vwt;        // [vW][P] inverse-distance weights
   const double2* vzb;       // per vertex {zvtx, bits: string index | degree << 32}
+  // per chunk of 256 vertices (one block of the vertex pass), the tiles that
+  // own its vertices' incident cells: vc_tile[vc_ptr[c] .. vc_ptr[c+1])
+  const int *vc_ptr, *vc_tile;
   const int *qs_ptr, *qs_cell, *qs_slot;
   const double *qs_rcx, *qs_rcy, *qs_nx, *qs_ny, *qs_len;
   const int* in_type;
@@ -130,6 +133,7 @@ struct State {
   const CellD* din;   // state at the start of the stage
   CellD* dout;        // state for the next stage
   double* zeta;       // water surface of the next stage's input, NaN at dry cells
+  uint8_t* twet;      // per tile: some owned cell's zeta is not NaN (written with zeta)
   double *gwork, *hvtx, *zetavtx, *phi, *part;
   uint8_t *dry_post, *vwet;
   double *sq, *sh;    // current Q (or stage) of every inflow string, stage of every outflow string
@@ -261,7 +265,7 @@ __device__ __forceinline__ bool find_dry(double h, double u, double v, double h_
 template <int CAP>
 __global__ void __launch_bounds__(CAP)
 k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u0,
-       const double* v0, CellD* d, double* zeta, double* part) {
+       const double* v0, CellD* d, double* zeta, uint8_t* twet, double* part) {
   // solverFVT.f90:123-127 and the first find_dry_cells, one block per tile;
   // the CFL candidates of step 1 go to the per-tile partials (slot 3),
   // reduced by k_control
@@ -270,6 +274,7 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
   const int4 hd0 = tl.hdr[2 * blockIdx.x], hd1 = tl.hdr[2 * blockIdx.x + 1];
   const int c0 = hd0.x, T = hd0.y, b = hd1.z;
   double cand = DBL_MAX;
+  bool wet_out = false;
   if (t < T) {
     const int i = c0 + t;
     const double h = h0[i], uu = u0[i], vv = v0[i];
@@ -280,6 +285,7 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
     const bool dr = find_dry(h, uu, vv, o.h_dry, o.h_wet, &um);
     st4(&d[i], h, uu, vv, dr ? ONE : ZERO);
     zeta[i] = dr ? __builtin_nan("") : h + g.cs[i].z;
+    wet_out = !dr;
     const double wc = dr ? ZERO : sqrt(G * h);
     const Tri tr = load_tri(g, i);
     const double area = tri_area(tr);
@@ -287,6 +293,8 @@ k_init(Geo g, Opts o, Tiles tl, double* slot0, const double* h0, const double* u
     const double dd = um + wc + SMALL;
     cand = o.cfl * area / (dd * perim);
   }
+  const int any_wet = __syncthreads_or(wet_out);
+  if (t == 0) twet[b] = any_wet != 0;
   s_c[t] = cand;
   __syncthreads();
   constexpr int P2 = pow2_floor(CAP);
@@ -565,7 +573,7 @@ __device__ __forceinline__ Vtx vertex_idw(const Geo& g, const Opts& o, const Cel
 __device__ __forceinline__ void vertex_stage(const Geo& g, const Opts& o, const double* zeta,
                                              double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
                                              const int* vdone, const double* sh, int first,
-                                             int p) {
+                                             bool alldry, int p) {
   if (p >= g.P) return;
   const double2 zb = g.vzb[p];
   const double zv = zb.x;
@@ -576,7 +584,9 @@ __device__ __forceinline__ void vertex_stage(const Geo& g, const Opts& o, const 
   const size_t P = (size_t)g.P;
   double id = ZERO, a = ZERO, zr = zv;
   int counter = 0;
-  for (int t = 0; t < deg; t += 4) {
+  // in an all-dry chunk every incident cell's zeta is NaN and adds nothing
+  const int nread = alldry ? 0 : deg;
+  for (int t = 0; t < nread; t += 4) {
     // four cells per round, all loads issued before the sums
     int k[4];
     double w[4], zk[4];
@@ -608,25 +618,37 @@ __device__ __forceinline__ void vertex_stage(const Geo& g, const Opts& o, const 
   hvtx[p] = hv;
 }
 
+// Every incident cell of chunk c's vertices is dry in the stage input: no
+// tile owning one has a wet cell (the flags are written with zeta).
+__device__ __forceinline__ bool chunk_dry(const Geo& g, const uint8_t* twet, int c) {
+  for (int j = g.vc_ptr[c]; j < g.vc_ptr[c + 1]; ++j)
+    if (twet[g.vc_tile[j]]) return false;
+  return true;
+}
+
 // The vertex pass of stages 2 and up.
 __global__ void __launch_bounds__(256)
-k_vertex_stage(Geo g, Opts o, const double* zeta, double* hvtx, uint8_t* vwet, const Ctrl* ctrl,
-               const double* sh) {
-  vertex_stage(g, o, zeta, hvtx, vwet, ctrl, nullptr, sh, 0, xcd_tile() * blockDim.x + threadIdx.x);
+k_vertex_stage(Geo g, Opts o, const double* zeta, const uint8_t* twet, double* hvtx,
+               uint8_t* vwet, const Ctrl* ctrl, const double* sh) {
+  const int c = xcd_tile();
+  vertex_stage(g, o, zeta, hvtx, vwet, ctrl, nullptr, sh, 0, chunk_dry(g, twet, c),
+               c * blockDim.x + threadIdx.x);
 }
 
 // The first stage's vertex pass and, as its first blocks (one per reach), the
 // step's k_control: the two need only the previous step's last stage and
 // write disjoint data, so one launch runs both.
 __global__ void __launch_bounds__(256)
-k_vertex_first(Geo g, Opts o, const double* zeta, uint8_t* vwet, const int* vdone, CtrlArgs A) {
+k_vertex_first(Geo g, Opts o, const double* zeta, const uint8_t* twet, uint8_t* vwet,
+               const int* vdone, CtrlArgs A) {
   const int nr = g.nr;
   if ((int)blockIdx.x < nr) {
     control_block<256>(g, o, A, blockIdx.x);
     return;
   }
-  const int p = xcd_map(gridDim.x - nr, blockIdx.x - nr) * blockDim.x + threadIdx.x;
-  vertex_stage(g, o, zeta, A.hvtx, vwet, A.ctrl, vdone, A.sh, 1, p);
+  const int c = xcd_map(gridDim.x - nr, blockIdx.x - nr);
+  vertex_stage(g, o, zeta, A.hvtx, vwet, A.ctrl, vdone, A.sh, 1, chunk_dry(g, twet, c),
+               c * blockDim.x + threadIdx.x);
 }
 
 // All vertices after the time loop (solverFVT.f90:483), with zetavtx.  With
@@ -1436,6 +1458,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
 
   // 4. update of the tile cells
   double top = ZERO, bot = ZERO, mxphi = ZERO, cand = DBL_MAX, topq = ZERO, botq = ZERO;
+  bool wet_out = false;  // the cell's zeta written below is not NaN
   if (t < T) {
     const int i = c0 + t;
     const uint64_t meta = pf_meta;
@@ -1647,6 +1670,7 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um);
       st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
       S.zeta[i] = dn ? __builtin_nan("") : hn + zi;
+      wet_out = !dn;
     } else {
       // find_dry_cells + fric_terms_impl (solverFVT.f90:335-336)
       double um;
@@ -1700,10 +1724,16 @@ k_stage(Geo g, Opts o, Tiles tl, Stage st, State S) {
       const bool dn = find_dry(hn, un, vn, h_dry, h_wet, &um2);
       st4(&S.dout[i], hn, un, vn, dn ? ONE : ZERO);
       S.zeta[i] = dn ? __builtin_nan("") : hn + zi;
+      wet_out = !dn;
       const double wc = dn ? ZERO : sqrt(G * hn);
       cand = o.cfl * area / ((um2 + wc + SMALL) * perim);
     }
     }  // MODE != 1
+  }
+  if (MODE != 1) {
+    // the tile's flag for the next vertex pass, written with its zeta
+    const int any_wet = __syncthreads_or(wet_out);
+    if (t == 0) S.twet[b] = any_wet != 0;
   }
   if (MODE == 1 || !st.last) return;
   // per-tile partials: a fixed butterfly within each wave (every lane ends
@@ -2627,6 +2657,31 @@ void setup_device(storm_fvt* C) {
   tl.erec = dupload(own, P.t_erec);
   tl.nt = P.nt;
   {
+    // per 256-vertex chunk of the vertex pass, the tiles owning the cells
+    // incident to its vertices (the vertex pass skips its gathers where all
+    // of them are dry)
+    std::vector<int> tile_of(N);
+    for (int b = 0; b < P.nt; ++b)
+      for (int i = P.t_ptr[b]; i < P.t_ptr[b + 1]; ++i) tile_of[i] = b;
+    const int PP = P.P, nch = (PP + 255) / 256;
+    std::vector<int> vc_ptr(nch + 1, 0), vc_tile, seen;
+    for (int c = 0; c < nch; ++c) {
+      seen.clear();
+      for (int p = 256 * c; p < std::min(PP, 256 * (c + 1)); ++p)
+        for (int t = 0; t < P.vW; ++t) {
+          const int k = P.vell[(size_t)t * PP + p];
+          if (k >= 0) seen.push_back(tile_of[k]);
+        }
+      std::sort(seen.begin(), seen.end());
+      seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+      vc_tile.insert(vc_tile.end(), seen.begin(), seen.end());
+      vc_ptr[c + 1] = (int)vc_tile.size();
+    }
+    if (vc_tile.empty()) vc_tile.push_back(0);
+    C->g.vc_ptr = dupload(own, vc_ptr);
+    C->g.vc_tile = dupload(own, vc_tile);
+  }
+  {
     std::vector<RInfo> ri(P.nr);
     for (int r = 0; r < P.nr; ++r) {
       RInfo& q = ri[r];
@@ -2649,6 +2704,7 @@ void setup_device(storm_fvt* C) {
   S.phi = dalloc<double>(own, 3 * N);
   S.dry_post = dalloc<uint8_t>(own, N);
   S.zeta = dalloc<double>(own, N);
+  S.twet = dalloc<uint8_t>(own, P.nt);
   S.part = dalloc<double>(own, PART * (size_t)P.nt);
   C->d_a = dalloc<CellD>(own, N);
   C->d_b = dalloc<CellD>(own, N);
@@ -2746,7 +2802,8 @@ void setup_device(storm_fvt* C) {
 template <int CAP>
 void launch_init(storm_fvt* C) {
   hipLaunchKernelGGL(k_init<CAP>, dim3(C->tl.nt), dim3(CAP), 0, C->stream, C->g, C->o, C->tl,
-                     C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.zeta, C->S.part);
+                     C->slot_ptr(0), C->d_h0, C->d_u0, C->d_v0, C->d_a, C->S.zeta, C->S.twet,
+                     C->S.part);
 }
 
 template <int CAP, bool XSF = false, int MODE = 0>
@@ -2867,10 +2924,10 @@ void launch_step(storm_fvt* C, bool launch = true) {
       const int nvb = (P.P + 255) / 256;
       if (s == 1)
         hipLaunchKernelGGL(k_vertex_first, dim3(P.nr + nvb), dim3(256), 0, C->stream, C->g, C->o,
-                           C->S.zeta, C->S.vwet, C->S.vdone, ctrl_args(C, 1));
+                           C->S.zeta, C->S.twet, C->S.vwet, C->S.vdone, ctrl_args(C, 1));
       else
         hipLaunchKernelGGL(k_vertex_stage, dim3(nvb), dim3(256), 0, C->stream, C->g, C->o,
-                           C->S.zeta, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
+                           C->S.zeta, C->S.twet, C->S.hvtx, C->S.vwet, C->S.ctrl, C->S.sh);
       if (C->o.gradient == STORM_GRAD_GAUSS && C->o.limiter != 0)
         hipLaunchKernelGGL(k_gauss1, dim3((N + 255) / 256), dim3(256), 0, C->stream, C->g,
                            C->S.din, C->S.gwork);
