@@ -5,7 +5,7 @@ BF16 MFMA GEMMs with FP32 accumulation, hand-written forward, backward and
 AdamW kernels) predicts an initial cell state (h, hu, hv) of a reach from its
 bed elevation, Manning n, and boundary discharge and stage.  The FP64 solver
 then starts from the prediction and runs to the same residual as a cold
-start, so it is both the fallback and the judge of the prediction: a poor
+start, so it both backs up and checks the prediction: a poor
 prediction costs steps, never accuracy.  No counterpart exists in the
 reference, which starts from the user's initial conditions
 (Source_xsy-64bit/dataprepFVT.f90:90-130).

@@ -10,7 +10,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
-# relative L-infinity bound on h, hu, hv after N steps (north-star bound 1e-10)
+# relative L-infinity bound on h, hu, hv after N steps (the parity bound, 1e-10)
 RTOL = 1e-10
 
 
